@@ -1,0 +1,228 @@
+"""Benchmark: Mrealisations/s of the null-collision path tracer on one MI355X.
+
+Workload (default): the precision configuration -- htrdr-atmosphere
+shortwave image, 1024x1024 pixels, 256 realisations per pixel, monochromatic
+550 nm, cloud grid 256x256x64 over 25.6x25.6x3.2 km (base 1 km) with the
+seeded lognormal LWC law masked to 50% cover, octree with 2^3 leaf blocks,
+50 gas layers, Lambertian ground (albedo 0.3), sun at 30 deg zenith,
+pinhole camera 60 deg fov tilted 60 deg down from (12800, -2000, 5000).
+
+A step is one slice of `--spp-per-step` realisations per pixel over the whole
+image; the default 16 steps x 16 spp = 256 spp is the full image. The scene,
+its octree and the per-pixel accumulators are resident in HBM before the
+timed region starts. The timed region is bracketed by device
+synchronisations and every step is timed with HIP events on the stream its
+kernels are launched on.
+
+Printed: one JSON line (rank 0). `value` is whole-job throughput in
+Mrealisations/s; `time_to_0p5pct_s` is the time the solver needs, at the
+measured rate, to bring 99% of the pixels with E > 0 to a 0.5% relative
+standard error (N_req(p) = (sigma_p / (0.005 E_p))^2 from the measured
+per-pixel variance). `roofline` prices the dominant kernel's algorithmic
+bytes (12 B per octree-leaf lookup, 4 B per LWC fetch, per-pixel
+accumulator traffic) against the 8 TB/s HBM peak; `cpu_baseline` is the CPU
+oracle timed on the host cores over a bounded pixel sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mrealisations/s & time-to-0.5%-std-error, FP32 batched vs FP64 plain"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=16)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--config", type=int, default=1)
+    p.add_argument("--solver", choices=("plain", "wavefront"), default="plain")
+    p.add_argument("--spp-per-step", type=int, default=16)
+    p.add_argument("--width", type=int, default=0, help="override the image width/height")
+    p.add_argument("--cpu-seconds", type=float, default=15.0)
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--seed", type=int, default=0)
+    return p.parse_args()
+
+
+def cpu_baseline(synth, desc, target_s):
+    """Oracle (FP64 C/OpenMP restatement) timed on a strided pixel sample."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ffi as O
+    from htrdr_kmin_amd import atmosphere as A
+    threads = min(16, os.cpu_count() or 1)
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    sky = O.Sky(synth.scene_ptr)
+    npix = desc.width * desc.height
+    d = A.map_preset(1)
+    for f, _ in d._fields_:
+        setattr(d, f, getattr(desc, f))
+    d.spp = 2
+    d.spp_offset = 0
+    # calibration on 64 pixels, then a sample sized for about target_s
+    calib = np.arange(0, npix, npix // 64, dtype=np.uint64)[:64]
+    t0 = time.perf_counter()
+    sky.draw_pixels(d, calib, threads)
+    rate = len(calib) * d.spp / max(time.perf_counter() - t0, 1e-6)
+    n = int(min(npix, max(64, rate * target_s / d.spp)))
+    pix = np.linspace(0, npix - 1, n).astype(np.uint64)
+    t0 = time.perf_counter()
+    sky.draw_pixels(d, pix, threads)
+    dt = time.perf_counter() - t0
+    sky.close()
+    return {"value": n * d.spp / dt / 1e6, "unit": "Mrealisations/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{n} pixels evenly strided over the {desc.width}x{desc.height} image x "
+                      f"{d.spp} realisations, FP64 C/OpenMP oracle, {dt:.1f} s"}
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus > 1 or world > 1:
+        sys.exit("bench.py runs on one GPU only: this build has no multi-GPU path "
+                 "(pixels would shard by Morton tiles, see DESIGN.md); use --gpus 1")
+
+    import numpy as np
+    from htrdr_kmin_amd import _ffi as F
+    from htrdr_kmin_amd import atmosphere as A
+    from htrdr_kmin_amd import hip as H
+
+    synth = A.Synth(args.config, seed=args.seed)
+    desc = A.map_preset(args.config)
+    if args.width:
+        desc.width = desc.height = args.width
+    desc.solver = F.SOLVER_WAVEFRONT_FP32 if args.solver == "wavefront" else F.SOLVER_PLAIN_FP64
+    desc.seed = args.seed
+    npix = desc.width * desc.height
+    S = args.spp_per_step
+    ctx = A.Context(synth.scene_ptr)
+    acc = H.DeviceBuffer(npix * A.ACCUM_DTYPE.itemsize)
+    stream = H.Stream()
+
+    def step(k):
+        desc.spp = S
+        desc.spp_offset = k * S
+        ctx.render_accum(desc, acc.address, stream.h.value)
+
+    # Warm-up; the first warm-up step also collects the event counters of the
+    # byte model (a separate, counting build of the kernel).
+    acc.zero()
+    ctx.set_counting(True)
+    t_w = time.perf_counter()
+    step(0)
+    stream.synchronize()
+    counters = ctx.counters()
+    ctx.set_counting(False)
+    print(f"[bench] warm-up step 0 (counting) {time.perf_counter() - t_w:.2f} s", file=sys.stderr,
+          flush=True)
+    for w in range(1, args.warmup):
+        step(w)
+        stream.synchronize()
+        print(f"[bench] warm-up step {w} done", file=sys.stderr, flush=True)
+
+    # Timed region
+    acc.zero()
+    ev = [(H.Event(), H.Event()) for _ in range(args.steps)]
+    H.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ev[k][0].record(stream)
+        step(k)
+        ev[k][1].record(stream)
+        ev[k][1].synchronize()
+        print(f"[bench] step {k}: {ev[k][0].elapsed_ms(ev[k][1]):.1f} ms", file=sys.stderr,
+              flush=True)
+    stream.synchronize()
+    H.synchronize()
+    wall = time.perf_counter() - t0
+    kms = [a.elapsed_ms(b) for a, b in ev]
+    total_s = sum(kms) / 1e3
+
+    real = npix * S * args.steps
+    value = real / total_s / 1e6
+
+    # time to 0.5% relative standard error
+    h = acc.download(np.zeros(npix, A.ACCUM_DTYPE))
+    E, SE = A.accum_estimate(h)
+    n = h["n"].astype(np.float64)
+    pos = E > 0
+    sigma = SE[pos] * np.sqrt(n[pos])
+    nreq = (sigma / (0.005 * E[pos])) ** 2
+    spp_req = float(np.quantile(nreq, 0.99)) if nreq.size else float("nan")
+    t_per_real = total_s / real
+    t05 = spp_req * npix * t_per_real
+
+    # roofline of the dominant kernel (one launch = one step)
+    creal = max(counters["nrealisations"], 1)
+    bytes_leaf = 12.0 * counters["nleaf"] / creal
+    bytes_cell = 4.0 * counters["nlwc"] / creal
+    bytes_pix = 4 * 64.0 / S  # accumulator + slice partials, read and written
+    bytes_real = bytes_leaf + bytes_cell + bytes_pix
+    avg_launch_s = total_s / args.steps
+    achieved = bytes_real * npix * S / avg_launch_s / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{args.solver}_cfg{args.config}.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "Mrealisations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(total_s * 1e3 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.solver == "wavefront" else "f64",
+        "data": "synthetic (seeded scene generator, seed %d)" % args.seed,
+        "config": {
+            "workload": "htrdr-atmosphere SW image %dx%d, %d realisations/px (%d steps x %d), "
+                        "550 nm, cloud grid 256x256x64 over 25.6x25.6x3.2 km, 2^3 leaf blocks"
+                        % (desc.width, desc.height, S * args.steps, args.steps, S),
+            "solver": "FP32 batched wavefront" if args.solver == "wavefront" else "FP64 plain one-path-per-thread",
+            "config_index": args.config,
+            "spp_per_step": S,
+            "wall_s": round(wall, 3),
+        },
+        "time_to_0p5pct_s": round(t05, 3),
+        "spp_required_0p5pct_q99": round(spp_req, 1),
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 3),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 6),
+            "traffic": traffic,
+            "bytes_per_realisation": round(bytes_real, 2),
+            "events_per_realisation": {
+                "leaf_lookups": round(counters["nleaf"] / creal, 2),
+                "segments": round(counters["nseg"] / creal, 2),
+                "collisions": round(counters["ncoll"] / creal, 2),
+                "lwc_fetches": round(counters["nlwc"] / creal, 2),
+                "scatter_surface_events": round(counters["nevents"] / creal, 3),
+            },
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(synth, desc, args.cpu_seconds)
+    stream.destroy()
+    acc.free()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

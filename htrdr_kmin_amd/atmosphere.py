@@ -1,0 +1,259 @@
+"""Host-side mirror of htrdr-atmosphere's draw-map path over libhtgx.so.
+
+This is the Python face of the drop-in boundary: the same steps as
+``atmosphere_draw_map`` (src/atmosphere/htrdr_atmosphere_draw_map.c:593-641)
+-- render the map through ``htgx_draw_map`` (the replacement of
+``htrdr_draw_map``, src/core/htrdr_draw_map.c:497-621), then write the
+htrdr-image text with ``dump_buffer`` (..._draw_map.c:477-587) and log the
+per-path time and, for flux maps, the overall flux. Errors are the library's
+rsys-compatible result codes, raised as ``HtgxError``.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _ffi as F
+
+XWAVE_DTYPE = np.dtype([("radiance_E", "<f8"), ("radiance_SE", "<f8"),
+                        ("temperature_E", "<f8"), ("temperature_SE", "<f8"),
+                        ("time_sum", "<f8"), ("time_sum2", "<f8"),
+                        ("time_n", "<u8"), ("time_nfail", "<u8")])
+FLUX_DTYPE = np.dtype([("flux_sum", "<f8"), ("flux_sum2", "<f8"),
+                       ("flux_n", "<u8"), ("flux_nfail", "<u8"),
+                       ("time_sum", "<f8"), ("time_sum2", "<f8"),
+                       ("time_n", "<u8"), ("time_nfail", "<u8")])
+ACCUM_DTYPE = np.dtype([("sum", "<f8"), ("sum_c", "<f8"), ("sum2", "<f8"),
+                        ("sum2_c", "<f8"), ("tsum", "<f8"), ("tsum2", "<f8"),
+                        ("n", "<u8"), ("nfail", "<u8")])
+assert XWAVE_DTYPE.itemsize == C.sizeof(F.PixelXwave) == 64
+assert FLUX_DTYPE.itemsize == C.sizeof(F.PixelFlux) == 64
+assert ACCUM_DTYPE.itemsize == C.sizeof(F.McAccum) == 64
+
+
+class Synth:
+    """Seeded synthetic scene (benchmark configuration ``config``)."""
+
+    def __init__(self, config=0, seed=0, **overrides):
+        lib = F.load()
+        self.params = F.SynthParams()
+        F.check("htgx_synth_params_preset", lib.htgx_synth_params_preset(config, C.byref(self.params)))
+        self.params.seed = seed
+        for k, v in overrides.items():
+            field = getattr(self.params, k)
+            if isinstance(field, C.Array):
+                for i, x in enumerate(v):
+                    field[i] = x
+            else:
+                setattr(self.params, k, v)
+        self._h = C.c_void_p()
+        F.check("htgx_synth_create", lib.htgx_synth_create(C.byref(self.params), C.byref(self._h)))
+        self.scene_ptr = lib.htgx_synth_get_scene(self._h)
+        self._lib = lib
+
+    @property
+    def scene(self):
+        return self.scene_ptr.contents
+
+    def lwc(self):
+        sc = self.scene.clouds
+        nx, ny, nz = sc.def_[0], sc.def_[1], sc.def_[2]
+        if not nx:
+            return np.zeros((0, 0, 0), np.float32)
+        return np.ctypeslib.as_array(sc.lwc, shape=(nz, ny, nx)).copy()
+
+    def close(self):
+        if self._h:
+            self._lib.htgx_synth_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def map_preset(config, **overrides):
+    """Sensor and spectral settings of benchmark configuration ``config``."""
+    lib = F.load()
+    d = F.MapDesc()
+    F.check("htgx_synth_map_preset", lib.htgx_synth_map_preset(config, C.byref(d)))
+    for k, v in overrides.items():
+        field = getattr(d, k)
+        if isinstance(field, C.Array):
+            for i, x in enumerate(v):
+                field[i] = x
+        else:
+            setattr(d, k, v)
+    return d
+
+
+class Context:
+    """A libhtgx context: the scene resident on the current HIP device."""
+
+    def __init__(self, scene_ptr):
+        self._lib = F.load()
+        self._h = C.c_void_p()
+        F.check("htgx_ctx_create", self._lib.htgx_ctx_create(scene_ptr, C.byref(self._h)))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def svo_info(self):
+        nl, nn, nf = C.c_uint32(), C.c_uint64(), C.c_uint64()
+        F.check("htgx_ctx_get_svo_info",
+                self._lib.htgx_ctx_get_svo_info(self._h, C.byref(nl), C.byref(nn), C.byref(nf)))
+        return nl.value, nn.value, nf.value
+
+    def draw_map(self, desc):
+        """Pixels of the map as a structured array of shape (height, width)."""
+        dt = FLUX_DTYPE if desc.sensor == F.SENSOR_RECTANGLE else XWAVE_DTYPE
+        out = np.zeros((desc.height, desc.width), dt)
+        F.check("htgx_draw_map", self._lib.htgx_draw_map(
+            self._h, C.byref(desc), out.ctypes.data_as(C.c_void_p), desc.width * dt.itemsize))
+        return out
+
+    def render_accum(self, desc, accum_ptr, stream=None):
+        F.check("htgx_render_accum", self._lib.htgx_render_accum(
+            self._h, C.byref(desc), C.c_void_p(accum_ptr), C.c_void_p(stream or 0)))
+
+    def set_counting(self, on):
+        F.check("htgx_set_counting", self._lib.htgx_set_counting(self._h, 1 if on else 0))
+
+    def counters(self):
+        c = F.Counters()
+        F.check("htgx_get_counters", self._lib.htgx_get_counters(self._h, C.byref(c)))
+        return {k: getattr(c, k) for k, _ in F.Counters._fields_}
+
+    def trace_segments(self, rays, prop, iband=0, iquad=0, max_segs=256):
+        rays = np.ascontiguousarray(rays, np.float64).reshape(-1, 8)
+        n = rays.shape[0]
+        segs = (F.Segment * (n * max_segs))()
+        nsegs = np.zeros(n, np.uint32)
+        F.check("htgx_debug_trace_segments", self._lib.htgx_debug_trace_segments(
+            self._h, rays.ctypes.data_as(F.c_double_p), n, prop, iband, iquad, segs, max_segs,
+            nsegs.ctypes.data_as(F.c_uint32_p)))
+        arr = np.ctypeslib.as_array(segs).reshape(n, max_segs)
+        return arr, nsegs
+
+    def realisations(self, desc, pixels, reals):
+        pixels = np.ascontiguousarray(pixels, np.uint64)
+        reals = np.ascontiguousarray(reals, np.uint64)
+        out = np.zeros(pixels.size, np.float64)
+        F.check("htgx_debug_realisations", self._lib.htgx_debug_realisations(
+            self._h, C.byref(desc), pixels.ctypes.data_as(F.c_uint64_p),
+            reals.ctypes.data_as(F.c_uint64_p), pixels.size, out.ctypes.data_as(F.c_double_p)))
+        return out
+
+    def sample_dirs(self, kind, param, dir_in, u):
+        u = np.ascontiguousarray(u, np.float64).reshape(-1, 2)
+        d = np.ascontiguousarray(dir_in, np.float64)
+        out = np.zeros((u.shape[0], 3), np.float64)
+        F.check("htgx_debug_sample_dirs", self._lib.htgx_debug_sample_dirs(
+            self._h, kind, param, d.ctypes.data_as(F.c_double_p), u.ctypes.data_as(F.c_double_p),
+            u.shape[0], out.ctypes.data_as(F.c_double_p)))
+        return out
+
+    def close(self):
+        if self._h:
+            self._lib.htgx_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def threefry(ctr, key):
+    """Threefry-4x64-20 blocks computed on the device."""
+    lib = F.load()
+    ctr = np.ascontiguousarray(ctr, np.uint64).reshape(-1, 4)
+    key = np.ascontiguousarray(key, np.uint64).reshape(-1, 4)
+    out = np.zeros_like(ctr)
+    F.check("htgx_debug_threefry", lib.htgx_debug_threefry(
+        ctr.ctypes.data_as(F.c_uint64_p), key.ctypes.data_as(F.c_uint64_p), ctr.shape[0],
+        out.ctypes.data_as(F.c_uint64_p)))
+    return out
+
+
+def finalize(desc, accum):
+    """Reference pixel structs from a host copy of the device accumulators."""
+    lib = F.load()
+    accum = np.ascontiguousarray(accum, ACCUM_DTYPE)
+    dt = FLUX_DTYPE if desc.sensor == F.SENSOR_RECTANGLE else XWAVE_DTYPE
+    out = np.zeros((desc.height, desc.width), dt)
+    F.check("htgx_finalize_pixels", lib.htgx_finalize_pixels(
+        C.byref(desc), accum.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p),
+        desc.width * dt.itemsize))
+    return out
+
+
+def estimate(s, s2, n):
+    """htrdr_accum_get_estimation (src/core/htrdr_accum.h:48-68), vectorised."""
+    s = np.asarray(s, np.float64)
+    s2 = np.asarray(s2, np.float64)
+    n = np.asarray(n, np.float64)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        E = np.where(n > 0, s / np.maximum(n, 1), 0.0)
+        V = np.maximum(s2 / np.maximum(n, 1) - E * E, 0.0)
+        SE = np.where(n > 0, np.sqrt(V / np.maximum(n, 1)), 0.0)
+    return E, SE
+
+
+def accum_estimate(acc):
+    return estimate(acc["sum"] + acc["sum_c"], acc["sum2"] + acc["sum2_c"], acc["n"])
+
+
+def _fmt_g(x):
+    return "%g" % x
+
+
+def dump_buffer(pixels, desc, stream):
+    """Write an htrdr-image (doc/htrdr-image.5) as dump_buffer does
+    (src/atmosphere/htrdr_atmosphere_draw_map.c:477-587). Returns the
+    per-path time accumulator and, for flux maps, the flux accumulator
+    (sum, sum2, n) summed over the pixels with nweights > 0."""
+    h, w = pixels.shape
+    stream.write("%lu %lu\n" % (w, h))
+    tacc = [0.0, 0.0, 0]
+    facc = [0.0, 0.0, 0]
+
+    def dump_accum(s, s2, n, acc):
+        if n == 0:
+            stream.write("0 0 ")
+            return
+        E, SE = estimate(s, s2, n)
+        stream.write("%s %s " % (_fmt_g(float(E)), _fmt_g(float(SE))))
+        if acc is not None:
+            acc[0] += s
+            acc[1] += s2
+            acc[2] += n
+
+    flux = desc.sensor == F.SENSOR_RECTANGLE
+    for y in range(h):
+        for x in range(w):
+            p = pixels[y, x]
+            if flux:
+                dump_accum(p["flux_sum"], p["flux_sum2"], int(p["flux_n"]), facc)
+                stream.write("0 0 0 0 ")
+            else:
+                # ..._draw_map.c:536-540: "%g %g %f %f 0 0 "; the temperature
+                # columns are written 0 0 for shortwave maps (tile memory is
+                # left uninitialised there in the reference).
+                stream.write("%s %s %f %f 0 0 " % (
+                    _fmt_g(float(p["temperature_E"])), _fmt_g(float(p["temperature_SE"])),
+                    float(p["radiance_E"]), float(p["radiance_SE"])))
+            dump_accum(p["time_sum"], p["time_sum2"], int(p["time_n"]), tacc)
+            stream.write("\n")
+        stream.write("\n")
+    return tacc, facc

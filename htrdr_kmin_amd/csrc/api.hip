@@ -1,0 +1,811 @@
+/*
+ * C-ABI of libhtgx.so: scene upload, plain FP64 solver, finalisation and
+ * debug entry points (declarations and reference citations in htgx.h).
+ */
+#include "htgx.h"
+#include "htgx_device.h"
+#include "htgx_internal.h"
+#include "svo.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+using namespace htgx;
+
+#define HIP_OK(expr) do {                                                     \
+  hipError_t e_ = (expr);                                                    \
+  if(e_ != hipSuccess) {                                                     \
+    fprintf(stderr, "htgx: %s failed: %s (%s:%d)\n", #expr,                  \
+      hipGetErrorString(e_), __FILE__, __LINE__);                            \
+    return e_ == hipErrorOutOfMemory ? HTGX_MEM_ERR : HTGX_UNKNOWN_ERR;       \
+  }                                                                           \
+} while(0)
+
+/*******************************************************************************
+ * Kernels
+ ******************************************************************************/
+namespace {
+
+/* Pixel of item i of a realisation slice: 8x8 tiles (one wavefront per tile,
+ * the tile size of src/core/htrdr_draw_map.c:45), pixels of a tile in Morton
+ * order like draw_tile (src/core/htrdr_draw_map.c:301-319). */
+__device__ __forceinline__ void
+tile_pixel(uint32_t i, uint32_t ntx, uint32_t* x, uint32_t* y)
+{
+  const uint32_t tile = i >> 6, m = i & 63u;
+  *x = (tile % ntx) * 8u + ((m & 1u) | ((m >> 1) & 2u) | ((m >> 2) & 4u));
+  *y = (tile / ntx) * 8u + (((m >> 1) & 1u) | ((m >> 2) & 2u) | ((m >> 3) & 4u));
+}
+
+__device__ __forceinline__ uint64_t
+wave_sum(uint64_t v)
+{
+  for(int off = 32; off > 0; off >>= 1)
+    v += (uint64_t)__shfl_xor((unsigned long long)v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ void
+flush_counters(const Counters& c, unsigned long long* counters)
+{
+  const uint64_t a = wave_sum(c.nleaf), b = wave_sum(c.nseg), d = wave_sum(c.ncoll),
+                 e = wave_sum(c.ncell), f = wave_sum(c.nev);
+  if((threadIdx.x & 63u) == 0) {
+    atomicAdd(counters + 1, (unsigned long long)a);
+    atomicAdd(counters + 2, (unsigned long long)b);
+    atomicAdd(counters + 3, (unsigned long long)d);
+    atomicAdd(counters + 4, (unsigned long long)e);
+    atomicAdd(counters + 5, (unsigned long long)f);
+  }
+}
+
+/* One thread = one pixel x one slice of its realisations, one path at a
+ * time, FP64 throughout: the "plain" solver and the FP64 GPU oracle. It is
+ * the device counterpart of draw_pixel_xwave / draw_pixel_flux
+ * (src/atmosphere/htrdr_atmosphere_draw_map.c:210-436) driven by the tile
+ * loop of draw_map (src/core/htrdr_draw_map.c:324-491). */
+template<bool COUNT>
+__global__ void __launch_bounds__(256)
+plain_kernel(DevScene S, DevMap M, uint32_t ntx, uint32_t items_per_slice,
+             uint32_t nslices, uint32_t spt, htgx_mc_accum* partials,
+             unsigned long long* counters)
+{
+  const uint64_t item = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  Counters cnt = {0, 0, 0, 0, 0};
+  uint32_t x = 0, y = 0, slice = 0;
+  bool active = item < (uint64_t)items_per_slice * nslices;
+  if(active) {
+    slice = (uint32_t)(item / items_per_slice);
+    tile_pixel((uint32_t)(item % items_per_slice), ntx, &x, &y);
+    active = x < M.width && y < M.height;
+  }
+  if(active) {
+    const uint64_t npix = (uint64_t)M.width * M.height;
+    const uint64_t pixel = (uint64_t)y * M.width + x;
+    const uint32_t rb = slice * spt;
+    const uint32_t re = min(rb + spt, M.spp);
+    Counters* pc = COUNT ? &cnt : nullptr;
+    htgx_mc_accum a;
+    a.sum = a.sum_c = a.sum2 = a.sum2_c = a.tsum = a.tsum2 = 0;
+    a.n = a.nfail = 0;
+    Path P;
+    Stream rng;
+    TrackOut out;
+    uint32_t r = rb;
+    uint64_t t0 = wall_clock64();
+    real_begin(S, M, P, rng, pixel, (uint64_t)M.spp_offset + r);
+    /* Flat loop: a lane whose realisation ends starts its next one at once
+     * instead of waiting at a per-realisation barrier for the other lanes. */
+    while(r < re) {
+      while(P.stage == ST_WALK_DONE) walk_done(S, M, P);
+      if(P.stage == ST_REAL_DONE) {
+        const uint64_t t1 = wall_clock64();
+        if(P.rejected) {
+          a.nfail++;
+        } else {
+          const double us = (double)(t1 - t0) * M.us_per_tick;
+          nsum(a.sum, a.sum_c, P.weight);
+          nsum(a.sum2, a.sum2_c, P.weight * P.weight);
+          a.tsum += us;
+          a.tsum2 += us * us;
+          a.n++;
+        }
+        if(++r >= re) break;
+        t0 = t1;
+        real_begin(S, M, P, rng, pixel, (uint64_t)M.spp_offset + r);
+        continue;
+      }
+      int mode, prop;
+      double o[3], d[3], r1;
+      stage_query(P, &mode, &prop, o, d, &r1);
+      track(S, rng, mode, prop, P.iband, P.iquad, o, d, r1, out, pc);
+      stage_consume(S, P, rng, out, pc);
+    }
+    partials[(size_t)slice * npix + pixel] = a;
+  }
+  if(COUNT) flush_counters(cnt, counters);
+}
+
+/* Fold the slices of each pixel, in slice order, into its accumulator */
+__global__ void __launch_bounds__(256)
+merge_kernel(const htgx_mc_accum* partials, uint32_t nslices, uint64_t npix,
+             htgx_mc_accum* accum)
+{
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(p >= npix) return;
+  htgx_mc_accum a = accum[p];
+  for(uint32_t s = 0; s < nslices; ++s) {
+    const htgx_mc_accum q = partials[(size_t)s * npix + p];
+    nsum(a.sum, a.sum_c, q.sum);
+    a.sum_c += q.sum_c;
+    nsum(a.sum2, a.sum2_c, q.sum2);
+    a.sum2_c += q.sum2_c;
+    a.tsum += q.tsum;
+    a.tsum2 += q.tsum2;
+    a.n += q.n;
+    a.nfail += q.nfail;
+  }
+  accum[p] = a;
+}
+
+__global__ void
+trace_segments_kernel(DevScene S, const double* rays, uint32_t nrays, int prop,
+                      uint32_t iband, uint32_t iquad, htgx_segment* segs,
+                      uint32_t max_segs, uint32_t* nsegs)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= nrays) return;
+  const double* r = rays + (size_t)i * 8;
+  SegIter it;
+  Segment seg;
+  uint32_t n = 0;
+  it.init(S, r, r + 3, r[6], r[7], nullptr);
+  while(it.next(S, prop, iband, iquad, seg, nullptr)) {
+    if(n < max_segs) {
+      htgx_segment& o = segs[(size_t)i * max_segs + n];
+      o.t0 = seg.t0;
+      o.t1 = seg.t1;
+      o.kmin = seg.kmin;
+      o.kmax = seg.kmax;
+      o.leaf = seg.in_cloud ? (int64_t)seg.leaf : -1;
+      o.layer = seg.layer;
+      o.pad = 0;
+    }
+    n++;
+  }
+  nsegs[i] = n;
+}
+
+__global__ void __launch_bounds__(64)
+realisations_kernel(DevScene S, DevMap M, const uint64_t* pix, const uint64_t* real,
+                    uint32_t n, double* out)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  Path P;
+  Stream rng;
+  TrackOut o;
+  real_begin(S, M, P, rng, pix[i], real[i]);
+  run_until_done(S, M, P, rng, o, nullptr);
+  out[i] = P.rejected ? __longlong_as_double(0x7ff8000000000000LL) : P.weight;
+}
+
+__global__ void
+threefry_kernel(const uint64_t* ctr, const uint64_t* key, uint32_t n, uint64_t* out)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  threefry4x64_20(ctr + 4 * (size_t)i, key + 4 * (size_t)i, out + 4 * (size_t)i);
+}
+
+__global__ void
+sample_dirs_kernel(DevScene S, int kind, double param, double dx, double dy, double dz,
+                   const double* u, uint32_t n, double* out)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  const double din[3] = {dx, dy, dz};
+  const double u0 = u[2 * (size_t)i], u1 = u[2 * (size_t)i + 1];
+  double* o = out + 3 * (size_t)i;
+  switch(kind) {
+    case 0: local_to_world(din, hg_cos_sample(param, u0), 2.0 * kPi * u1, o); break;
+    case 1: local_to_world(din, rayleigh_cos_sample(u0), 2.0 * kPi * u1, o); break;
+    case 2: {
+      const double cos_t = 1.0 - u0 * (1.0 - S.sun_cos_half);
+      local_to_world(S.sun_dir, cos_t, 2.0 * kPi * u1, o);
+    } break;
+    default: cos_hemisphere(din, u0, u1, o); break;
+  }
+}
+
+template<class T>
+int upload(htgx_ctx* ctx, const T* host, size_t n, const T** out)
+{
+  void* d = nullptr;
+  if(!n) { *out = nullptr; return HTGX_OK; }
+  HIP_OK(hipMalloc(&d, n * sizeof(T)));
+  ctx->bufs.push_back(d);
+  HIP_OK(hipMemcpy(d, host, n * sizeof(T), hipMemcpyHostToDevice));
+  *out = (const T*)d;
+  return HTGX_OK;
+}
+
+} /* namespace */
+
+/*******************************************************************************
+ * Context
+ ******************************************************************************/
+static int
+check_scene(const htgx_scene_desc* s)
+{
+  if(!s) return HTGX_BAD_ARG;
+  const htgx_spectral_desc& sp = s->spectral;
+  if(!sp.nbands || !sp.band_lo || !sp.band_hi || !sp.nquads || !sp.quad_weights)
+    return HTGX_BAD_ARG;
+  for(uint32_t b = 0; b < sp.nbands; ++b) {
+    if(!sp.nquads[b] || !(sp.band_hi[b] >= sp.band_lo[b])) return HTGX_BAD_ARG;
+    if(b && sp.band_lo[b] < sp.band_hi[b - 1]) return HTGX_BAD_ARG;
+  }
+  if(!s->gas.nlayers || !(s->gas.layer_thickness > 0) || !s->gas.ka || !s->gas.ks)
+    return HTGX_BAD_ARG;
+  const htgx_cloud_desc& c = s->clouds;
+  const bool clouds = c.def[0] || c.def[1] || c.def[2];
+  if(clouds && (!c.def[0] || !c.def[1] || !c.def[2] || !c.lwc || !c.cext || !c.ssa || !c.hg_g))
+    return HTGX_BAD_ARG;
+  if(s->ground.type == HTGX_GROUND_PLANE) {
+    if(!s->ground.nmtls || s->ground.plane_mtl >= s->ground.nmtls || !s->ground.mtl_albedo)
+      return HTGX_BAD_ARG;
+  } else if(s->ground.type != HTGX_GROUND_NONE) {
+    return HTGX_BAD_ARG; /* meshes: see htgx_ctx_create */
+  }
+  if(!(s->optical_thickness >= 0)) return HTGX_BAD_ARG;
+  return HTGX_OK;
+}
+
+extern "C" int
+htgx_ctx_create(const htgx_scene_desc* scene, htgx_ctx** out)
+{
+  int res;
+  if(!out) return HTGX_BAD_ARG;
+  if((res = check_scene(scene)) != HTGX_OK) return res;
+  htgx_ctx* ctx = new(std::nothrow) htgx_ctx();
+  if(!ctx) return HTGX_MEM_ERR;
+  *out = nullptr;
+  res = htgx_ctx_init(ctx, *scene);
+  if(res != HTGX_OK) { htgx_ctx_destroy(ctx); return res; }
+  *out = ctx;
+  return HTGX_OK;
+}
+
+int
+htgx_ctx_init(htgx_ctx* ctx, const htgx_scene_desc& sc)
+{
+  int res;
+  DevScene& S = ctx->S;
+  std::memset(&S, 0, sizeof(S));
+  HIP_OK(hipGetDevice(&ctx->device));
+  {
+    int rate_khz = 0;
+    HIP_OK(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, ctx->device));
+    ctx->us_per_tick = rate_khz > 0 ? 1000.0 / (double)rate_khz : 0.01;
+  }
+
+  /* Spectral tables */
+  const uint32_t nb = sc.spectral.nbands;
+  std::vector<uint32_t> qoff(nb + 1);
+  uint32_t nq = 0;
+  for(uint32_t b = 0; b < nb; ++b) { qoff[b] = nq; nq += sc.spectral.nquads[b]; }
+  qoff[nb] = nq;
+  std::vector<double> qcdf(nq);
+  for(uint32_t b = 0; b < nb; ++b) {
+    double acc = 0;
+    for(uint32_t q = 0; q < sc.spectral.nquads[b]; ++q) {
+      acc += sc.spectral.quad_weights[qoff[b] + q];
+      qcdf[qoff[b] + q] = acc;
+    }
+    qcdf[qoff[b] + sc.spectral.nquads[b] - 1] = 1.0;
+  }
+  ctx->band_lo.assign(sc.spectral.band_lo, sc.spectral.band_lo + nb);
+  ctx->band_hi.assign(sc.spectral.band_hi, sc.spectral.band_hi + nb);
+  S.nbands = nb;
+  if((res = upload(ctx, sc.spectral.band_hi, nb, &S.band_hi))) return res;
+  if((res = upload(ctx, qoff.data(), qoff.size(), &S.quad_off))) return res;
+  if((res = upload(ctx, qcdf.data(), qcdf.size(), &S.quad_cdf))) return res;
+
+  /* Gas */
+  const uint32_t nl = sc.gas.nlayers;
+  S.nlayers = nl;
+  S.zb = sc.gas.z_bottom;
+  S.dzl = sc.gas.layer_thickness;
+  S.inv_dzl = 1.0 / sc.gas.layer_thickness;
+  if((res = upload(ctx, sc.gas.ka, (size_t)nq * nl, &S.gas_ka))) return res;
+  if((res = upload(ctx, sc.gas.ks, (size_t)nb * nl, &S.gas_ks))) return res;
+  if(sc.gas.temperature) {
+    if((res = upload(ctx, sc.gas.temperature, nl, &S.gas_T))) return res;
+    ctx->has_temperature = true;
+  }
+
+  /* Clouds + octree */
+  const htgx_cloud_desc& cl = sc.clouds;
+  if(cl.def[0] && cl.def[1] && cl.def[2]) {
+    SvoHost svo;
+    if((res = svo_build(sc, svo)) != HTGX_OK) return res;
+    S.has_clouds = 1;
+    S.B = svo.B;
+    for(int a = 0; a < 3; ++a) {
+      S.def[a] = cl.def[a];
+      S.dims0[a] = svo.dims[0][a];
+      S.lower[a] = cl.lower[a];
+      S.bs[a] = svo.bs[a];
+      S.inv_bs[a] = svo.inv_bs[a];
+      S.inv_cs[a] = svo.inv_cs[a];
+    }
+    const size_t ncells = (size_t)cl.def[0] * cl.def[1] * cl.def[2];
+    if((res = upload(ctx, cl.lwc, ncells, &S.lwc))) return res;
+    if((res = upload(ctx, svo.lmap.data(), svo.lmap.size(), &S.lmap))) return res;
+    if((res = upload(ctx, svo.bounds.data(), svo.bounds.size(), &S.bounds))) return res;
+    if((res = upload(ctx, cl.cext, nb, &S.cext))) return res;
+    if((res = upload(ctx, cl.ssa, nb, &S.ssa))) return res;
+    if((res = upload(ctx, cl.hg_g, nb, &S.hg_g))) return res;
+    ctx->nlevels = svo.nlevels;
+    ctx->nnodes = svo.nnodes;
+    ctx->nleaves = svo.nleaves;
+    ctx->svo_bytes = svo.lmap.size() * sizeof(uint32_t) + svo.bounds.size() * sizeof(float2);
+  }
+
+  /* Ground */
+  S.ground_type = sc.ground.type;
+  if(sc.ground.type == HTGX_GROUND_PLANE) {
+    S.plane_z = sc.ground.plane_z;
+    S.plane_mtl = sc.ground.plane_mtl;
+    if((res = upload(ctx, sc.ground.mtl_albedo, sc.ground.nmtls, &S.mtl_albedo))) return res;
+    if(sc.ground.mtl_temperature)
+      if((res = upload(ctx, sc.ground.mtl_temperature, sc.ground.nmtls, &S.mtl_T))) return res;
+  }
+
+  /* Sun: src/atmosphere/htrdr_atmosphere.c:89-111 + ..._sun.c:84-87 */
+  {
+    const double az = sc.sun_azimuth * kPi / 180.0;
+    const double el = sc.sun_elevation * kPi / 180.0;
+    S.sun_dir[0] = std::cos(el) * std::cos(az);
+    S.sun_dir[1] = std::cos(el) * std::sin(az);
+    S.sun_dir[2] = std::sin(el);
+    S.sun_cos_half = std::cos(4.6524e-3);
+    S.sun_solid_angle = 2.0 * kPi * (1.0 - S.sun_cos_half);
+  }
+
+  HIP_OK(hipMalloc(&ctx->d_counters, 8 * sizeof(unsigned long long)));
+  HIP_OK(hipMemset(ctx->d_counters, 0, 8 * sizeof(unsigned long long)));
+  return HTGX_OK;
+}
+
+extern "C" int
+htgx_ctx_destroy(htgx_ctx* ctx)
+{
+  if(!ctx) return HTGX_BAD_ARG;
+  for(void* p : ctx->bufs) (void)hipFree(p);
+  if(ctx->d_partials) (void)hipFree(ctx->d_partials);
+  if(ctx->d_wcdf) (void)hipFree(ctx->d_wcdf);
+  if(ctx->d_wpdf) (void)hipFree(ctx->d_wpdf);
+  if(ctx->d_counters) (void)hipFree(ctx->d_counters);
+  htgx_wavefront_release(ctx);
+  delete ctx;
+  return HTGX_OK;
+}
+
+extern "C" int
+htgx_ctx_get_svo_info(const htgx_ctx* ctx, uint32_t* nlevels, uint64_t* nnodes,
+                      uint64_t* nleaves)
+{
+  if(!ctx) return HTGX_BAD_ARG;
+  if(nlevels) *nlevels = ctx->nlevels;
+  if(nnodes) *nnodes = ctx->nnodes;
+  if(nleaves) *nleaves = ctx->nleaves;
+  return HTGX_OK;
+}
+
+/*******************************************************************************
+ * Map setup
+ ******************************************************************************/
+static void cross(double r[3], const double a[3], const double b[3])
+{
+  const double t0 = a[1]*b[2] - a[2]*b[1];
+  const double t1 = a[2]*b[0] - a[0]*b[2];
+  const double t2 = a[0]*b[1] - a[1]*b[0];
+  r[0] = t0; r[1] = t1; r[2] = t2;
+}
+
+static double normalize(double r[3], const double a[3])
+{
+  const double len = std::sqrt(a[0]*a[0] + a[1]*a[1] + a[2]*a[2]);
+  r[0] = a[0] / len; r[1] = a[1] / len; r[2] = a[2] / len;
+  return len;
+}
+
+int
+htgx_map_setup(htgx_ctx* ctx, const htgx_map_desc* d, DevMap* M)
+{
+  if(!ctx || !d || !M) return HTGX_BAD_ARG;
+  /* check_draw_map_args, src/core/htrdr_draw_map.c:66-79 */
+  if(!d->width || !d->height || !d->spp) return HTGX_BAD_ARG;
+  if(d->width > 16384 || d->height > 16384) return HTGX_BAD_ARG; /* htrdr_args.c:61 */
+  if(d->sensor != HTGX_SENSOR_CAMERA && d->sensor != HTGX_SENSOR_RECTANGLE) return HTGX_BAD_ARG;
+  if(d->spectral_type != HTGX_SPECTRAL_SW && d->spectral_type != HTGX_SPECTRAL_LW)
+    return HTGX_BAD_ARG;
+  if(d->spectral_type == HTGX_SPECTRAL_LW && !ctx->has_temperature) return HTGX_BAD_ARG;
+  if(!(d->wlen_range[1] >= d->wlen_range[0])) return HTGX_BAD_ARG;
+  if(d->solver != HTGX_SOLVER_PLAIN_FP64 && d->solver != HTGX_SOLVER_WAVEFRONT_FP32)
+    return HTGX_BAD_ARG;
+
+  std::memset(M, 0, sizeof(*M));
+  M->width = d->width;
+  M->height = d->height;
+  M->spp = d->spp;
+  M->spp_offset = d->spp_offset;
+  M->seed = d->seed;
+  M->sensor = d->sensor;
+  M->spectral_type = d->spectral_type;
+  M->us_per_tick = ctx->us_per_tick;
+
+  double t[3];
+  for(int a = 0; a < 3; ++a) { M->pos[a] = d->pos[a]; t[a] = d->tgt[a] - d->pos[a]; }
+  if(!(normalize(M->z, t) > 0)) return HTGX_BAD_ARG;
+  cross(t, M->z, d->up);
+  if(!(normalize(M->x, t) > 0)) return HTGX_BAD_ARG;
+  if(d->sensor == HTGX_SENSOR_CAMERA) {
+    /* pinhole camera; fov in ]0, 180[ (htrdr_args.c:77-83) */
+    if(!(d->fov_y > 0 && d->fov_y < 180)) return HTGX_BAD_ARG;
+    cross(M->y, M->x, M->z);
+    M->tan_half = std::tan(d->fov_y * kPi / 180.0 * 0.5);
+    M->aspect = (double)d->width / (double)d->height;
+  } else {
+    /* src/core/htrdr_rectangle.c:93-131 */
+    if(!(d->rect_size[0] > 0 && d->rect_size[1] > 0)) return HTGX_BAD_ARG;
+    cross(t, M->z, M->x);
+    if(!(normalize(M->y, t) > 0)) return HTGX_BAD_ARG;
+    for(int a = 0; a < 3; ++a) {
+      M->axis_x[a] = M->x[a] * d->rect_size[0] * 0.5;
+      M->axis_y[a] = M->y[a] * d->rect_size[1] * 0.5;
+    }
+  }
+
+  /* Wavelength sampler: htrdr_atmosphere_create, src/atmosphere/
+   * htrdr_atmosphere.c:384-415, clamping the range to the sky's bands */
+  double range[2] = {d->wlen_range[0], d->wlen_range[1]};
+  range[0] = std::max(range[0], ctx->band_lo.front());
+  range[1] = std::min(range[1], ctx->band_hi.back());
+  if(!(range[1] >= range[0])) return HTGX_BAD_ARG;
+  double Tref = d->ref_temperature;
+  if(Tref <= 0) Tref = d->spectral_type == HTGX_SPECTRAL_LW ? 290.0 : 5778.0;
+  const size_t nwb = (size_t)std::rint(range[1] - range[0]);
+  M->range[0] = range[0];
+  M->range[1] = range[1];
+  M->Tref = Tref;
+  if(nwb) {
+    M->wmode = 1;
+    const bool cached = ctx->d_wcdf && ctx->w_nbands == nwb && ctx->w_range[0] == range[0]
+      && ctx->w_range[1] == range[1] && ctx->w_Tref == Tref;
+    M->band_len = (range[1] - range[0]) / (double)nwb;
+    if(!cached) {
+      /* setup_wlen_planck_ran_cdf, src/core/htrdr_ran_wlen_planck.c:55-134 */
+      std::vector<double> pdf(nwb), cdf(nwb);
+      double sum = 0;
+      for(size_t i = 0; i < nwb; ++i) {
+        const double lo = range[0] + (double)i * M->band_len;
+        const double hi = std::min(lo + M->band_len, range[1]);
+        pdf[i] = blackbody_fraction(lo * 1e-9, hi * 1e-9, Tref);
+        sum += pdf[i];
+      }
+      for(size_t i = 0; i < nwb; ++i) {
+        pdf[i] /= sum;
+        cdf[i] = i ? pdf[i] + cdf[i - 1] : pdf[i];
+      }
+      cdf[nwb - 1] = 1.0;
+      if(ctx->d_wcdf) { (void)hipFree(ctx->d_wcdf); ctx->d_wcdf = nullptr; }
+      if(ctx->d_wpdf) { (void)hipFree(ctx->d_wpdf); ctx->d_wpdf = nullptr; }
+      HIP_OK(hipMalloc(&ctx->d_wcdf, nwb * sizeof(double)));
+      HIP_OK(hipMalloc(&ctx->d_wpdf, nwb * sizeof(double)));
+      HIP_OK(hipMemcpy(ctx->d_wcdf, cdf.data(), nwb * sizeof(double), hipMemcpyHostToDevice));
+      HIP_OK(hipMemcpy(ctx->d_wpdf, pdf.data(), nwb * sizeof(double), hipMemcpyHostToDevice));
+      ctx->w_nbands = nwb;
+      ctx->w_range[0] = range[0];
+      ctx->w_range[1] = range[1];
+      ctx->w_Tref = Tref;
+    }
+    M->nwbands = (uint32_t)nwb;
+    M->wcdf = ctx->d_wcdf;
+    M->wpdf = ctx->d_wpdf;
+  } else if(std::fabs(range[0] - range[1]) <= 1e-6) {
+    M->wmode = 0;
+  } else {
+    M->wmode = 2;
+  }
+  return HTGX_OK;
+}
+
+/*******************************************************************************
+ * Rendering
+ ******************************************************************************/
+static int
+render_plain(htgx_ctx* ctx, const DevMap& M, htgx_mc_accum* accum, hipStream_t stream)
+{
+  const uint64_t npix = (uint64_t)M.width * M.height;
+  const uint32_t ntx = (M.width + 7) / 8, nty = (M.height + 7) / 8;
+  const uint64_t items_per_slice = (uint64_t)ntx * nty * 64;
+  /* Enough threads to fill the 256 CUs several times over, each running as
+   * many consecutive realisations of its pixel as possible. */
+  const uint64_t target = 1ull << 21;
+  uint32_t nslices = (uint32_t)std::min<uint64_t>(M.spp,
+    std::max<uint64_t>(1, (target + items_per_slice - 1) / items_per_slice));
+  const uint32_t spt = (M.spp + nslices - 1) / nslices;
+  nslices = (M.spp + spt - 1) / spt;
+  if(items_per_slice * nslices > 0xFFFFFFFFull) return HTGX_BAD_ARG;
+
+  const size_t need = (size_t)nslices * npix * sizeof(htgx_mc_accum);
+  if(need > ctx->partials_cap) {
+    if(ctx->d_partials) { HIP_OK(hipFree(ctx->d_partials)); ctx->d_partials = nullptr; }
+    HIP_OK(hipMalloc(&ctx->d_partials, need));
+    ctx->partials_cap = need;
+  }
+  const uint64_t nthreads = items_per_slice * nslices;
+  const uint32_t nblocks = (uint32_t)((nthreads + 255) / 256);
+  if(ctx->counting) {
+    HIP_OK(hipMemsetAsync(ctx->d_counters, 0, 8 * sizeof(unsigned long long), stream));
+    plain_kernel<true><<<nblocks, 256, 0, stream>>>(ctx->S, M, ntx, (uint32_t)items_per_slice,
+      nslices, spt, (htgx_mc_accum*)ctx->d_partials, ctx->d_counters);
+  } else {
+    plain_kernel<false><<<nblocks, 256, 0, stream>>>(ctx->S, M, ntx, (uint32_t)items_per_slice,
+      nslices, spt, (htgx_mc_accum*)ctx->d_partials, ctx->d_counters);
+  }
+  HIP_OK(hipGetLastError());
+  merge_kernel<<<(uint32_t)((npix + 255) / 256), 256, 0, stream>>>(
+    (const htgx_mc_accum*)ctx->d_partials, nslices, npix, accum);
+  HIP_OK(hipGetLastError());
+  ctx->last_nrealisations = npix * M.spp;
+  return HTGX_OK;
+}
+
+extern "C" int
+htgx_render_accum(htgx_ctx* ctx, const htgx_map_desc* desc, htgx_mc_accum* accum, void* stream)
+{
+  DevMap M;
+  int res;
+  if(!ctx || !desc || !accum) return HTGX_BAD_ARG;
+  if((res = htgx_map_setup(ctx, desc, &M)) != HTGX_OK) return res;
+  if(desc->solver == HTGX_SOLVER_WAVEFRONT_FP32)
+    return htgx_render_wavefront(ctx, M, accum, (hipStream_t)stream);
+  return render_plain(ctx, M, accum, (hipStream_t)stream);
+}
+
+extern "C" int
+htgx_set_counting(htgx_ctx* ctx, int enable)
+{
+  if(!ctx) return HTGX_BAD_ARG;
+  ctx->counting = enable ? 1 : 0;
+  return HTGX_OK;
+}
+
+extern "C" int
+htgx_get_counters(htgx_ctx* ctx, htgx_counters* c)
+{
+  if(!ctx || !c) return HTGX_BAD_ARG;
+  unsigned long long h[8];
+  HIP_OK(hipDeviceSynchronize());
+  HIP_OK(hipMemcpy(h, ctx->d_counters, sizeof(h), hipMemcpyDeviceToHost));
+  c->nrealisations = ctx->last_nrealisations;
+  c->nleaf = h[1];
+  c->ncoll = h[3];
+  c->nbvh = 0;
+  c->ntri = 0;
+  c->nevents = h[5];
+  c->nseg = h[2];
+  c->nlwc = h[4];
+  return HTGX_OK;
+}
+
+/* Estimate of an accumulator (src/core/htrdr_accum.h:48-68) */
+static void
+estimate(double S, double S2, uint64_t n, double* E, double* SE)
+{
+  if(!n) { *E = 0; *SE = 0; return; }
+  const double N = (double)n;
+  const double e = S / N;
+  const double V = std::max(S2 / N - e * e, 0.0);
+  *E = e;
+  *SE = std::sqrt(V / N);
+}
+
+/* htrdr_brightness_temperature / htrdr_radiance_temperature,
+ * src/core/htrdr_spectral.c:31-121 */
+static double
+radiance_temperature(double lmin, double lmax, double radiance)
+{
+  const size_t MAX_ITER = 100;
+  double avg = radiance;
+  if(lmin != lmax) avg /= (lmax - lmin);
+  const double eps_T = 1e-4, eps_B = avg * 1e-8;
+  double T = 0, T0 = 0, T1 = 0, T2 = 200, B, B0 = 0;
+  size_t i;
+  for(i = 0; i < MAX_ITER; ++i) {
+    if(planck_range(lmin, lmax, T2) >= avg) break;
+    T2 *= 2;
+  }
+  if(i >= MAX_ITER) return 0;
+  for(i = 0; i < MAX_ITER; ++i) {
+    T = (T1 + T2) * 0.5;
+    B = planck_range(lmin, lmax, T);
+    if(B < avg) T1 = T; else T2 = T;
+    if(std::fabs(T - T0) < eps_T || std::fabs(B - B0) < eps_B) break;
+    T0 = T;
+    B0 = B;
+  }
+  if(i >= MAX_ITER) return 0;
+  return T;
+}
+
+extern "C" int
+htgx_finalize_pixels(const htgx_map_desc* d, const htgx_mc_accum* acc, void* pixels, size_t pitch)
+{
+  if(!d || !acc || !pixels || !d->width || !d->height) return HTGX_BAD_ARG;
+  const size_t elmt = d->sensor == HTGX_SENSOR_RECTANGLE
+    ? sizeof(htgx_pixel_flux) : sizeof(htgx_pixel_xwave);
+  if(pitch < elmt * d->width) return HTGX_BAD_ARG;
+  for(uint32_t y = 0; y < d->height; ++y) {
+    char* row = (char*)pixels + (size_t)y * pitch;
+    for(uint32_t x = 0; x < d->width; ++x) {
+      const htgx_mc_accum& a = acc[(size_t)y * d->width + x];
+      const double S = a.sum + a.sum_c, S2 = a.sum2 + a.sum2_c;
+      htgx_accum time;
+      time.sum_weights = a.tsum;
+      time.sum_weights_sqr = a.tsum2;
+      time.nweights = (size_t)a.n;
+      time.nfailures = 0;
+      if(d->sensor == HTGX_SENSOR_RECTANGLE) {
+        htgx_pixel_flux* p = (htgx_pixel_flux*)(row + x * elmt);
+        p->flux.sum_weights = S;
+        p->flux.sum_weights_sqr = S2;
+        p->flux.nweights = (size_t)a.n;
+        p->flux.nfailures = 0;
+        p->time = time;
+      } else {
+        htgx_pixel_xwave* p = (htgx_pixel_xwave*)(row + x * elmt);
+        estimate(S, S2, a.n, &p->radiance.E, &p->radiance.SE);
+        p->radiance_temperature.E = 0;
+        p->radiance_temperature.SE = 0;
+        if(d->spectral_type == HTGX_SPECTRAL_LW) {
+          /* draw_pixel_xwave, src/atmosphere/htrdr_atmosphere_draw_map.c:425-435 */
+          const double l0 = d->wlen_range[0] * 1e-9, l1 = d->wlen_range[1] * 1e-9;
+          p->radiance_temperature.E = radiance_temperature(l0, l1, p->radiance.E);
+          const double tmin = radiance_temperature(l0, l1, p->radiance.E - p->radiance.SE);
+          const double tmax = radiance_temperature(l0, l1, p->radiance.E + p->radiance.SE);
+          p->radiance_temperature.SE = tmax - tmin;
+        }
+        p->time = time;
+      }
+    }
+  }
+  return HTGX_OK;
+}
+
+extern "C" int
+htgx_draw_map(htgx_ctx* ctx, const htgx_map_desc* desc, void* pixels, size_t pitch)
+{
+  if(!ctx || !desc || !pixels) return HTGX_BAD_ARG;
+  if(!desc->width || !desc->height || !desc->spp) return HTGX_BAD_ARG;
+  const size_t elmt = desc->sensor == HTGX_SENSOR_RECTANGLE
+    ? sizeof(htgx_pixel_flux) : sizeof(htgx_pixel_xwave);
+  if(pitch < elmt * desc->width) return HTGX_BAD_ARG;
+  const size_t npix = (size_t)desc->width * desc->height;
+  htgx_mc_accum* d_acc = nullptr;
+  HIP_OK(hipMalloc(&d_acc, npix * sizeof(htgx_mc_accum)));
+  int res = HTGX_OK;
+  std::vector<htgx_mc_accum> h(npix);
+  if(hipMemset(d_acc, 0, npix * sizeof(htgx_mc_accum)) != hipSuccess) res = HTGX_UNKNOWN_ERR;
+  if(res == HTGX_OK) res = htgx_render_accum(ctx, desc, d_acc, nullptr);
+  if(res == HTGX_OK && hipMemcpy(h.data(), d_acc, npix * sizeof(htgx_mc_accum),
+                                 hipMemcpyDeviceToHost) != hipSuccess)
+    res = HTGX_UNKNOWN_ERR;
+  (void)hipFree(d_acc);
+  if(res == HTGX_OK) res = htgx_finalize_pixels(desc, h.data(), pixels, pitch);
+  return res;
+}
+
+/*******************************************************************************
+ * Debug entry points
+ ******************************************************************************/
+template<class T>
+static int dalloc(T** p, size_t n)
+{
+  HIP_OK(hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T)));
+  return HTGX_OK;
+}
+
+extern "C" int
+htgx_debug_trace_segments(htgx_ctx* ctx, const double* rays, uint32_t nrays, int prop,
+                          uint32_t iband, uint32_t iquad, htgx_segment* segs,
+                          uint32_t max_segs, uint32_t* nsegs)
+{
+  if(!ctx || !rays || !segs || !nsegs || !nrays || !max_segs) return HTGX_BAD_ARG;
+  if(prop < HTGX_Ka || prop > HTGX_Kext || iband >= ctx->S.nbands) return HTGX_BAD_ARG;
+  double* d_rays; htgx_segment* d_segs; uint32_t* d_n;
+  int res;
+  if((res = dalloc(&d_rays, (size_t)nrays * 8))) return res;
+  if((res = dalloc(&d_segs, (size_t)nrays * max_segs))) return res;
+  if((res = dalloc(&d_n, nrays))) return res;
+  HIP_OK(hipMemcpy(d_rays, rays, (size_t)nrays * 8 * sizeof(double), hipMemcpyHostToDevice));
+  trace_segments_kernel<<<(nrays + 127) / 128, 128>>>(ctx->S, d_rays, nrays, prop, iband,
+                                                       iquad, d_segs, max_segs, d_n);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpy(segs, d_segs, (size_t)nrays * max_segs * sizeof(htgx_segment),
+                   hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(nsegs, d_n, nrays * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  HIP_OK(hipFree(d_rays)); HIP_OK(hipFree(d_segs)); HIP_OK(hipFree(d_n));
+  return HTGX_OK;
+}
+
+extern "C" int
+htgx_debug_realisations(htgx_ctx* ctx, const htgx_map_desc* desc, const uint64_t* pix,
+                        const uint64_t* real, uint32_t n, double* weights)
+{
+  if(!ctx || !desc || !pix || !real || !weights || !n) return HTGX_BAD_ARG;
+  const uint64_t npix = (uint64_t)desc->width * desc->height;
+  for(uint32_t i = 0; i < n; ++i) if(pix[i] >= npix) return HTGX_BAD_ARG;
+  if(desc->solver == HTGX_SOLVER_WAVEFRONT_FP32)
+    return htgx_debug_realisations_fp32(ctx, desc, pix, real, n, weights);
+  DevMap M;
+  int res;
+  if((res = htgx_map_setup(ctx, desc, &M)) != HTGX_OK) return res;
+  uint64_t *d_p, *d_r; double* d_w;
+  if((res = dalloc(&d_p, n))) return res;
+  if((res = dalloc(&d_r, n))) return res;
+  if((res = dalloc(&d_w, n))) return res;
+  HIP_OK(hipMemcpy(d_p, pix, n * sizeof(uint64_t), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(d_r, real, n * sizeof(uint64_t), hipMemcpyHostToDevice));
+  realisations_kernel<<<(n + 63) / 64, 64>>>(ctx->S, M, d_p, d_r, n, d_w);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpy(weights, d_w, n * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_OK(hipFree(d_p)); HIP_OK(hipFree(d_r)); HIP_OK(hipFree(d_w));
+  return HTGX_OK;
+}
+
+extern "C" int
+htgx_debug_threefry(const uint64_t* ctr, const uint64_t* key, uint32_t n, uint64_t* out)
+{
+  if(!ctr || !key || !out || !n) return HTGX_BAD_ARG;
+  uint64_t *d_c, *d_k, *d_o;
+  int res;
+  if((res = dalloc(&d_c, 4 * (size_t)n))) return res;
+  if((res = dalloc(&d_k, 4 * (size_t)n))) return res;
+  if((res = dalloc(&d_o, 4 * (size_t)n))) return res;
+  HIP_OK(hipMemcpy(d_c, ctr, 4 * (size_t)n * sizeof(uint64_t), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(d_k, key, 4 * (size_t)n * sizeof(uint64_t), hipMemcpyHostToDevice));
+  threefry_kernel<<<(n + 255) / 256, 256>>>(d_c, d_k, n, d_o);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpy(out, d_o, 4 * (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToHost));
+  HIP_OK(hipFree(d_c)); HIP_OK(hipFree(d_k)); HIP_OK(hipFree(d_o));
+  return HTGX_OK;
+}
+
+extern "C" int
+htgx_debug_sample_dirs(htgx_ctx* ctx, int kind, double param, const double dir_in[3],
+                       const double* u, uint32_t n, double* out)
+{
+  if(!ctx || !dir_in || !u || !out || !n || kind < 0 || kind > 3) return HTGX_BAD_ARG;
+  double *d_u, *d_o;
+  int res;
+  if((res = dalloc(&d_u, 2 * (size_t)n))) return res;
+  if((res = dalloc(&d_o, 3 * (size_t)n))) return res;
+  HIP_OK(hipMemcpy(d_u, u, 2 * (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+  sample_dirs_kernel<<<(n + 255) / 256, 256>>>(ctx->S, kind, param, dir_in[0], dir_in[1],
+                                               dir_in[2], d_u, n, d_o);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpy(out, d_o, 3 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_OK(hipFree(d_u)); HIP_OK(hipFree(d_o));
+  return HTGX_OK;
+}

@@ -1,0 +1,49 @@
+/* Internal state of a libhtgx.so context (not part of the C-ABI). */
+#ifndef HTGX_INTERNAL_H
+#define HTGX_INTERNAL_H
+
+#include "htgx.h"
+#include "htgx_device.h"
+
+#include <hip/hip_runtime.h>
+#include <vector>
+
+struct htgx_wavefront_state;
+
+struct htgx_ctx {
+  int device = 0;
+  htgx::DevScene S;
+  std::vector<void*> bufs;          /* scene buffers owned by the context */
+  std::vector<double> band_lo, band_hi;
+  bool has_temperature = false;
+  uint32_t nlevels = 0;
+  uint64_t nnodes = 0, nleaves = 0;
+  size_t svo_bytes = 0;
+  double us_per_tick = 0.01;
+  /* per-slice partial accumulators of the plain solver */
+  void* d_partials = nullptr;
+  size_t partials_cap = 0;
+  /* wavelength CDF of the last map (cached by range and Tref) */
+  double* d_wcdf = nullptr;
+  double* d_wpdf = nullptr;
+  size_t w_nbands = 0;
+  double w_range[2] = {0, 0};
+  double w_Tref = 0;
+  /* event counters */
+  int counting = 0;
+  unsigned long long* d_counters = nullptr;
+  uint64_t last_nrealisations = 0, last_ncell = 0, last_nseg = 0;
+  /* FP32 wavefront solver state */
+  htgx_wavefront_state* wf = nullptr;
+};
+
+int htgx_ctx_init(htgx_ctx* ctx, const htgx_scene_desc& scene);
+int htgx_map_setup(htgx_ctx* ctx, const htgx_map_desc* desc, htgx::DevMap* map);
+int htgx_render_wavefront(htgx_ctx* ctx, const htgx::DevMap& map, htgx_mc_accum* accum,
+                          hipStream_t stream);
+int htgx_debug_realisations_fp32(htgx_ctx* ctx, const htgx_map_desc* desc,
+                                 const uint64_t* pix, const uint64_t* real, uint32_t n,
+                                 double* weights);
+void htgx_wavefront_release(htgx_ctx* ctx);
+
+#endif /* HTGX_INTERNAL_H */

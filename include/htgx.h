@@ -1,0 +1,351 @@
+/*
+ * htgx -- MI355X (gfx950) null-collision path-tracing core for htrdr-atmosphere.
+ *
+ * C-ABI of the GPU library libhtgx.so. Plain C types only: pointers, sizes,
+ * fixed-width integers and doubles. Every entry point returns an `int' whose
+ * values are those of rsys `res_T' (RES_OK = 0, RES_BAD_ARG, RES_MEM_ERR,
+ * RES_IO_ERR, RES_UNKNOWN_ERR, RES_BAD_OP), so a caller written against
+ * htrdr's core can test them exactly as it tests `htrdr_draw_map'.
+ *
+ * What the library replaces in the reference (file:line under the htrdr tree):
+ *  - htrdr_draw_map + draw_map + draw_tile, the OpenMP/MPI tile dispatcher
+ *    (src/core/htrdr_draw_map.c:271-621, src/core/htrdr_draw_map.h:89-93),
+ *    together with the per-pixel callbacks it drives for the atmosphere
+ *    command: draw_pixel_xwave and draw_pixel_flux
+ *    (src/atmosphere/htrdr_atmosphere_draw_map.c:210-436);
+ *  - the radiative random walks they call,
+ *    atmosphere_compute_radiance_sw / _lw
+ *    (src/atmosphere/htrdr_atmosphere_compute_radiance_sw.c:257-511,
+ *     src/atmosphere/htrdr_atmosphere_compute_radiance_lw.c:141-279);
+ *  - the star-vx octree traversal and htsky property fetches they rely on
+ *    (call sites ..._sw.c:87-88,130-131,163-166,202-203,243-244,369-370);
+ *  - the per-tile star-sp Threefry proxies (src/core/htrdr_draw_map.c:427-449),
+ *    replaced by counter-based Threefry-4x64-20 streams keyed by
+ *    (seed, pixel, realisation).
+ *
+ * Pixel layouts written by htgx_draw_map are bit-compatible with
+ * struct atmosphere_pixel_xwave / atmosphere_pixel_flux
+ * (src/atmosphere/htrdr_atmosphere_c.h:46-68) and struct htrdr_accum /
+ * htrdr_estimate (src/core/htrdr_accum.h:31-44), so the reference's
+ * dump_buffer (src/atmosphere/htrdr_atmosphere_draw_map.c:546-587) can write
+ * them unchanged.
+ */
+#ifndef HTGX_H
+#define HTGX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Result codes: same values as rsys res_T. */
+enum htgx_res {
+  HTGX_OK = 0,
+  HTGX_BAD_ARG = 1,
+  HTGX_MEM_ERR = 2,
+  HTGX_IO_ERR = 3,
+  HTGX_UNKNOWN_ERR = 4,
+  HTGX_BAD_OP = 5
+};
+
+/* Same order as enum htrdr_spectral_type (src/core/htrdr_spectral.h:35-39) */
+enum htgx_spectral_type {
+  HTGX_SPECTRAL_LW = 0,
+  HTGX_SPECTRAL_SW = 1,
+  HTGX_SPECTRAL_SW_CIE_XYZ = 2 /* Not supported: rejected with HTGX_BAD_ARG */
+};
+
+/* Radiance components, enum atmosphere_radiance_cpnt_flag
+ * (src/atmosphere/htrdr_atmosphere_c.h:38-44) */
+enum htgx_radiance_cpnt {
+  HTGX_RADIANCE_DIRECT = 1,
+  HTGX_RADIANCE_DIFFUSE = 2,
+  HTGX_RADIANCE_ALL = 3
+};
+
+/* Optical properties, enum htsky_property as used at the call sites
+ * (..._sw.c:87,163,397,483; ..._lw.c:78,113) */
+enum htgx_property {
+  HTGX_Ka = 0,   /* absorption coefficient */
+  HTGX_Ks = 1,   /* scattering coefficient */
+  HTGX_Kext = 2  /* extinction coefficient */
+};
+
+enum htgx_sensor_type {
+  HTGX_SENSOR_CAMERA = 0,    /* pinhole perspective camera, option -C */
+  HTGX_SENSOR_RECTANGLE = 1  /* flux-density map, option -p */
+};
+
+enum htgx_solver {
+  HTGX_SOLVER_PLAIN_FP64 = 0,    /* one path per thread, FP64 everywhere */
+  HTGX_SOLVER_WAVEFRONT_FP32 = 1 /* FP32 transport, FP64 Tmin + accumulators */
+};
+
+enum htgx_ground_type {
+  HTGX_GROUND_NONE = 0,  /* no ground geometry */
+  HTGX_GROUND_PLANE = 1, /* horizontal plane z = plane_z, i.e. a flat quad
+                            repeated to infinity with -R */
+  HTGX_GROUND_MESH = 2   /* triangle mesh traced through a GPU BVH */
+};
+
+/*******************************************************************************
+ * Output pixel formats (bit-compatible with the reference)
+ ******************************************************************************/
+/* struct htrdr_accum, src/core/htrdr_accum.h:31-36 */
+struct htgx_accum {
+  double sum_weights;
+  double sum_weights_sqr;
+  size_t nweights;
+  size_t nfailures;
+};
+
+/* struct htrdr_estimate, src/core/htrdr_accum.h:41-44 */
+struct htgx_estimate {
+  double E;
+  double SE;
+};
+
+/* struct atmosphere_pixel_xwave, src/atmosphere/htrdr_atmosphere_c.h:46-50 */
+struct htgx_pixel_xwave {
+  struct htgx_estimate radiance; /* W/m^2/sr (W/m^2/sr/nm if monochromatic) */
+  struct htgx_estimate radiance_temperature; /* K, longwave only */
+  struct htgx_accum time; /* microseconds per realisation */
+};
+
+/* struct atmosphere_pixel_flux, src/atmosphere/htrdr_atmosphere_c.h:59-62 */
+struct htgx_pixel_flux {
+  struct htgx_accum flux; /* W/m^2 */
+  struct htgx_accum time; /* microseconds per realisation */
+};
+
+/* Device-side per-pixel Monte Carlo accumulator. The weight sums are FP64
+ * with Neumaier compensation; they are folded into struct htgx_accum when the
+ * map is finalised. 64 bytes. */
+struct htgx_mc_accum {
+  double sum, sum_c;   /* sum of weights and its compensation term */
+  double sum2, sum2_c; /* sum of squared weights and its compensation */
+  double tsum, tsum2;  /* per-realisation time (us) and its square */
+  uint64_t n;          /* accepted realisations */
+  uint64_t nfail;      /* rejected samples (flux map, inside geometry) */
+};
+
+/*******************************************************************************
+ * Scene description (what htsky + the ground + the sun provide in the
+ * reference; src/atmosphere/htrdr_atmosphere.c:288-446)
+ ******************************************************************************/
+/* Spectral bands and correlated-k quadrature: htsky_find_spectral_band and
+ * htsky_spectral_band_sample_quadrature (call site ..._draw_map.c:383-384) */
+struct htgx_spectral_desc {
+  uint32_t nbands;
+  const double* band_lo;      /* [nbands] nm, sorted and contiguous */
+  const double* band_hi;      /* [nbands] nm */
+  const uint32_t* nquads;     /* [nbands] #quadrature points per band */
+  const double* quad_weights; /* [sum nquads] weights, summing to 1 per band */
+};
+
+/* Gas: horizontally infinite, piecewise constant in uniform layers along Z */
+struct htgx_gas_desc {
+  uint32_t nlayers;
+  double z_bottom;        /* m */
+  double layer_thickness; /* m */
+  const double* ka;       /* [sum nquads][nlayers] m^-1, per (band, quad) */
+  const double* ks;       /* [nbands][nlayers] m^-1 */
+  const double* temperature; /* [nlayers] K; may be NULL for shortwave */
+};
+
+/* Clouds: liquid water content on a regular grid. Per band, the particle
+ * extinction is kext = cext * LWC with single scattering albedo ssa and a
+ * Henyey-Greenstein phase function of asymmetry g. */
+struct htgx_cloud_desc {
+  uint32_t def[3];   /* #cells along X, Y, Z; 0 => no clouds */
+  double lower[3];   /* m */
+  double upper[3];   /* m */
+  const float* lwc;  /* [def[2]][def[1]][def[0]] g/m^3, X fastest */
+  const double* cext; /* [nbands] m^2/g */
+  const double* ssa;  /* [nbands] */
+  const double* hg_g; /* [nbands] */
+};
+
+/* Ground. The plane ground is one material; a mesh carries one material index
+ * per triangle. Materials are Lambertian reflectors with an albedo and a
+ * temperature (the emission temperature used by the longwave walk,
+ * ..._lw.c:266-270). */
+struct htgx_ground_desc {
+  int type;              /* enum htgx_ground_type */
+  double plane_z;        /* HTGX_GROUND_PLANE */
+  uint32_t plane_mtl;    /* HTGX_GROUND_PLANE material index */
+  uint32_t nverts;       /* HTGX_GROUND_MESH */
+  uint32_t ntris;
+  const float* verts;    /* [nverts][3] m */
+  const uint32_t* tris;  /* [ntris][3] */
+  const uint32_t* tri_mtl; /* [ntris] */
+  uint32_t nmtls;
+  const double* mtl_albedo;      /* [nmtls] */
+  const double* mtl_temperature; /* [nmtls] K */
+};
+
+struct htgx_scene_desc {
+  struct htgx_spectral_desc spectral;
+  struct htgx_gas_desc gas;
+  struct htgx_cloud_desc clouds;
+  struct htgx_ground_desc ground;
+  double sun_azimuth;   /* degrees, option -D */
+  double sun_elevation; /* degrees, option -D */
+  double optical_thickness; /* octree merge threshold, option -T */
+  uint32_t leaf_block;  /* edge (in cells) of the finest octree leaf block:
+                           1, 2, 4, ...; the cloud definition must be a
+                           multiple of it */
+};
+
+/*******************************************************************************
+ * Map description: what htrdr_draw_map_args + the atmosphere command state
+ * carry for one image or flux map (src/core/htrdr_draw_map.h:58-63,
+ * src/atmosphere/htrdr_atmosphere_c.h:95-126)
+ ******************************************************************************/
+struct htgx_map_desc {
+  uint32_t width, height; /* image definition, option -i def=WxH */
+  uint32_t spp;           /* realisations per pixel in this call */
+  uint32_t spp_offset;    /* index of the first realisation of this call */
+  int sensor;             /* enum htgx_sensor_type */
+  double pos[3], tgt[3], up[3]; /* camera (-C) or rectangle (-p) frame */
+  double fov_y;           /* camera vertical field of view, degrees */
+  double rect_size[2];    /* rectangle size, m */
+  int spectral_type;      /* enum htgx_spectral_type */
+  double wlen_range[2];   /* nm, option -s sw=/lw= */
+  double ref_temperature; /* K, option -s Tref= */
+  uint64_t seed;          /* key of the Threefry streams */
+  int solver;             /* enum htgx_solver */
+};
+
+/*******************************************************************************
+ * Context: device copy of the scene (SVO, LWC grid, gas tables, spectral
+ * tables, ground) on the current HIP device. One context per GPU; not
+ * re-entrant.
+ ******************************************************************************/
+struct htgx_ctx;
+
+/* Build the octree on the host, upload everything to the current device.
+ * Replaces the htsky_create + ground/sun setup of htrdr_atmosphere_create
+ * (src/atmosphere/htrdr_atmosphere.c:348-382). */
+int htgx_ctx_create(const struct htgx_scene_desc* scene, struct htgx_ctx** ctx);
+int htgx_ctx_destroy(struct htgx_ctx* ctx);
+
+/* Number of nodes, leaves and octree levels (diagnostics). */
+int htgx_ctx_get_svo_info
+  (const struct htgx_ctx* ctx,
+   uint32_t* nlevels,
+   uint64_t* nnodes,
+   uint64_t* nleaves);
+
+/* Replaces htrdr_draw_map (src/core/htrdr_draw_map.c:497-621) as called by
+ * atmosphere_draw_map (src/atmosphere/htrdr_atmosphere_draw_map.c:593-616):
+ * renders desc->spp realisations per pixel and writes one pixel struct per
+ * pixel, row-major, into the caller's HOST memory `pixels' whose rows are
+ * `pitch' bytes apart. The pixel struct is htgx_pixel_xwave for a camera and
+ * htgx_pixel_flux for a rectangle. Returns HTGX_BAD_ARG on the conditions
+ * of check_draw_map_args (src/core/htrdr_draw_map.c:66-79): null spp,
+ * null definition, pitch smaller than a row. */
+int htgx_draw_map
+  (struct htgx_ctx* ctx,
+   const struct htgx_map_desc* desc,
+   void* pixels,
+   size_t pitch);
+
+/* Device-resident variant for callers that keep the accumulators in HBM and
+ * render in several calls (realisation slices [spp_offset, spp_offset+spp)).
+ * `accum' is a DEVICE pointer to width*height struct htgx_mc_accum, row-major;
+ * `stream' is a hipStream_t (NULL = default stream). Asynchronous. */
+int htgx_render_accum
+  (struct htgx_ctx* ctx,
+   const struct htgx_map_desc* desc,
+   struct htgx_mc_accum* accum,
+   void* stream);
+
+/* Fold device accumulators into reference pixel structs (host memory). For
+ * a longwave camera, the brightness temperature and its error are computed as
+ * in draw_pixel_xwave (src/atmosphere/htrdr_atmosphere_draw_map.c:423-435). */
+int htgx_finalize_pixels
+  (const struct htgx_map_desc* desc,
+   const struct htgx_mc_accum* accum_host, /* HOST copy of the accumulators */
+   void* pixels,
+   size_t pitch);
+
+/* Counters of the last htgx_render_accum call on this context (summed over
+ * all realisations): leaf visits, tentative collisions, ground-BVH nodes and
+ * triangles tested, and the number of realisations. Used for the algorithmic
+ * byte model of the roofline. Only maintained when counting is enabled. */
+struct htgx_counters {
+  uint64_t nrealisations;
+  uint64_t nleaf;   /* octree leaf x gas layer segments visited */
+  uint64_t ncoll;   /* tentative collisions (property fetches) */
+  uint64_t nbvh;    /* BVH nodes visited */
+  uint64_t ntri;    /* triangles tested */
+  uint64_t nevents; /* scattering / surface events */
+  uint64_t nseg;    /* segments handed to the null-collision loops */
+  uint64_t nlwc;    /* cloud LWC fetches (4 B each) */
+};
+int htgx_set_counting(struct htgx_ctx* ctx, int enable);
+int htgx_get_counters(struct htgx_ctx* ctx, struct htgx_counters* counters);
+
+/*******************************************************************************
+ * Test entry points (device execution of single building blocks so that they
+ * can be compared with the CPU oracle on identical inputs)
+ ******************************************************************************/
+/* One segment of the front-to-back traversal: the intersection of an octree
+ * leaf (or the cloud-free region) with a gas layer, as handed to the hit
+ * filters of ..._sw.c:74,148 and ..._lw.c:65 (svx_hit.distance[0..1] plus the
+ * voxel bounds fetched at ..._sw.c:163-166). */
+struct htgx_segment {
+  double t0, t1;     /* distances along the ray */
+  double kmin, kmax; /* bounds of the requested property on the segment */
+  int64_t leaf;      /* global id of the octree leaf, -1 outside the clouds */
+  int32_t layer;     /* gas layer index, -1 outside the gas */
+  int32_t pad;
+};
+
+/* Trace nrays rays (org/dir/range packed as 8 doubles per ray) and write at
+ * most max_segs segments per ray to segs[iray*max_segs ...] and the count to
+ * nsegs[iray] (may exceed max_segs: then only the first max_segs are
+ * written). All pointers are HOST memory. */
+int htgx_debug_trace_segments
+  (struct htgx_ctx* ctx,
+   const double* rays, /* [nrays][8] = org[3], dir[3], range[2] */
+   uint32_t nrays,
+   int prop, uint32_t iband, uint32_t iquad,
+   struct htgx_segment* segs,
+   uint32_t max_segs,
+   uint32_t* nsegs);
+
+/* Weight of individual realisations (pixel index y*W+x, realisation index),
+ * computed by the solver desc->solver. For the plain FP64 solver these use the
+ * same random streams as htgx_draw_map. HOST arrays. */
+int htgx_debug_realisations
+  (struct htgx_ctx* ctx,
+   const struct htgx_map_desc* desc,
+   const uint64_t* pixel_ids,
+   const uint64_t* realisation_ids,
+   uint32_t n,
+   double* weights);
+
+/* Threefry-4x64-20 blocks computed on the device: out[i][0..3] =
+ * threefry(ctr[i][0..3], key[i][0..3]). HOST arrays. */
+int htgx_debug_threefry
+  (const uint64_t* ctr, const uint64_t* key, uint32_t n, uint64_t* out);
+
+/* Direction samplers on the device (phase functions, sun cone, cosine
+ * hemisphere). kind: 0 = HG(param) relative to dir_in, 1 = Rayleigh relative
+ * to dir_in, 2 = sun cone (param unused), 3 = cosine hemisphere around dir_in.
+ * u = [n][2] canonical numbers, out = [n][3]. For the sun cone, the scene sun
+ * of ctx is used. HOST arrays. */
+int htgx_debug_sample_dirs
+  (struct htgx_ctx* ctx,
+   int kind, double param, const double dir_in[3],
+   const double* u, uint32_t n, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HTGX_H */

@@ -1,0 +1,1710 @@
+/*
+ * CPU FP64 oracle of the htrdr-atmosphere null-collision core.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for libhtgx.so. It is never linked
+ * into, loaded by or called from the product; only tests/,
+ * __graft_entry__.smoke() and bench.py's CPU-baseline leg use it.
+ *
+ * Design: written for clarity, not speed. The octree is a pointer tree built
+ * recursively and traversed recursively front to back (a different algorithm
+ * from the GPU's level-map stepping), the random streams are a from-spec
+ * Threefry-4x64-20, and the walks follow the reference functions line by line
+ * (citations are file:line under the htrdr tree).
+ */
+#define _POSIX_C_SOURCE 200809L
+#include "oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define PI 3.14159265358979323846
+
+/*******************************************************************************
+ * Threefry-4x64-20 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+ * easy as 1, 2, 3", SC'11, section 3 and table 4). Replaces star-sp's
+ * Threefry proxy of src/core/htrdr_draw_map.c:433-442.
+ ******************************************************************************/
+static uint64_t rotl64(uint64_t x, unsigned r) { return (x << r) | (x >> (64 - r)); }
+
+void
+oracle_threefry4x64_20(const uint64_t ctr[4], const uint64_t key[4], uint64_t out[4])
+{
+  /* Rotation constants R_{4x64} of the paper's table 4, rows d mod 8 */
+  static const unsigned R[8][2] = {
+    {14,16},{52,57},{23,40},{5,37},{25,33},{46,12},{58,22},{32,32}};
+  uint64_t ks[5];
+  uint64_t x[4];
+  int r, i;
+  ks[4] = 0x1BD11BDAA9FC1A22ULL; /* key-schedule parity constant C240 */
+  for(i = 0; i < 4; ++i) { ks[i] = key[i]; ks[4] ^= key[i]; x[i] = ctr[i] + ks[i]; }
+  for(r = 0; r < 20; ++r) {
+    const unsigned* rc = R[r % 8];
+    if((r % 2) == 0) {
+      x[0] += x[1]; x[1] = rotl64(x[1], rc[0]); x[1] ^= x[0];
+      x[2] += x[3]; x[3] = rotl64(x[3], rc[1]); x[3] ^= x[2];
+    } else {
+      x[0] += x[3]; x[3] = rotl64(x[3], rc[0]); x[3] ^= x[0];
+      x[2] += x[1]; x[1] = rotl64(x[1], rc[1]); x[1] ^= x[2];
+    }
+    if((r % 4) == 3) { /* key injection s after every 4 rounds */
+      const uint64_t s = (uint64_t)(r / 4 + 1);
+      for(i = 0; i < 4; ++i) x[i] += ks[(s + (uint64_t)i) % 5];
+      x[3] += s;
+    }
+  }
+  for(i = 0; i < 4; ++i) out[i] = x[i];
+}
+
+/* One random stream per realisation: key = (seed, pixel, realisation, 0),
+ * counter = (block, 0, 0, 0); the 4 words of a block are used in order and
+ * mapped to [0,1) with their 53 high bits. */
+struct orng {
+  uint64_t key[4];
+  uint64_t block;
+  uint64_t buf[4];
+  int idx;
+};
+
+static void
+orng_init(struct orng* rng, uint64_t seed, uint64_t pixel, uint64_t realisation)
+{
+  rng->key[0] = seed; rng->key[1] = pixel; rng->key[2] = realisation;
+  rng->key[3] = 0;
+  rng->block = 0;
+  rng->idx = 4;
+}
+
+static double
+canonical(struct orng* rng)
+{
+  if(rng->idx == 4) {
+    const uint64_t ctr[4] = {rng->block, 0, 0, 0};
+    oracle_threefry4x64_20(ctr, rng->key, rng->buf);
+    rng->block += 1;
+    rng->idx = 0;
+  }
+  return (double)(rng->buf[rng->idx++] >> 11) * 0x1.0p-53;
+}
+
+/* Exp(1) sample, ssp_ran_exp(rng, 1) (called at ..._sw.c:140,212,239,359):
+ * inverse CDF on 1-u so that the result is finite */
+static double ran_exp(struct orng* rng) { return -log(1.0 - canonical(rng)); }
+
+void
+oracle_stream_canonicals
+  (uint64_t seed, uint64_t pixel, uint64_t realisation, uint32_t n, double* out)
+{
+  struct orng rng;
+  uint32_t i;
+  orng_init(&rng, seed, pixel, realisation);
+  for(i = 0; i < n; ++i) out[i] = canonical(&rng);
+}
+
+/*******************************************************************************
+ * Small vector helpers
+ ******************************************************************************/
+static double dot3(const double a[3], const double b[3])
+{ return a[0]*b[0] + a[1]*b[1] + a[2]*b[2]; }
+static void cross3(double r[3], const double a[3], const double b[3])
+{
+  double t[3];
+  t[0] = a[1]*b[2] - a[2]*b[1];
+  t[1] = a[2]*b[0] - a[0]*b[2];
+  t[2] = a[0]*b[1] - a[1]*b[0];
+  r[0] = t[0]; r[1] = t[1]; r[2] = t[2];
+}
+static double normalize3(double r[3], const double a[3])
+{
+  const double len = sqrt(dot3(a, a));
+  r[0] = a[0]/len; r[1] = a[1]/len; r[2] = a[2]/len;
+  return len;
+}
+
+/* Orthonormal basis (b1, b2) completing a unit vector n (Duff, Burgess,
+ * Christensen, Hery, Kensler, Liani, Villemin, "Building an orthonormal
+ * basis, revisited", JCGT 6(1), 2017, listing 3). */
+static void
+basis(const double n[3], double b1[3], double b2[3])
+{
+  const double sign = n[2] >= 0 ? 1.0 : -1.0;
+  const double a = -1.0 / (sign + n[2]);
+  const double b = n[0] * n[1] * a;
+  b1[0] = 1.0 + sign * n[0] * n[0] * a;
+  b1[1] = sign * b;
+  b1[2] = -sign * n[0];
+  b2[0] = b;
+  b2[1] = sign + n[1] * n[1] * a;
+  b2[2] = -n[1];
+}
+
+/* World direction from a local (cos theta, phi) around the unit vector n */
+static void
+local_to_world(const double n[3], double cos_t, double phi, double out[3])
+{
+  double b1[3], b2[3];
+  const double sin_t = sqrt(fmax(0.0, 1.0 - cos_t*cos_t));
+  const double cx = cos(phi) * sin_t;
+  const double cy = sin(phi) * sin_t;
+  basis(n, b1, b2);
+  out[0] = b1[0]*cx + b2[0]*cy + n[0]*cos_t;
+  out[1] = b1[1]*cx + b2[1]*cy + n[1]*cos_t;
+  out[2] = b1[2]*cx + b2[2]*cy + n[2]*cos_t;
+}
+
+/*******************************************************************************
+ * Spectral functions: src/core/htrdr_spectral.h:56-148, htrdr_spectral.c
+ ******************************************************************************/
+/* Planck's law in W/m^2/sr/m, src/core/htrdr_spectral.h:102-116 */
+double
+oracle_planck_monochromatic(double lambda, double temperature)
+{
+  const double c = 299792458.0;
+  const double h = 6.62607015e-34;
+  const double k = 1.380649e-23;
+  const double l2 = lambda * lambda;
+  const double l5 = l2 * l2 * lambda;
+  return ((2.0 * h * c * c) / l5) / (exp(h * c / (lambda * k * temperature)) - 1.0);
+}
+
+/* Fraction of black-body emission beyond v = C2/(lambda T), series of
+ * src/core/htrdr_spectral.h:56-83 (5 exponential terms for v >= 2, a
+ * polynomial in v below) */
+double
+oracle_wiebelt(double v)
+{
+  const double f = 15.0 / (PI*PI*PI*PI);
+  const double z0 = 1.0/3.0;
+  const double z1 = 1.0/8.0;
+  const double z2 = 1.0/60.0;
+  const double z4 = 1.0/5040.0;
+  const double z6 = 1.0/272160.0;
+  const double z8 = 1.0/13305600.0;
+  double w, v2, v4;
+  int m;
+  if(v >= 2.0) {
+    w = 0;
+    for(m = 1; m < 6; ++m)
+      w += exp(-m*v)/(m*m*m*m) * (((m*v+3)*m*v+6)*m*v+6);
+    w = w * f;
+  } else {
+    v2 = v*v;
+    v4 = v2*v2;
+    w = z0 - z1*v + z2*v2 - z4*v2*v2 + z6*v4*v2 - z8*v4*v4;
+    w = 1. - f*v2*v*w;
+  }
+  return w;
+}
+
+/* src/core/htrdr_spectral.h:85-99 */
+double
+oracle_blackbody_fraction(double lambda0, double lambda1, double T)
+{
+  const double C2 = 1.43877735e-2; /* m.K */
+  return oracle_wiebelt(C2 / lambda1 / T) - oracle_wiebelt(C2 / lambda0 / T);
+}
+
+/* src/core/htrdr_spectral.h:118-148 */
+double
+oracle_planck(double lmin, double lmax, double T)
+{
+  const double T2 = T*T;
+  const double T4 = T2*T2;
+  if(lmin == lmax) return oracle_planck_monochromatic(lmin, T);
+  return oracle_blackbody_fraction(lmin, lmax, T) * 5.6696e-8 * T4 / (PI * (lmax - lmin));
+}
+
+/* Doubling then bisection, src/core/htrdr_spectral.c:31-89 */
+int
+oracle_brightness_temperature(double lmin, double lmax, double radiance, double* out)
+{
+  const size_t MAX_ITER = 100;
+  const double eps_T = 1e-4;
+  const double eps_B = radiance * 1e-8;
+  double T = 0, T0 = 0, T1 = 0, T2 = 200, B, B0 = 0;
+  size_t i;
+  for(i = 0; i < MAX_ITER; ++i) {
+    if(oracle_planck(lmin, lmax, T2) >= radiance) break;
+    T2 *= 2;
+  }
+  if(i >= MAX_ITER) return HTGX_BAD_OP;
+  for(i = 0; i < MAX_ITER; ++i) {
+    T = (T1 + T2) * 0.5;
+    B = oracle_planck(lmin, lmax, T);
+    if(B < radiance) T1 = T; else T2 = T;
+    if(fabs(T - T0) < eps_T || fabs(B - B0) < eps_B) break;
+    T0 = T;
+    B0 = B;
+  }
+  if(i >= MAX_ITER) return HTGX_BAD_OP;
+  *out = T;
+  return HTGX_OK;
+}
+
+/* src/core/htrdr_spectral.c:91-121 */
+double
+oracle_radiance_temperature(double lmin, double lmax, double radiance)
+{
+  double T = 0;
+  double avg = radiance;
+  if(lmin != lmax) avg /= (lmax - lmin);
+  if(oracle_brightness_temperature(lmin, lmax, avg, &T) != HTGX_OK) T = 0;
+  return T;
+}
+
+/*******************************************************************************
+ * Planck wavelength sampler: src/core/htrdr_ran_wlen_planck.c
+ ******************************************************************************/
+struct oracle_wlen_planck {
+  double range[2];
+  double band_len;
+  double Tref;
+  size_t nbands;
+  double* pdf;
+  double* cdf;
+};
+
+/* src/atmosphere/htrdr_atmosphere.c:53-70 */
+size_t
+oracle_spectral_bands_count(const double range_nm[2])
+{
+  return (size_t)rint(range_nm[1] - range_nm[0]);
+}
+
+/* src/core/htrdr_ran_wlen_planck.c:287-343 (creation) and :55-134 (CDF) */
+int
+oracle_wlen_planck_create
+  (const double range[2], size_t nbands, double Tref, struct oracle_wlen_planck** out)
+{
+  struct oracle_wlen_planck* p = calloc(1, sizeof(*p));
+  size_t i;
+  double sum = 0;
+  if(!p) return HTGX_MEM_ERR;
+  p->range[0] = range[0];
+  p->range[1] = range[1];
+  p->Tref = Tref;
+  p->nbands = nbands;
+  if(nbands) {
+    p->band_len = (range[1] - range[0]) / (double)nbands;
+    p->pdf = malloc(nbands * sizeof(double));
+    p->cdf = malloc(nbands * sizeof(double));
+    if(!p->pdf || !p->cdf) { oracle_wlen_planck_destroy(p); return HTGX_MEM_ERR; }
+    for(i = 0; i < nbands; ++i) {
+      double lo = range[0] + (double)i * p->band_len;
+      double hi = fmin(lo + p->band_len, range[1]);
+      p->pdf[i] = oracle_blackbody_fraction(lo*1e-9, hi*1e-9, Tref);
+      sum += p->pdf[i];
+    }
+    for(i = 0; i < nbands; ++i) {
+      p->pdf[i] /= sum;
+      p->cdf[i] = i ? p->pdf[i] + p->cdf[i-1] : p->pdf[i];
+    }
+    p->cdf[nbands-1] = 1.0;
+  }
+  *out = p;
+  return HTGX_OK;
+}
+
+void
+oracle_wlen_planck_destroy(struct oracle_wlen_planck* p)
+{
+  if(!p) return;
+  free(p->pdf);
+  free(p->cdf);
+  free(p);
+}
+
+/* Bisection on the black-body fraction, src/core/htrdr_ran_wlen_planck.c:136-216 */
+static double
+wlen_sample_continue(const struct oracle_wlen_planck* p, double r, double* pdf)
+{
+  const double rm0 = p->range[0] * 1e-9, rm1 = p->range[1] * 1e-9;
+  double lmin = rm0, lmax = rm1, l = 0, lprev = 0, bf = 0, bfprev = 0;
+  const double bfmm = oracle_blackbody_fraction(rm0, rm1, p->Tref);
+  size_t i;
+  for(i = 0; i < 100; ++i) {
+    l = (lmin + lmax) * 0.5;
+    bf = oracle_blackbody_fraction(rm0, l, p->Tref);
+    if(bf / bfmm < r) lmin = l; else lmax = l;
+    if(fabs(lprev - l) < 1e-15 || fabs(bfprev - bf) < 1e-6) break;
+    lprev = l;
+    bfprev = bf;
+  }
+  if(pdf) {
+    const double Bl = oracle_planck(l, l, p->Tref);
+    const double Bm = oracle_planck(rm0, rm1, p->Tref);
+    *pdf = Bl / (Bm * (rm1 - rm0)) * 1e-9;
+  }
+  return l * 1e9;
+}
+
+/* src/core/htrdr_ran_wlen_planck.c:218-268 and :359-376 */
+double
+oracle_wlen_planck_sample
+  (const struct oracle_wlen_planck* p, double r0, double r1, double* pdf)
+{
+  if(p->nbands) {
+    /* first band whose cumulative is strictly greater than r0 */
+    size_t lo = 0, hi = p->nbands - 1;
+    double blo;
+    while(lo < hi) {
+      const size_t mid = (lo + hi) / 2;
+      if(p->cdf[mid] > r0) hi = mid; else lo = mid + 1;
+    }
+    blo = p->range[0] + (double)lo * p->band_len;
+    if(pdf) *pdf = p->pdf[lo] * (1.0 / p->band_len);
+    return blo + p->band_len * r1;
+  } else if(fabs(p->range[0] - p->range[1]) <= 1e-6) {
+    if(pdf) *pdf = 1;
+    return p->range[0];
+  } else {
+    return wlen_sample_continue(p, r0, pdf);
+  }
+}
+
+/*******************************************************************************
+ * Sky: gas layers + cloud octree (the property model and acceleration
+ * structure htsky builds at src/atmosphere/htrdr_atmosphere.c:366-382).
+ ******************************************************************************/
+struct onode {
+  float mn, mx;         /* exact FP32 min/max of the LWC over the node */
+  uint8_t level;
+  uint8_t leaf;
+  uint32_t x, y, z;     /* node coordinates at its level */
+  struct onode* child[8];
+};
+
+struct oracle_sky {
+  struct htgx_scene_desc scene; /* shallow copy (arrays owned by caller) */
+  uint32_t* quad_offset;       /* [nbands] first quadrature point of a band */
+  double* quad_cdf;            /* [sum nquads] */
+  /* clouds */
+  int has_clouds;
+  uint32_t B;                  /* leaf block */
+  uint32_t nlevels;
+  uint32_t dims[32][3];        /* #nodes per axis per level */
+  uint64_t level_offset[32];
+  double cs[3], inv_cs[3];     /* cell size and inverse */
+  double bs[3], inv_bs[3];     /* block size and inverse */
+  double cext_max;
+  struct onode* root;
+  uint64_t nnodes_alloc, nleaves;
+  /* gas */
+  double inv_dz_layer;
+  /* sun */
+  double sun_dir[3];
+  double sun_cos_half;
+  double sun_solid_angle;
+  /* spectral sampler per (range) is created per draw_map call */
+};
+
+static int
+cell_exists(const struct oracle_sky* s, int64_t x, int64_t y, int64_t z)
+{
+  return x >= 0 && y >= 0 && z >= 0
+      && x < s->scene.clouds.def[0] && y < s->scene.clouds.def[1]
+      && z < s->scene.clouds.def[2];
+}
+
+/* Recursive octree build. A level-0 node is a leaf block of BxBxB cells; a
+ * node of a higher level merges into a leaf when all its children are leaves
+ * and cext_max * (LWC_max - LWC_min) * edge <= T, with edge the largest
+ * nominal edge of the node. This is the k_min/k_max merge of htsky driven by
+ * the optical thickness threshold -T (doc/htrdr-atmosphere.1.in:226-228,
+ * default src/../config.mk:52). */
+static struct onode*
+build_node(struct oracle_sky* s, uint32_t level, uint32_t x, uint32_t y, uint32_t z)
+{
+  struct onode* n = calloc(1, sizeof(*n));
+  if(!n) return NULL;
+  n->level = (uint8_t)level;
+  n->x = x; n->y = y; n->z = z;
+  s->nnodes_alloc++;
+  if(level == 0) {
+    const uint32_t B = s->B;
+    uint32_t i, j, k;
+    float mn = FLT_MAX, mx = -FLT_MAX;
+    for(k = z*B; k < (z+1)*B; ++k)
+    for(j = y*B; j < (y+1)*B; ++j)
+    for(i = x*B; i < (x+1)*B; ++i) {
+      const float v = s->scene.clouds.lwc
+        [((size_t)k*s->scene.clouds.def[1] + j)*s->scene.clouds.def[0] + i];
+      if(v < mn) mn = v;
+      if(v > mx) mx = v;
+    }
+    n->mn = mn; n->mx = mx;
+    n->leaf = 1;
+  } else {
+    int c, all_leaves = 1;
+    float mn = FLT_MAX, mx = -FLT_MAX;
+    double edge = 0;
+    for(c = 0; c < 8; ++c) {
+      const uint32_t cx = 2*x + (uint32_t)(c & 1);
+      const uint32_t cy = 2*y + (uint32_t)((c >> 1) & 1);
+      const uint32_t cz = 2*z + (uint32_t)((c >> 2) & 1);
+      if(cx >= s->dims[level-1][0] || cy >= s->dims[level-1][1]
+      || cz >= s->dims[level-1][2]) continue;
+      n->child[c] = build_node(s, level-1, cx, cy, cz);
+      if(!n->child[c]) return NULL;
+      if(n->child[c]->mn < mn) mn = n->child[c]->mn;
+      if(n->child[c]->mx > mx) mx = n->child[c]->mx;
+      all_leaves &= n->child[c]->leaf;
+    }
+    n->mn = mn; n->mx = mx;
+    {
+      int a;
+      for(a = 0; a < 3; ++a) {
+        const double e = s->bs[a] * (double)(1u << level);
+        if(e > edge) edge = e;
+      }
+    }
+    if(all_leaves
+    && s->cext_max * ((double)mx - (double)mn) * edge
+       <= s->scene.optical_thickness) {
+      n->leaf = 1;
+      for(c = 0; c < 8; ++c) { free(n->child[c]); n->child[c] = NULL; }
+    }
+  }
+  return n;
+}
+
+static void
+free_node(struct onode* n)
+{
+  int c;
+  if(!n) return;
+  for(c = 0; c < 8; ++c) free_node(n->child[c]);
+  free(n);
+}
+
+static void
+count_leaves(const struct onode* n, uint64_t* nleaves)
+{
+  int c;
+  if(n->leaf) { (*nleaves)++; return; }
+  for(c = 0; c < 8; ++c) if(n->child[c]) count_leaves(n->child[c], nleaves);
+}
+
+static void
+sun_setup(struct oracle_sky* s)
+{
+  /* spherical_to_cartesian_dir, src/atmosphere/htrdr_atmosphere.c:89-111;
+   * cone constants, src/atmosphere/htrdr_atmosphere_sun.c:84-87 */
+  const double az = s->scene.sun_azimuth * PI / 180.0;
+  const double el = s->scene.sun_elevation * PI / 180.0;
+  s->sun_dir[0] = cos(el) * cos(az);
+  s->sun_dir[1] = cos(el) * sin(az);
+  s->sun_dir[2] = sin(el);
+  s->sun_cos_half = cos(4.6524e-3);
+  s->sun_solid_angle = 2.0 * PI * (1.0 - s->sun_cos_half);
+}
+
+int
+oracle_sky_create(const struct htgx_scene_desc* scene, struct oracle_sky** out)
+{
+  struct oracle_sky* s;
+  uint32_t b, nq = 0;
+  if(!scene || !out) return HTGX_BAD_ARG;
+  s = calloc(1, sizeof(*s));
+  if(!s) return HTGX_MEM_ERR;
+  s->scene = *scene;
+  s->quad_offset = calloc(scene->spectral.nbands + 1, sizeof(uint32_t));
+  for(b = 0; b < scene->spectral.nbands; ++b) {
+    s->quad_offset[b] = nq;
+    nq += scene->spectral.nquads[b];
+  }
+  s->quad_offset[scene->spectral.nbands] = nq;
+  s->quad_cdf = calloc(nq ? nq : 1, sizeof(double));
+  for(b = 0; b < scene->spectral.nbands; ++b) {
+    uint32_t q;
+    double acc = 0;
+    for(q = 0; q < scene->spectral.nquads[b]; ++q) {
+      acc += scene->spectral.quad_weights[s->quad_offset[b] + q];
+      s->quad_cdf[s->quad_offset[b] + q] = acc;
+    }
+    s->quad_cdf[s->quad_offset[b] + scene->spectral.nquads[b] - 1] = 1.0;
+  }
+  s->inv_dz_layer = 1.0 / scene->gas.layer_thickness;
+  sun_setup(s);
+
+  s->has_clouds = scene->clouds.def[0] && scene->clouds.def[1]
+               && scene->clouds.def[2] && scene->clouds.lwc;
+  if(s->has_clouds) {
+    int a;
+    uint32_t l;
+    s->B = scene->leaf_block ? scene->leaf_block : 1;
+    for(a = 0; a < 3; ++a) {
+      if(scene->clouds.def[a] % s->B) { oracle_sky_destroy(s); return HTGX_BAD_ARG; }
+      s->cs[a] = (scene->clouds.upper[a] - scene->clouds.lower[a])
+               / (double)scene->clouds.def[a];
+      s->inv_cs[a] = 1.0 / s->cs[a];
+      s->bs[a] = s->cs[a] * (double)s->B;
+      s->inv_bs[a] = 1.0 / s->bs[a];
+      s->dims[0][a] = scene->clouds.def[a] / s->B;
+    }
+    s->nlevels = 1;
+    s->level_offset[0] = 0;
+    while(s->dims[s->nlevels-1][0] > 1 || s->dims[s->nlevels-1][1] > 1
+       || s->dims[s->nlevels-1][2] > 1) {
+      l = s->nlevels;
+      for(a = 0; a < 3; ++a) s->dims[l][a] = (s->dims[l-1][a] + 1) / 2;
+      s->level_offset[l] = s->level_offset[l-1]
+        + (uint64_t)s->dims[l-1][0] * s->dims[l-1][1] * s->dims[l-1][2];
+      s->nlevels++;
+    }
+    s->cext_max = 0;
+    for(b = 0; b < scene->spectral.nbands; ++b)
+      if(scene->clouds.cext[b] > s->cext_max) s->cext_max = scene->clouds.cext[b];
+    s->root = build_node(s, s->nlevels - 1, 0, 0, 0);
+    if(!s->root) { oracle_sky_destroy(s); return HTGX_MEM_ERR; }
+    count_leaves(s->root, &s->nleaves);
+  }
+  *out = s;
+  return HTGX_OK;
+}
+
+void
+oracle_sky_destroy(struct oracle_sky* s)
+{
+  if(!s) return;
+  free_node(s->root);
+  free(s->quad_offset);
+  free(s->quad_cdf);
+  free(s);
+}
+
+int
+oracle_sky_get_svo_info
+  (const struct oracle_sky* s, uint32_t* nlevels, uint64_t* nnodes, uint64_t* nleaves)
+{
+  if(!s) return HTGX_BAD_ARG;
+  if(nlevels) *nlevels = s->has_clouds ? s->nlevels : 0;
+  if(nnodes) *nnodes = s->has_clouds
+    ? s->level_offset[s->nlevels-1] + 1 : 0;
+  if(nleaves) *nleaves = s->nleaves;
+  return HTGX_OK;
+}
+
+static uint64_t
+leaf_id(const struct oracle_sky* s, const struct onode* n)
+{
+  const uint32_t* d = s->dims[n->level];
+  return s->level_offset[n->level]
+       + ((uint64_t)n->z * d[1] + n->y) * d[0] + n->x;
+}
+
+int
+oracle_leaf_of_cell
+  (const struct oracle_sky* s, const uint32_t cell[3], int64_t* leaf,
+   float* mn, float* mx)
+{
+  const struct onode* n;
+  uint32_t blk[3];
+  int a;
+  if(!s || !s->has_clouds) return HTGX_BAD_ARG;
+  if(!cell_exists(s, cell[0], cell[1], cell[2])) return HTGX_BAD_ARG;
+  for(a = 0; a < 3; ++a) blk[a] = cell[a] / s->B;
+  n = s->root;
+  while(!n->leaf) {
+    const uint32_t l = (uint32_t)n->level - 1;
+    const int c = (int)(((blk[0] >> l) & 1) | (((blk[1] >> l) & 1) << 1)
+                | (((blk[2] >> l) & 1) << 2));
+    n = n->child[c];
+  }
+  *leaf = (int64_t)leaf_id(s, n);
+  *mn = n->mn;
+  *mx = n->mx;
+  return HTGX_OK;
+}
+
+/* Node box in block units along each axis */
+static void
+node_range(const struct oracle_sky* s, const struct onode* n, uint32_t lo[3], uint32_t hi[3])
+{
+  int a;
+  const uint32_t c[3] = {n->x, n->y, n->z};
+  for(a = 0; a < 3; ++a) {
+    const uint32_t sz = 1u << n->level;
+    lo[a] = c[a] * sz;
+    hi[a] = (c[a] + 1) * sz;
+    if(hi[a] > s->dims[0][a]) hi[a] = s->dims[0][a];
+  }
+}
+
+/* World coordinate of the block plane k along axis a */
+static double
+plane(const struct oracle_sky* s, int a, uint32_t k)
+{
+  return s->scene.clouds.lower[a] + (double)k * s->bs[a];
+}
+
+struct ray {
+  double o[3], d[3], inv[3];
+  double range[2];
+};
+
+/* Slab test of a box given by block planes */
+static void
+slab(const struct oracle_sky* s, const struct ray* r, const uint32_t lo[3],
+     const uint32_t hi[3], double* tn, double* tf)
+{
+  int a;
+  *tn = -INFINITY;
+  *tf = INFINITY;
+  for(a = 0; a < 3; ++a) {
+    const double plo = plane(s, a, lo[a]);
+    const double phi = plane(s, a, hi[a]);
+    if(r->d[a] != 0) {
+      const double t0 = (plo - r->o[a]) * r->inv[a];
+      const double t1 = (phi - r->o[a]) * r->inv[a];
+      const double n = t0 < t1 ? t0 : t1;
+      const double f = t0 < t1 ? t1 : t0;
+      if(n > *tn) *tn = n;
+      if(f < *tf) *tf = f;
+    } else if(r->o[a] < plo || r->o[a] > phi) {
+      *tn = INFINITY;
+      *tf = -INFINITY;
+    }
+  }
+}
+
+struct leaf_hit {
+  double tf;        /* leaf exit distance (unclipped) */
+  int64_t id;
+  float mn, mx;
+  uint32_t lo[3], hi[3]; /* block range */
+};
+
+struct leaf_list {
+  struct leaf_hit* v;
+  size_t n, cap;
+};
+
+static void
+leaf_list_push(struct leaf_list* l, const struct leaf_hit* h)
+{
+  if(l->n == l->cap) {
+    l->cap = l->cap ? l->cap * 2 : 64;
+    l->v = realloc(l->v, l->cap * sizeof(*l->v));
+  }
+  l->v[l->n++] = *h;
+}
+
+/* Recursive front-to-back traversal: the leaves the ray crosses with a
+ * positive length inside its range (svx_octree_trace_ray semantics as used by
+ * htsky_trace_ray, call sites ..._sw.c:243-244,369-370, ..._lw.c:205-206). */
+static void
+traverse(const struct oracle_sky* s, const struct onode* n, const struct ray* r,
+         struct leaf_list* out)
+{
+  uint32_t lo[3], hi[3];
+  double tn, tf;
+  node_range(s, n, lo, hi);
+  slab(s, r, lo, hi, &tn, &tf);
+  if(fmax(tn, r->range[0]) >= fmin(tf, r->range[1])) return;
+  if(n->leaf) {
+    struct leaf_hit h;
+    int a;
+    h.tf = tf;
+    h.id = (int64_t)leaf_id(s, n);
+    h.mn = n->mn;
+    h.mx = n->mx;
+    for(a = 0; a < 3; ++a) { h.lo[a] = lo[a]; h.hi[a] = hi[a]; }
+    leaf_list_push(out, &h);
+  } else {
+    const struct onode* kids[8];
+    double ktn[8];
+    int nk = 0, c, i, j;
+    for(c = 0; c < 8; ++c) {
+      uint32_t clo[3], chi[3];
+      double ctn, ctf;
+      if(!n->child[c]) continue;
+      node_range(s, n->child[c], clo, chi);
+      slab(s, r, clo, chi, &ctn, &ctf);
+      if(fmax(ctn, r->range[0]) >= fmin(ctf, r->range[1])) continue;
+      kids[nk] = n->child[c];
+      ktn[nk] = ctn;
+      nk++;
+    }
+    /* insertion sort on the entry distance */
+    for(i = 1; i < nk; ++i) {
+      for(j = i; j > 0 && ktn[j-1] > ktn[j]; --j) {
+        const double t = ktn[j]; const struct onode* k = kids[j];
+        ktn[j] = ktn[j-1]; kids[j] = kids[j-1];
+        ktn[j-1] = t; kids[j-1] = k;
+      }
+    }
+    for(i = 0; i < nk; ++i) traverse(s, kids[i], r, out);
+  }
+}
+
+/* Gas and particle coefficients of a property (htsky_fetch_raw_property and
+ * htsky_fetch_svx_voxel_property, call sites ..._sw.c:87-88,130-131,163-166,
+ * 202-203,427-430, ..._lw.c:78-79,113-116). The cloud part is
+ * c(prop, band) * LWC with c = cext*(1-ssa), cext*ssa or cext. */
+static double
+gas_k(const struct oracle_sky* s, int prop, uint32_t iband, uint32_t iquad, int32_t j)
+{
+  const uint32_t nl = s->scene.gas.nlayers;
+  const double ka = s->scene.gas.ka[(size_t)(s->quad_offset[iband] + iquad) * nl + (uint32_t)j];
+  const double ks = s->scene.gas.ks[(size_t)iband * nl + (uint32_t)j];
+  switch(prop) {
+    case HTGX_Ka: return ka;
+    case HTGX_Ks: return ks;
+    default: return ka + ks;
+  }
+}
+
+static double
+cloud_c(const struct oracle_sky* s, int prop, uint32_t iband)
+{
+  const double ce = s->scene.clouds.cext[iband];
+  const double w = s->scene.clouds.ssa[iband];
+  switch(prop) {
+    case HTGX_Ka: return ce * (1.0 - w);
+    case HTGX_Ks: return ce * w;
+    default: return ce;
+  }
+}
+
+/* A segment of the overlay of the octree leaves with the gas layers */
+struct oseg {
+  double t0, t1, kmin, kmax;
+  int in_cloud;
+  struct leaf_hit leaf;
+  int32_t layer;  /* -1 if no gas */
+};
+
+typedef int (*seg_cb)(const struct oseg* seg, void* ctx); /* 0 = stop */
+
+/* Enumerate the segments of [range[0], range[1]] along the ray, front to
+ * back, calling cb on each segment of positive length that holds some medium.
+ * Returns 1 if cb stopped the traversal. */
+static int
+for_each_segment
+  (const struct oracle_sky* s, const double org[3], const double dir[3],
+   const double range[2], int prop, uint32_t iband, uint32_t iquad,
+   seg_cb cb, void* ctx)
+{
+  struct ray r;
+  struct leaf_list leaves = {NULL, 0, 0};
+  double tbox = INFINITY; /* entry distance into the cloud grid */
+  size_t ileaf = 0;
+  int cloud_state = 2; /* 0 before the grid, 1 inside, 2 after / no clouds */
+  const uint32_t nl = s->scene.gas.nlayers;
+  const double zb = s->scene.gas.z_bottom, dzl = s->scene.gas.layer_thickness;
+  int32_t j;
+  double t, tmax;
+  int a, stopped = 0;
+
+  for(a = 0; a < 3; ++a) {
+    r.o[a] = org[a];
+    r.d[a] = dir[a];
+    r.inv[a] = 1.0 / dir[a];
+  }
+  r.range[0] = range[0];
+  r.range[1] = range[1];
+  t = range[0];
+  tmax = range[1];
+
+  if(s->has_clouds) {
+    uint32_t lo[3] = {0,0,0};
+    double tn, tf;
+    slab(s, &r, lo, s->dims[0], &tn, &tf);
+    if(fmax(tn, t) < fmin(tf, tmax)) {
+      traverse(s, s->root, &r, &leaves);
+      if(leaves.n) {
+        if(tn > t) { cloud_state = 0; tbox = tn; } else { cloud_state = 1; }
+      }
+    }
+  }
+
+  /* gas layer of the starting point */
+  {
+    const double z = org[2] + t * dir[2];
+    const double fj = floor((z - zb) * s->inv_dz_layer);
+    j = fj < -1.0 ? -1 : fj > (double)nl ? (int32_t)nl : (int32_t)fj;
+  }
+
+  while(t < tmax) {
+    double tc, tg, t1;
+    /* next cloud boundary */
+    if(cloud_state == 0) tc = tbox;
+    else if(cloud_state == 1) tc = fmax(leaves.v[ileaf].tf, t);
+    else tc = INFINITY;
+    /* next gas boundary */
+    if(dir[2] > 0 && j < (int32_t)nl) {
+      tg = (zb + (double)(j + 1) * dzl - org[2]) * r.inv[2];
+    } else if(dir[2] < 0 && j >= 0) {
+      tg = (zb + (double)j * dzl - org[2]) * r.inv[2];
+    } else {
+      tg = INFINITY;
+    }
+    tg = fmax(tg, t);
+    t1 = fmin(fmin(tc, tg), tmax);
+
+    if(t1 > t) {
+      const int in_gas = j >= 0 && j < (int32_t)nl;
+      const int in_cloud = cloud_state == 1;
+      if(in_gas || in_cloud) {
+        struct oseg seg;
+        const double g = in_gas ? gas_k(s, prop, iband, iquad, j) : 0.0;
+        const double c = in_cloud ? cloud_c(s, prop, iband) : 0.0;
+        seg.t0 = t;
+        seg.t1 = t1;
+        seg.in_cloud = in_cloud;
+        seg.layer = in_gas ? j : -1;
+        if(in_cloud) {
+          seg.leaf = leaves.v[ileaf];
+          seg.kmin = g + c * (double)seg.leaf.mn;
+          seg.kmax = g + c * (double)seg.leaf.mx;
+        } else {
+          memset(&seg.leaf, 0, sizeof(seg.leaf));
+          seg.leaf.id = -1;
+          seg.kmin = g;
+          seg.kmax = g;
+        }
+        if(!cb(&seg, ctx)) { stopped = 1; break; }
+      }
+    }
+    if(t1 >= tmax) break;
+    t = t1;
+    if(tg == t1) j += dir[2] > 0 ? 1 : -1;
+    if(tc == t1) {
+      if(cloud_state == 0) cloud_state = 1;
+      else if(cloud_state == 1) {
+        ileaf++;
+        if(ileaf >= leaves.n) cloud_state = 2;
+      }
+    }
+    if(tg == INFINITY && tc == INFINITY) break;
+  }
+  free(leaves.v);
+  return stopped;
+}
+
+struct collect_ctx {
+  struct htgx_segment* segs;
+  uint32_t max, n;
+};
+
+static int
+collect_cb(const struct oseg* seg, void* p)
+{
+  struct collect_ctx* c = p;
+  if(c->n < c->max) {
+    struct htgx_segment* o = c->segs + c->n;
+    o->t0 = seg->t0;
+    o->t1 = seg->t1;
+    o->kmin = seg->kmin;
+    o->kmax = seg->kmax;
+    o->leaf = seg->in_cloud ? seg->leaf.id : -1;
+    o->layer = seg->layer;
+    o->pad = 0;
+  }
+  c->n++;
+  return 1;
+}
+
+int
+oracle_trace_segments
+  (const struct oracle_sky* s, const double ray[8], int prop, uint32_t iband,
+   uint32_t iquad, struct htgx_segment* segs, uint32_t max_segs, uint32_t* nsegs)
+{
+  struct collect_ctx c;
+  c.segs = segs;
+  c.max = max_segs;
+  c.n = 0;
+  for_each_segment(s, ray, ray + 3, ray + 6, prop, iband, iquad, collect_cb, &c);
+  *nsegs = c.n;
+  return HTGX_OK;
+}
+
+/* Property at a point inside a segment. The gas layer is the segment's and the
+ * cloud cell is clamped into the segment's leaf so that k lies in
+ * [kmin, kmax] (the hint of htsky_fetch_raw_property, ..._sw.c:202-204). */
+static double
+raw_in_seg(const struct oracle_sky* s, const struct oseg* seg, int prop,
+           int cpnt, uint32_t iband, uint32_t iquad, const double x[3])
+{
+  double g = 0.0, c = 0.0;
+  if((cpnt & 1) && seg->layer >= 0) g = gas_k(s, prop, iband, iquad, seg->layer);
+  if((cpnt & 2) && seg->in_cloud) {
+    int a;
+    int64_t ci[3];
+    float v;
+    for(a = 0; a < 3; ++a) {
+      const double f = floor((x[a] - s->scene.clouds.lower[a]) * s->inv_cs[a]);
+      const int64_t lo = (int64_t)seg->leaf.lo[a] * s->B;
+      const int64_t hi = (int64_t)seg->leaf.hi[a] * s->B - 1;
+      int64_t i = f < (double)lo ? lo : f > (double)hi ? hi : (int64_t)f;
+      ci[a] = i;
+    }
+    v = s->scene.clouds.lwc
+      [((size_t)ci[2]*s->scene.clouds.def[1] + (size_t)ci[1])*s->scene.clouds.def[0]
+       + (size_t)ci[0]];
+    c = cloud_c(s, prop, iband) * (double)v;
+  }
+  return g + c;
+}
+
+double
+oracle_fetch_raw
+  (const struct oracle_sky* s, int prop, int cpnt, uint32_t iband, uint32_t iquad,
+   const double pos[3])
+{
+  /* Standalone lookup (no segment hint): layer and cell from the position */
+  double g = 0.0, c = 0.0;
+  const double fj = floor((pos[2] - s->scene.gas.z_bottom) * s->inv_dz_layer);
+  if((cpnt & 1) && fj >= 0 && fj < (double)s->scene.gas.nlayers)
+    g = gas_k(s, prop, iband, iquad, (int32_t)fj);
+  if((cpnt & 2) && s->has_clouds) {
+    int a, inside = 1;
+    int64_t ci[3];
+    for(a = 0; a < 3; ++a) {
+      const double f = floor((pos[a] - s->scene.clouds.lower[a]) * s->inv_cs[a]);
+      if(f < 0 || f >= (double)s->scene.clouds.def[a]) inside = 0;
+      ci[a] = (int64_t)f;
+    }
+    if(inside) {
+      const float v = s->scene.clouds.lwc
+        [((size_t)ci[2]*s->scene.clouds.def[1] + (size_t)ci[1])
+         * s->scene.clouds.def[0] + (size_t)ci[0]];
+      c = cloud_c(s, prop, iband) * (double)v;
+    }
+  }
+  return g + c;
+}
+
+/*******************************************************************************
+ * Null-collision hit filters
+ ******************************************************************************/
+/* k_min/k_max transmissivity, ..._sw.c:147-251 */
+struct tr_ctx {
+  const struct oracle_sky* s;
+  struct orng* rng;
+  const double* o; const double* d;
+  int prop; uint32_t iband, iquad;
+  double Ts, Tmin;
+  int collided;
+};
+
+static int
+tr_cb(const struct oseg* seg, void* p)
+{
+  struct tr_ctx* c = p;
+  const double kmin = seg->kmin, kmax = seg->kmax;
+  double t = seg->t0;
+  c->Tmin += (seg->t1 - seg->t0) * kmin;               /* ..._sw.c:169 */
+  for(;;) {
+    const double Tdif = (seg->t1 - t) * (kmax - kmin);  /* ..._sw.c:175-176 */
+    if(c->Ts > Tdif || !(kmax > kmin)) {                /* ..._sw.c:179-183 */
+      c->Ts -= Tdif;
+      return 1;
+    } else {
+      double x[3], k, proba;
+      t += c->Ts / (kmax - kmin);                       /* ..._sw.c:190-193 */
+      x[0] = c->o[0] + t * c->d[0];
+      x[1] = c->o[1] + t * c->d[1];
+      x[2] = c->o[2] + t * c->d[2];
+      k = raw_in_seg(c->s, seg, c->prop, 3, c->iband, c->iquad, x);
+      proba = (k - kmin) / (kmax - kmin);               /* ..._sw.c:206 */
+      if(canonical(c->rng) < proba) { c->collided = 1; return 0; }
+      c->Ts = ran_exp(c->rng);                          /* ..._sw.c:212 */
+    }
+  }
+}
+
+static double
+transmissivity
+  (const struct oracle_sky* s, struct orng* rng, int prop, uint32_t iband,
+   uint32_t iquad, const double pos[3], const double dir[3], const double range[2])
+{
+  struct tr_ctx c;
+  c.s = s; c.rng = rng; c.o = pos; c.d = dir;
+  c.prop = prop; c.iband = iband; c.iquad = iquad;
+  c.Ts = ran_exp(rng);                                  /* ..._sw.c:239 */
+  c.Tmin = 0;
+  c.collided = 0;
+  for_each_segment(s, pos, dir, range, prop, iband, iquad, tr_cb, &c);
+  if(c.collided) return 0;                              /* ..._sw.c:246-250 */
+  return c.Tmin ? exp(-c.Tmin) : 1.0;
+}
+
+double
+oracle_transmissivity
+  (const struct oracle_sky* s, uint64_t seed, uint64_t pixel, uint64_t realisation,
+   int prop, uint32_t iband, uint32_t iquad, const double pos[3],
+   const double dir[3], const double range[2])
+{
+  struct orng rng;
+  orng_init(&rng, seed, pixel, realisation);
+  return transmissivity(s, &rng, prop, iband, iquad, pos, dir, range);
+}
+
+/* Delta tracking on the Ks majorant, ..._sw.c:73-145 */
+struct sc_ctx {
+  const struct oracle_sky* s;
+  struct orng* rng;
+  const double* o; const double* d;
+  uint32_t iband, iquad;
+  double Ts;
+  double dst;
+  int hit;
+  struct oseg seg; /* segment of the collision */
+};
+
+static int
+sc_cb(const struct oseg* seg, void* p)
+{
+  struct sc_ctx* c = p;
+  const double ks_max = seg->kmax;
+  double t = seg->t0;
+  for(;;) {
+    const double T = (seg->t1 - t) * ks_max;           /* ..._sw.c:96-97 */
+    if(c->Ts > T || !(ks_max > 0)) {                   /* ..._sw.c:100-104 */
+      c->Ts -= T;
+      return 1;
+    } else {
+      double x[3], ks;
+      t += c->Ts / ks_max;                             /* ..._sw.c:111-114 */
+      if(t > 1e9) return 1;                            /* ..._sw.c:123 */
+      x[0] = c->o[0] + t * c->d[0];
+      x[1] = c->o[1] + t * c->d[1];
+      x[2] = c->o[2] + t * c->d[2];
+      ks = raw_in_seg(c->s, seg, HTGX_Ks, 3, c->iband, c->iquad, x);
+      if(canonical(c->rng) < ks / ks_max) {            /* ..._sw.c:134-138 */
+        c->hit = 1;
+        c->dst = t;
+        c->seg = *seg;
+        return 0;
+      }
+      c->Ts = ran_exp(c->rng);                         /* ..._sw.c:140 */
+    }
+  }
+}
+
+/* Kext majorant with absorption / scattering / null events, ..._lw.c:64-135 */
+enum { EV_ABS, EV_SCA, EV_NONE };
+struct lw_ctx {
+  const struct oracle_sky* s;
+  struct orng* rng;
+  const double* o; const double* d;
+  uint32_t iband, iquad;
+  double Ts, dst;
+  int event;
+  struct oseg seg;
+};
+
+static int
+lw_cb(const struct oseg* seg, void* p)
+{
+  struct lw_ctx* c = p;
+  const double kext_max = seg->kmax;
+  double t = seg->t0;
+  for(;;) {
+    const double T = (seg->t1 - t) * kext_max;          /* ..._lw.c:84-85 */
+    if(c->Ts > T || !(kext_max > 0)) {
+      c->Ts -= T;
+      return 1;
+    } else {
+      double x[3], ka, ks, r, pa, ps;
+      t += c->Ts / kext_max;                            /* ..._lw.c:96-104 */
+      x[0] = c->o[0] + t * c->d[0];
+      x[1] = c->o[1] + t * c->d[1];
+      x[2] = c->o[2] + t * c->d[2];
+      ka = raw_in_seg(c->s, seg, HTGX_Ka, 3, c->iband, c->iquad, x);
+      ks = raw_in_seg(c->s, seg, HTGX_Ks, 3, c->iband, c->iquad, x);
+      r = canonical(c->rng);                            /* ..._lw.c:118-131 */
+      pa = ka / kext_max;
+      ps = ks / kext_max;
+      if(r < pa) { c->event = EV_ABS; c->dst = t; c->seg = *seg; return 0; }
+      if(r < pa + ps) { c->event = EV_SCA; c->dst = t; c->seg = *seg; return 0; }
+      c->Ts = ran_exp(c->rng);
+    }
+  }
+}
+
+/*******************************************************************************
+ * Ground: the plane repeated to infinity (-R over a flat quad,
+ * src/atmosphere/htrdr_atmosphere_ground.c:170-217 + src/core/htrdr_slab.c)
+ * with the self-hit rules of src/core/htrdr_geometry.c:116-150.
+ ******************************************************************************/
+struct ghit {
+  int hit;
+  double t;
+  double N[3];
+  uint32_t mtl;
+};
+
+static struct ghit
+ground_trace(const struct oracle_sky* s, const double o[3], const double d[3],
+             const double range[2], int from_ground)
+{
+  struct ghit h;
+  h.hit = 0;
+  h.t = FLT_MAX; /* distance of S3D_HIT_NULL */
+  if(s->scene.ground.type == HTGX_GROUND_PLANE && !from_ground && d[2] != 0) {
+    const double t = (s->scene.ground.plane_z - o[2]) / d[2];
+    if(t > range[0] && t < range[1]) {
+      h.hit = 1;
+      h.t = t;
+      h.N[0] = 0; h.N[1] = 0; h.N[2] = 1;
+      h.mtl = s->scene.ground.plane_mtl;
+    }
+  }
+  return h;
+}
+
+/*******************************************************************************
+ * Sun, phase functions, BRDF
+ ******************************************************************************/
+static int
+in_solar_cone(const struct oracle_sky* s, const double dir[3])
+{
+  return dot3(dir, s->sun_dir) >= s->sun_cos_half; /* ..._sun.c:154-166 */
+}
+
+/* ..._sun.c:126-136: uniform direction in the cone, pdf = 1/solid angle */
+static double
+sun_sample(const struct oracle_sky* s, struct orng* rng, double out[3])
+{
+  const double u0 = canonical(rng), u1 = canonical(rng);
+  const double cos_t = 1.0 - u0 * (1.0 - s->sun_cos_half);
+  local_to_world(s->sun_dir, cos_t, 2.0 * PI * u1, out);
+  return 1.0 / s->sun_solid_angle;
+}
+
+static double
+hg_cos_sample(double g, double u)
+{
+  if(fabs(g) < 1e-6) return 1.0 - 2.0 * u;
+  {
+    const double t = (1.0 - g*g) / (1.0 - g + 2.0*g*u);
+    double c = (1.0 + g*g - t*t) / (2.0*g);
+    return c < -1 ? -1 : c > 1 ? 1 : c;
+  }
+}
+
+static double
+hg_eval(double g, double mu)
+{
+  const double d = 1.0 + g*g - 2.0*g*mu;
+  return (1.0 - g*g) / (4.0 * PI * d * sqrt(d));
+}
+
+/* p(mu) = 3/(16 pi)(1+mu^2); inverse of the CDF (mu^3+3mu+4)/8 by Cardano */
+static double
+rayleigh_cos_sample(double u)
+{
+  const double q = 4.0 * u - 2.0;  /* -q/2 of the depressed cubic */
+  const double A = cbrt(q + sqrt(q*q + 1.0));
+  const double c = A - 1.0 / A;
+  return c < -1 ? -1 : c > 1 ? 1 : c;
+}
+
+static double
+rayleigh_eval(double mu) { return 3.0 / (16.0 * PI) * (1.0 + mu*mu); }
+
+double
+oracle_phase_eval(int kind, double param, double mu)
+{
+  return kind == 0 ? hg_eval(param, mu) : rayleigh_eval(mu);
+}
+
+/* Cosine-weighted hemisphere around n (ssp_ran_hemisphere_cos, ..._draw_map.c:104;
+ * Lambertian ssf_bsdf_sample) */
+static void
+cos_hemisphere(const double n[3], double u0, double u1, double out[3])
+{
+  local_to_world(n, sqrt(1.0 - u0), 2.0 * PI * u1, out);
+}
+
+int
+oracle_sample_dirs
+  (const struct oracle_sky* s, int kind, double param, const double dir_in[3],
+   const double* u, uint32_t n, double* out)
+{
+  uint32_t i;
+  for(i = 0; i < n; ++i) {
+    const double u0 = u[2*i], u1 = u[2*i+1];
+    double* o = out + 3*i;
+    switch(kind) {
+      case 0: local_to_world(dir_in, hg_cos_sample(param, u0), 2.0*PI*u1, o); break;
+      case 1: local_to_world(dir_in, rayleigh_cos_sample(u0), 2.0*PI*u1, o); break;
+      case 2: {
+        const double cos_t = 1.0 - u0 * (1.0 - s->sun_cos_half);
+        local_to_world(s->sun_dir, cos_t, 2.0*PI*u1, o);
+      } break;
+      case 3: cos_hemisphere(dir_in, u0, u1, o); break;
+      default: return HTGX_BAD_ARG;
+    }
+  }
+  return HTGX_OK;
+}
+
+/*******************************************************************************
+ * Random walks
+ ******************************************************************************/
+/* ..._sw.c:256-511 */
+static double
+radiance_sw
+  (const struct oracle_sky* s, struct orng* rng, int cpnt_mask,
+   const double pos_in[3], const double dir_in[3], double wlen,
+   uint32_t iband, uint32_t iquad)
+{
+  const double g = s->scene.clouds.hg_g ? s->scene.clouds.hg_g[iband] : 0.0;
+  const double L_sun = oracle_planck_monochromatic(wlen * 1e-9, 5778.0);
+  double pos[3], dir[3];
+  double ksi = 1, w = 0;
+  int from_ground = 0;
+  memcpy(pos, pos_in, sizeof(pos));
+  memcpy(dir, dir_in, sizeof(dir));
+
+  if((cpnt_mask & HTGX_RADIANCE_DIRECT) && in_solar_cone(s, dir)) { /* :324-337 */
+    const double range[2] = {0, FLT_MAX};
+    const struct ghit h = ground_trace(s, pos, dir, range, 0);
+    if(!h.hit) w = L_sun * transmissivity(s, rng, HTGX_Kext, iband, iquad, pos, dir, range);
+  }
+  if(!(cpnt_mask & HTGX_RADIANCE_DIFFUSE)) return w;
+
+  for(;;) {
+    struct sc_ctx sc;
+    struct ghit gh;
+    double range[2], pos_next[3], dir_next[3], sun_dir[3], N[3] = {0,0,0};
+    double Tr_abs, Tr, R, sun_pdf, refl = 1, dst, albedo = 0;
+    int surface, phase_kind = 0;
+    const double r_inf[2] = {0, DBL_MAX};
+
+    gh = ground_trace(s, pos, dir, r_inf, from_ground);   /* :355-356 */
+    sc.s = s; sc.rng = rng; sc.o = pos; sc.d = dir;
+    sc.iband = iband; sc.iquad = iquad;
+    sc.Ts = ran_exp(rng);                                 /* :359 */
+    sc.hit = 0;
+    sc.dst = 0;
+    range[0] = 0; range[1] = gh.t;                        /* :368 */
+    for_each_segment(s, pos, dir, range, HTGX_Ks, iband, iquad, sc_cb, &sc);
+    if(!gh.hit && !sc.hit) break;                         /* :373-375 */
+
+    surface = !sc.hit;                                    /* :384 */
+    dst = surface ? gh.t : sc.dst;
+    pos_next[0] = pos[0] + dir[0] * dst;                  /* :387-389 */
+    pos_next[1] = pos[1] + dir[1] * dst;
+    pos_next[2] = pos[2] + dir[2] * dst;
+    from_ground = surface;                                /* :392 */
+
+    range[0] = 0; range[1] = dst;                         /* :396-399 */
+    Tr_abs = transmissivity(s, rng, HTGX_Ka, iband, iquad, pos, dir, range);
+    if(Tr_abs <= 0) break;
+
+    if(surface) {                                         /* :402-420 */
+      const double wo[3] = {-dir[0], -dir[1], -dir[2]};
+      double u0, u1;
+      N[0] = gh.N[0]; N[1] = gh.N[1]; N[2] = gh.N[2];
+      if(dot3(N, wo) < 0) { N[0] = -N[0]; N[1] = -N[1]; N[2] = -N[2]; }
+      albedo = s->scene.ground.mtl_albedo[gh.mtl];
+      u0 = canonical(rng); u1 = canonical(rng);
+      cos_hemisphere(N, u0, u1, dir_next);
+      refl = albedo;
+    } else {                                              /* :422-443 */
+      const double ks_gas = raw_in_seg(s, &sc.seg, HTGX_Ks, 1, iband, iquad, pos_next);
+      const double ks_par = raw_in_seg(s, &sc.seg, HTGX_Ks, 2, iband, iquad, pos_next);
+      const double ks = ks_par + ks_gas;
+      double u0, u1, mu;
+      phase_kind = canonical(rng) < ks_gas / ks ? 1 : 0; /* 1 Rayleigh, 0 HG */
+      u0 = canonical(rng); u1 = canonical(rng);
+      mu = phase_kind ? rayleigh_cos_sample(u0) : hg_cos_sample(g, u0);
+      local_to_world(dir, mu, 2.0 * PI * u1, dir_next);
+    }
+
+    sun_pdf = sun_sample(s, rng, sun_dir);                /* :457-466 */
+    if(surface) {
+      const double c = dot3(N, sun_dir);
+      R = c < 0 ? 0 : albedo / PI * c;
+    } else {
+      const double mu = dot3(dir, sun_dir);
+      R = phase_kind ? rayleigh_eval(mu) : hg_eval(g, mu);
+    }
+
+    if(R <= 0) {                                          /* :470-485 */
+      Tr = 0;
+    } else {
+      const double r_sun[2] = {0, FLT_MAX};
+      const struct ghit h = ground_trace(s, pos_next, sun_dir, r_sun, from_ground);
+      Tr = h.hit ? 0
+        : transmissivity(s, rng, HTGX_Kext, iband, iquad, pos_next, sun_dir, r_sun);
+    }
+
+    ksi *= Tr_abs;                                        /* :495-496 */
+    w += ksi * L_sun * Tr * R / sun_pdf;
+
+    if(surface && canonical(rng) >= refl) break;          /* :499-500 */
+
+    memcpy(pos, pos_next, sizeof(pos));
+    memcpy(dir, dir_next, sizeof(dir));
+  }
+  return w;
+}
+
+static double
+layer_temperature(const struct oracle_sky* s, int32_t j)
+{
+  const int32_t nl = (int32_t)s->scene.gas.nlayers;
+  if(j < 0) j = 0;
+  if(j >= nl) j = nl - 1;
+  return s->scene.gas.temperature ? s->scene.gas.temperature[j] : 0.0;
+}
+
+/* ..._lw.c:140-279 */
+static double
+radiance_lw
+  (const struct oracle_sky* s, struct orng* rng, const double pos_in[3],
+   const double dir_in[3], double wlen, uint32_t iband, uint32_t iquad)
+{
+  const double g = s->scene.clouds.hg_g ? s->scene.clouds.hg_g[iband] : 0.0;
+  const double wlen_m = wlen * 1e-9;
+  double pos[3], dir[3], w = 0;
+  int from_ground = 0;
+  memcpy(pos, pos_in, sizeof(pos));
+  memcpy(dir, dir_in, sizeof(dir));
+
+  for(;;) {
+    struct lw_ctx c;
+    struct ghit gh;
+    double range[2], pos_next[3], dir_next[3], dst;
+    const double r_inf[2] = {0, DBL_MAX};
+
+    gh = ground_trace(s, pos, dir, r_inf, from_ground);   /* :188-190 */
+    c.s = s; c.rng = rng; c.o = pos; c.d = dir;
+    c.iband = iband; c.iquad = iquad;
+    c.Ts = ran_exp(rng);                                  /* :193 */
+    c.event = EV_NONE;
+    c.dst = 0;
+    range[0] = 0; range[1] = gh.t;                        /* :202 */
+    for_each_segment(s, pos, dir, range, HTGX_Kext, iband, iquad, lw_cb, &c);
+
+    if(!gh.hit && c.event == EV_NONE) { w = 0; break; }   /* :210-213 */
+
+    dst = c.event != EV_NONE ? c.dst : gh.t;
+    pos_next[0] = pos[0] + dir[0] * dst;                  /* :216-218 */
+    pos_next[1] = pos[1] + dir[1] * dst;
+    pos_next[2] = pos[2] + dir[2] * dst;
+
+    if(c.event == EV_ABS) {                               /* :221-227 */
+      w = oracle_planck_monochromatic(wlen_m, layer_temperature(s, c.seg.layer));
+      break;
+    }
+    if(c.event == EV_SCA) {                               /* :233-236 */
+      const double u0 = canonical(rng), u1 = canonical(rng);
+      local_to_world(dir, hg_cos_sample(g, u0), 2.0 * PI * u1, dir_next);
+      from_ground = 0;
+    } else {                                              /* :239-273 */
+      const double wo[3] = {-dir[0], -dir[1], -dir[2]};
+      double N[3], u0, u1, refl;
+      N[0] = gh.N[0]; N[1] = gh.N[1]; N[2] = gh.N[2];
+      if(dot3(N, wo) < 0) { N[0] = -N[0]; N[1] = -N[1]; N[2] = -N[2]; }
+      u0 = canonical(rng); u1 = canonical(rng);
+      cos_hemisphere(N, u0, u1, dir_next);
+      refl = s->scene.ground.mtl_albedo[gh.mtl];
+      if(canonical(rng) >= refl) {
+        const double T = s->scene.ground.mtl_temperature
+          ? s->scene.ground.mtl_temperature[gh.mtl] : 0.0;
+        w = T > 0 ? oracle_planck_monochromatic(wlen_m, T) : 0;
+        break;
+      }
+      from_ground = 1;
+    }
+    memcpy(pos, pos_next, sizeof(pos));
+    memcpy(dir, dir_next, sizeof(dir));
+  }
+  return w;
+}
+
+double
+oracle_radiance
+  (const struct oracle_sky* s, uint64_t seed, uint64_t pixel, uint64_t realisation,
+   int spectral_type, int cpnt_mask, const double pos[3], const double dir[3],
+   double wlen, uint32_t iband, uint32_t iquad)
+{
+  struct orng rng;
+  orng_init(&rng, seed, pixel, realisation);
+  return spectral_type == HTGX_SPECTRAL_LW
+    ? radiance_lw(s, &rng, pos, dir, wlen, iband, iquad)
+    : radiance_sw(s, &rng, cpnt_mask, pos, dir, wlen, iband, iquad);
+}
+
+/*******************************************************************************
+ * Per-pixel realisations: ..._draw_map.c:52-107, 210-322, 325-436
+ ******************************************************************************/
+struct sensor {
+  double pos[3];
+  double x[3], y[3], z[3]; /* camera: right, up, forward; rectangle: frame */
+  double tan_half, aspect;
+  double axis_x[3], axis_y[3];
+};
+
+static int
+sensor_setup(const struct htgx_map_desc* d, struct sensor* c)
+{
+  double t[3];
+  int a;
+  for(a = 0; a < 3; ++a) { c->pos[a] = d->pos[a]; t[a] = d->tgt[a] - d->pos[a]; }
+  if(normalize3(c->z, t) <= 0) return HTGX_BAD_ARG;
+  cross3(t, c->z, d->up);
+  if(normalize3(c->x, t) <= 0) return HTGX_BAD_ARG;
+  if(d->sensor == HTGX_SENSOR_CAMERA) {
+    /* Pinhole camera: x = image right, y = image up (film v = 0 is the top
+     * row, the first row of an htrdr-image, doc/htrdr-image.5) */
+    cross3(c->y, c->x, c->z);
+    c->tan_half = tan(d->fov_y * PI / 180.0 * 0.5);
+    c->aspect = (double)d->width / (double)d->height;
+  } else {
+    /* Rectangle frame, src/core/htrdr_rectangle.c:101-131 */
+    cross3(t, c->z, c->x);
+    normalize3(c->y, t);
+    for(a = 0; a < 3; ++a) {
+      c->axis_x[a] = c->x[a] * d->rect_size[0] * 0.5;
+      c->axis_y[a] = c->y[a] * d->rect_size[1] * 0.5;
+    }
+  }
+  return HTGX_OK;
+}
+
+struct spectral_ctx {
+  struct oracle_wlen_planck* planck;
+};
+
+static uint32_t
+find_band(const struct oracle_sky* s, double wlen)
+{
+  const uint32_t nb = s->scene.spectral.nbands;
+  uint32_t lo = 0, hi = nb - 1;
+  while(lo < hi) {
+    const uint32_t mid = (lo + hi) / 2;
+    if(s->scene.spectral.band_hi[mid] > wlen) hi = mid; else lo = mid + 1;
+  }
+  return lo;
+}
+
+static uint32_t
+sample_quad(const struct oracle_sky* s, uint32_t iband, double r)
+{
+  const uint32_t off = s->quad_offset[iband];
+  const uint32_t nq = s->scene.spectral.nquads[iband];
+  uint32_t q;
+  for(q = 0; q + 1 < nq; ++q) if(s->quad_cdf[off + q] > r) break;
+  return q;
+}
+
+static double
+realisation
+  (const struct oracle_sky* s, const struct htgx_map_desc* d,
+   const struct sensor* sn, const struct oracle_wlen_planck* planck,
+   uint64_t pixel, uint64_t ireal, int* rejected)
+{
+  struct orng rng;
+  const uint64_t px = pixel % d->width, py = pixel / d->width;
+  const double psx = 1.0 / (double)d->width, psy = 1.0 / (double)d->height;
+  double org[3], dir[3], r0, r1, r2, wlen, pdf, weight;
+  uint32_t iband, iquad;
+  int a;
+
+  orng_init(&rng, d->seed, pixel, ireal);
+  *rejected = 0;
+
+  if(d->sensor == HTGX_SENSOR_CAMERA) {
+    const double fx = ((double)px + canonical(&rng)) * psx;   /* :364-369 */
+    const double fy = ((double)py + canonical(&rng)) * psy;
+    double v[3];
+    (void)canonical(&rng); (void)canonical(&rng);           /* lens sample */
+    for(a = 0; a < 3; ++a) {
+      v[a] = sn->z[a] + sn->x[a] * ((2.0*fx - 1.0) * sn->tan_half * sn->aspect)
+                      + sn->y[a] * ((1.0 - 2.0*fy) * sn->tan_half);
+      org[a] = sn->pos[a];
+    }
+    normalize3(dir, v);
+  } else {
+    const double sx = ((double)px + canonical(&rng)) * psx;   /* :69-73 */
+    const double sy = ((double)py + canonical(&rng)) * psy;
+    const double up[3] = {0, 0, 1};
+    const double r_inf[2] = {0, DBL_MAX};
+    struct ghit h;
+    for(a = 0; a < 3; ++a)
+      org[a] = sn->pos[a] + sn->axis_x[a] * (sx*2.0 - 1.0) + sn->axis_y[a] * (sy*2.0 - 1.0);
+    h = ground_trace(s, org, up, r_inf, 0);                   /* :75-100 */
+    if(h.hit && dot3(up, h.N) >= 0) { *rejected = 1; return 0; }
+    {
+      const double u0 = canonical(&rng), u1 = canonical(&rng);
+      cos_hemisphere(sn->z, u0, u1, dir);                     /* :103-104 */
+    }
+  }
+
+  r0 = canonical(&rng); r1 = canonical(&rng); r2 = canonical(&rng);
+  wlen = oracle_wlen_planck_sample(planck, r0, r1, &pdf);
+  pdf *= 1e9;
+  iband = find_band(s, wlen);
+  iquad = sample_quad(s, iband, r2);
+
+  if(d->sensor == HTGX_SENSOR_CAMERA) {
+    weight = d->spectral_type == HTGX_SPECTRAL_LW
+      ? radiance_lw(s, &rng, org, dir, wlen, iband, iquad)
+      : radiance_sw(s, &rng, HTGX_RADIANCE_ALL, org, dir, wlen, iband, iquad);
+    weight /= pdf;
+  } else if(d->spectral_type == HTGX_SPECTRAL_LW) {           /* :265-268 */
+    weight = radiance_lw(s, &rng, org, dir, wlen, iband, iquad) * PI / pdf;
+  } else {                                                    /* :269-302 */
+    double sun_dir[3], Ld = 0, Lf, cosN;
+    (void)sun_sample(s, &rng, sun_dir);
+    cosN = dot3(sn->z, sun_dir);
+    if(cosN > 0)
+      Ld = radiance_sw(s, &rng, HTGX_RADIANCE_DIRECT, org, sun_dir, wlen, iband, iquad);
+    Lf = radiance_sw(s, &rng, HTGX_RADIANCE_DIFFUSE, org, dir, wlen, iband, iquad);
+    weight = (cosN * s->sun_solid_angle * Ld + PI * Lf) / pdf;
+  }
+  return weight;
+}
+
+static int
+make_planck(const struct oracle_sky* s, const struct htgx_map_desc* d,
+            struct oracle_wlen_planck** p)
+{
+  /* src/atmosphere/htrdr_atmosphere.c:384-415: the range is clamped to the
+   * sky's spectral data; default reference temperatures of
+   * src/atmosphere/htrdr_atmosphere_args.c:216-230 */
+  const uint32_t nb = s->scene.spectral.nbands;
+  double range[2];
+  double Tref = d->ref_temperature;
+  size_t n;
+  range[0] = fmax(d->wlen_range[0], s->scene.spectral.band_lo[0]);
+  range[1] = fmin(d->wlen_range[1], s->scene.spectral.band_hi[nb - 1]);
+  if(!(range[1] >= range[0])) return HTGX_BAD_ARG;
+  if(Tref <= 0) Tref = d->spectral_type == HTGX_SPECTRAL_LW ? 290.0 : 5778.0;
+  n = oracle_spectral_bands_count(range);
+  return oracle_wlen_planck_create(range, n, Tref, p);
+}
+
+double
+oracle_realisation
+  (const struct oracle_sky* s, const struct htgx_map_desc* d, uint64_t pixel,
+   uint64_t ireal)
+{
+  struct sensor sn;
+  struct oracle_wlen_planck* planck = NULL;
+  int rejected;
+  double w;
+  if(sensor_setup(d, &sn) != HTGX_OK) return NAN;
+  if(make_planck(s, d, &planck) != HTGX_OK) return NAN;
+  w = realisation(s, d, &sn, planck, pixel, ireal, &rejected);
+  oracle_wlen_planck_destroy(planck);
+  return rejected ? NAN : w;
+}
+
+/* Neumaier compensated summation */
+static void
+nsum(double* sum, double* c, double x)
+{
+  const double t = *sum + x;
+  if(fabs(*sum) >= fabs(x)) *c += (*sum - t) + x;
+  else *c += (x - t) + *sum;
+  *sum = t;
+}
+
+int
+oracle_draw_map
+  (const struct oracle_sky* s, const struct htgx_map_desc* d,
+   struct htgx_mc_accum* accum, int nthreads)
+{
+  struct sensor sn;
+  struct oracle_wlen_planck* planck = NULL;
+  int64_t npix, ip;
+  int res;
+  if(!s || !d || !accum || !d->width || !d->height || !d->spp) return HTGX_BAD_ARG;
+  if((res = sensor_setup(d, &sn)) != HTGX_OK) return res;
+  if((res = make_planck(s, d, &planck)) != HTGX_OK) return res;
+  npix = (int64_t)d->width * d->height;
+#ifdef _OPENMP
+  if(nthreads > 0) omp_set_num_threads(nthreads);
+  #pragma omp parallel for schedule(dynamic, 16)
+#endif
+  for(ip = 0; ip < npix; ++ip) {
+    struct htgx_mc_accum a;
+    uint32_t r;
+    memset(&a, 0, sizeof(a));
+    for(r = 0; r < d->spp; ++r) {
+      struct timespec t0, t1;
+      int rejected;
+      double w, usec;
+      clock_gettime(CLOCK_MONOTONIC, &t0);
+      w = realisation(s, d, &sn, planck, (uint64_t)ip, (uint64_t)(d->spp_offset + r), &rejected);
+      clock_gettime(CLOCK_MONOTONIC, &t1);
+      if(rejected) { a.nfail++; continue; }
+      usec = (double)(t1.tv_sec - t0.tv_sec) * 1e6 + (double)(t1.tv_nsec - t0.tv_nsec) * 1e-3;
+      nsum(&a.sum, &a.sum_c, w);
+      nsum(&a.sum2, &a.sum2_c, w * w);
+      a.tsum += usec;
+      a.tsum2 += usec * usec;
+      a.n++;
+    }
+    accum[ip] = a;
+  }
+  (void)nthreads;
+  oracle_wlen_planck_destroy(planck);
+  return HTGX_OK;
+}
+
+int
+oracle_draw_pixels
+  (const struct oracle_sky* s, const struct htgx_map_desc* d,
+   const uint64_t* pixels, uint64_t npixels, struct htgx_mc_accum* accum, int nthreads)
+{
+  struct sensor sn;
+  struct oracle_wlen_planck* planck = NULL;
+  int64_t i;
+  int res;
+  if(!s || !d || !pixels || !accum || !d->width || !d->height || !d->spp) return HTGX_BAD_ARG;
+  if((res = sensor_setup(d, &sn)) != HTGX_OK) return res;
+  if((res = make_planck(s, d, &planck)) != HTGX_OK) return res;
+#ifdef _OPENMP
+  if(nthreads > 0) omp_set_num_threads(nthreads);
+  #pragma omp parallel for schedule(dynamic, 4)
+#endif
+  for(i = 0; i < (int64_t)npixels; ++i) {
+    struct htgx_mc_accum a;
+    uint32_t r;
+    memset(&a, 0, sizeof(a));
+    for(r = 0; r < d->spp; ++r) {
+      int rejected;
+      const double w = realisation(s, d, &sn, planck, pixels[i],
+        (uint64_t)(d->spp_offset + r), &rejected);
+      if(rejected) { a.nfail++; continue; }
+      nsum(&a.sum, &a.sum_c, w);
+      nsum(&a.sum2, &a.sum2_c, w * w);
+      a.n++;
+    }
+    accum[i] = a;
+  }
+  (void)nthreads;
+  oracle_wlen_planck_destroy(planck);
+  return HTGX_OK;
+}
+
+/* src/core/htrdr_accum.h:48-68 */
+void
+oracle_accum_estimate(const struct htgx_mc_accum* acc, double* E, double* SE)
+{
+  if(!acc->n) { *E = 0; *SE = 0; return; }
+  {
+    const double N = (double)acc->n;
+    const double S = acc->sum + acc->sum_c;
+    const double S2 = acc->sum2 + acc->sum2_c;
+    const double e = S / N;
+    const double V = fmax(S2 / N - e*e, 0);
+    *E = e;
+    *SE = sqrt(V / N);
+  }
+}

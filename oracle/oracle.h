@@ -1,0 +1,145 @@
+/*
+ * CPU FP64 oracle of the htrdr-atmosphere null-collision core.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker the GPU library
+ * (libhtgx.so) is compared against; nothing in the product links, loads or
+ * calls it. Only tests/, __graft_entry__.smoke() and bench.py's CPU baseline
+ * leg may use it.
+ *
+ * It restates, in plain C with recursion and no GPU-oriented layout, the
+ * algorithms of the reference path (citations are file:line under the htrdr
+ * tree) and the semantics this build fixes for the un-vendored libraries
+ * (htsky property model and octree, svx traversal, star-sp samplers, star-sf
+ * phase functions, star-camera pinhole); see oracle.c for each function.
+ *
+ * Parity pins (what checks each function, tests/ file):
+ *   threefry4x64_20 ........ Random123 known-answer vectors + rocRAND's
+ *                            independent implementation (test_oracle_rng.py)
+ *   octree / segments ...... brute-force DDA over the finest cells on tiny
+ *                            grids, exact FP32 min/max (test_oracle_svo.py)
+ *   planck / wiebelt ....... Stefan-Boltzmann integral, Wien's law, numerical
+ *                            quadrature of Planck (test_oracle_spectral.py)
+ *   brightness temperature . closed-form monochromatic inverse, round trips
+ *   wavelength sampler ..... brute-force CDF on small band counts, mono case
+ *   phase / cone / lambert . chi-square against analytic pdfs, moments
+ *                            (test_oracle_samplers.py)
+ *   transmissivity ......... k_min = k_max gives exp(-tau) with zero variance;
+ *                            majorant inflation leaves the mean unchanged
+ *   SW / LW walks .......... closed forms: vacuum + Lambertian ground, purely
+ *                            absorbing slab, isothermal enclosure, optically
+ *                            thin single scattering (test_oracle_walks.py)
+ * No function of this oracle is "parity unpinned" except the bits of the
+ * reference's own random streams, which counter-based Threefry replaces by
+ * design (per-pixel statistical parity only).
+ */
+#ifndef HTGX_ORACLE_H
+#define HTGX_ORACLE_H
+
+#include "htgx.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+struct oracle_sky;
+
+/* Random streams */
+void oracle_threefry4x64_20
+  (const uint64_t ctr[4], const uint64_t key[4], uint64_t out[4]);
+/* n canonical numbers of the stream keyed by (seed, pixel, realisation) */
+void oracle_stream_canonicals
+  (uint64_t seed, uint64_t pixel, uint64_t realisation, uint32_t n, double* out);
+
+/* Scene */
+int oracle_sky_create(const struct htgx_scene_desc* scene, struct oracle_sky** sky);
+void oracle_sky_destroy(struct oracle_sky* sky);
+int oracle_sky_get_svo_info
+  (const struct oracle_sky* sky, uint32_t* nlevels, uint64_t* nnodes,
+   uint64_t* nleaves);
+
+/* Leaves/layer segments along a ray, in front-to-back order */
+int oracle_trace_segments
+  (const struct oracle_sky* sky,
+   const double ray[8], /* org[3], dir[3], range[2] */
+   int prop, uint32_t iband, uint32_t iquad,
+   struct htgx_segment* segs, uint32_t max_segs, uint32_t* nsegs);
+
+/* Leaf id and bounds (LWC min/max, g/m^3) of the octree leaf containing the
+ * cloud cell (ix,iy,iz) */
+int oracle_leaf_of_cell
+  (const struct oracle_sky* sky, const uint32_t cell[3], int64_t* leaf,
+   float* lwc_min, float* lwc_max);
+
+/* Property at a point (htsky_fetch_raw_property) */
+double oracle_fetch_raw
+  (const struct oracle_sky* sky, int prop, int cpnt /*1 gas,2 particles,3 all*/,
+   uint32_t iband, uint32_t iquad, const double pos[3]);
+
+/* Transmissivity estimator of ..._sw.c:219-251 for one realisation whose
+ * random stream is keyed by (seed, pixel, realisation). */
+double oracle_transmissivity
+  (const struct oracle_sky* sky, uint64_t seed, uint64_t pixel,
+   uint64_t realisation, int prop, uint32_t iband, uint32_t iquad,
+   const double pos[3], const double dir[3], const double range[2]);
+
+/* One realisation weight, same semantics as htgx_debug_realisations */
+double oracle_realisation
+  (const struct oracle_sky* sky, const struct htgx_map_desc* desc,
+   uint64_t pixel, uint64_t realisation);
+
+/* Radiance of one walk from a given ray (W/m^2/sr/m), wavelength in nm */
+double oracle_radiance
+  (const struct oracle_sky* sky, uint64_t seed, uint64_t pixel,
+   uint64_t realisation, int spectral_type, int cpnt_mask,
+   const double pos[3], const double dir[3], double wlen,
+   uint32_t iband, uint32_t iquad);
+
+/* The whole map into per-pixel accumulators (row-major), nthreads OpenMP
+ * threads (0 = all) */
+int oracle_draw_map
+  (const struct oracle_sky* sky, const struct htgx_map_desc* desc,
+   struct htgx_mc_accum* accum, int nthreads);
+
+/* A subset of the pixels of a map (pixel ids y*W+x) into accum[npixels] */
+int oracle_draw_pixels
+  (const struct oracle_sky* sky, const struct htgx_map_desc* desc,
+   const uint64_t* pixels, uint64_t npixels, struct htgx_mc_accum* accum,
+   int nthreads);
+
+/* Spectral helpers (src/core/htrdr_spectral.h, htrdr_spectral.c) */
+double oracle_planck_monochromatic(double lambda_m, double temperature);
+double oracle_wiebelt(double v);
+double oracle_blackbody_fraction(double lambda0_m, double lambda1_m, double T);
+double oracle_planck(double lambda_min_m, double lambda_max_m, double T);
+int oracle_brightness_temperature
+  (double lambda_min_m, double lambda_max_m, double radiance, double* T);
+double oracle_radiance_temperature
+  (double lambda_min_m, double lambda_max_m, double radiance);
+
+/* Planck wavelength sampler (src/core/htrdr_ran_wlen_planck.c) */
+struct oracle_wlen_planck;
+int oracle_wlen_planck_create
+  (const double range_nm[2], size_t nbands, double ref_temperature,
+   struct oracle_wlen_planck** out);
+void oracle_wlen_planck_destroy(struct oracle_wlen_planck* p);
+double oracle_wlen_planck_sample
+  (const struct oracle_wlen_planck* p, double r0, double r1, double* pdf_nm);
+/* #bands of the discrete CDF as htrdr_atmosphere_create computes it
+ * (src/atmosphere/htrdr_atmosphere.c:53-70) */
+size_t oracle_spectral_bands_count(const double range_nm[2]);
+
+/* Direction samplers, kind as htgx_debug_sample_dirs; the sun of `sky' */
+int oracle_sample_dirs
+  (const struct oracle_sky* sky, int kind, double param, const double dir_in[3],
+   const double* u, uint32_t n, double* out);
+double oracle_phase_eval(int kind, double param, double cos_theta);
+
+/* Per-pixel estimate (src/core/htrdr_accum.h:48-68) */
+void oracle_accum_estimate
+  (const struct htgx_mc_accum* acc, double* E, double* SE);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HTGX_ORACLE_H */
