@@ -1,9 +1,1158 @@
-/* FP32 wavefront solver (work in progress: not yet available). */
+/*
+ * FP32 batched wavefront solver: a persistent kernel whose lanes advance
+ * their own path by one micro-step per loop iteration.
+ *
+ * Structure (replaces the OpenMP tile loop of src/core/htrdr_draw_map.c:
+ * 324-491 and the per-pixel loops of ..._draw_map.c:210-436):
+ *  - work queue: chunks of K consecutive realisations of one pixel, handed
+ *    out by atomic counters, one counter per XCD over a contiguous range of
+ *    8x8 tiles (pixels of a tile in Morton order, src/core/htrdr_draw_map.c:
+ *    301-319) so that the waves of an XCD trace neighbouring pixels and share
+ *    that XCD's L2; an XCD whose range is empty takes chunks from the others;
+ *  - micro-steps: in one iteration a lane either fetches the next octree x
+ *    gas-layer segment of its current query (SEG), tests one tentative
+ *    collision (COLL) -- the null-collision loops of ..._sw.c:94-143,
+ *    174-215 and ..._lw.c:83-133 unrolled one collision at a time -- or
+ *    handles an event (EVENT: scattering, surface hit, escape, the start and
+ *    end of realisations). Lanes never wait for a slower lane's whole query;
+ *  - event regrouping: the EVENT branch is the expensive one (phase and sun
+ *    sampling, camera rays, spectral sampling); it runs only when at least
+ *    `ev_thresh` lanes of the wavefront need it, or when no lane can do
+ *    anything else, so that it executes with most of the wavefront active;
+ *  - precision: positions, directions, distances, coefficients and weights
+ *    are FP32; the k_min optical depth Tmin of the transmissivity estimator
+ *    and the per-pixel accumulators are FP64 (Neumaier compensated);
+ *  - results: every chunk writes its own partial accumulator; a merge kernel
+ *    folds them per pixel in a fixed order, so a map does not depend on the
+ *    schedule.
+ * Random numbers: the Threefry-4x64-20 stream of the realisation (key =
+ * seed, pixel, realisation), each 64-bit word giving two FP32 canonicals.
+ */
+#pragma clang fp contract(fast)
+
 #include "htgx_internal.h"
 
-int htgx_render_wavefront(htgx_ctx*, const htgx::DevMap&, htgx_mc_accum*, hipStream_t)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+using namespace htgx;
+
+#define HIP_OK(expr) do {                                                     \
+  hipError_t e_ = (expr);                                                    \
+  if(e_ != hipSuccess) {                                                     \
+    fprintf(stderr, "htgx: %s failed: %s (%s:%d)\n", #expr,                  \
+      hipGetErrorString(e_), __FILE__, __LINE__);                            \
+    return e_ == hipErrorOutOfMemory ? HTGX_MEM_ERR : HTGX_UNKNOWN_ERR;       \
+  }                                                                           \
+} while(0)
+
+namespace {
+
+constexpr float kPif = 3.14159265358979323846f;
+constexpr int kXcds = 8;
+
+/* FP32 copy of the scene constants (device arrays shared with the FP64
+ * solver where the layout is the same) */
+struct SceneF {
+  uint32_t nbands;
+  const double* band_hi;
+  const uint32_t* quad_off;
+  const double* quad_cdf;
+  uint32_t nlayers;
+  float zb, dzl, inv_dzl;
+  const float* gas_ka; /* [#quads][nlayers] */
+  const float* gas_ks; /* [nbands][nlayers] */
+  const float* gas_T;
+  int has_clouds;
+  uint32_t B;
+  uint32_t def[3];
+  uint32_t dims0[3];
+  float lower[3], bs[3], inv_bs[3], inv_cs[3];
+  const float* lwc;
+  const uint32_t* lmap;
+  const float2* bounds;
+  const float* cka; /* [nbands] cext*(1-ssa) */
+  const float* cks; /* [nbands] cext*ssa */
+  const float* hg_g;
+  int ground_type;
+  float plane_z;
+  uint32_t plane_mtl;
+  const float* mtl_albedo;
+  const float* mtl_T;
+  float sun_dir[3];
+  float sun_cos_half;
+  float sun_solid_angle;
+};
+
+struct Launch {
+  uint32_t width, height, ntx;
+  uint32_t K;            /* realisations per chunk */
+  uint32_t nsub;         /* chunks per pixel in this launch */
+  uint32_t ntiles;
+  uint32_t xcd_tiles;    /* tiles per XCD range */
+  uint32_t spp_offset;
+  uint32_t ev_thresh;    /* min #lanes needing the EVENT branch */
+  uint32_t max_steps;    /* micro-step budget of one query */
+  unsigned int* counters; /* [kXcds] next chunk of each XCD range */
+  htgx_mc_accum* partials; /* [ntiles*nsub*64] */
+  unsigned long long* stats; /* event counters */
+};
+
+enum { LS_SEG = 0, LS_COLL = 1, LS_EVENT = 2, LS_DONE = 3 };
+enum { MODE_TR = 0, MODE_SC = 1, MODE_LW = 2 };
+enum { EV_NONE = 0, EV_HIT = 1, EV_ABS = 2, EV_SCA = 3 };
+enum {
+  ST_DIRECT = 0, ST_SCATTER = 1, ST_ABS = 2, ST_SUN = 3, ST_LW = 4,
+  ST_WALK_DONE = 5, ST_REAL_DONE = 6, ST_BEGIN = 7, ST_FETCH = 8
+};
+
+/*******************************************************************************
+ * Lane state
+ ******************************************************************************/
+struct Lane {
+  /* work */
+  uint32_t chunk, pixel, r, r_end;
+  int lstate;
+  /* random stream: the words of the current Threefry block, consumed from
+   * b0 and shifted down (no dynamic indexing, so the state stays in
+   * registers) */
+  uint64_t b0, b1, b2, b3;
+  uint32_t block, idx;
+  /* realisation */
+  float pdf, g, L_sun, wlen;
+  uint32_t band, quad;
+  float org[3], dir0[3], cosN, Ld; /* flux SW */
+  uint64_t t0;
+  /* walk */
+  int stage, mask, phase;
+  bool from_ground, surface, rayleigh, gh_hit, pending;
+  uint32_t gh_mtl;
+  float pos[3], dir[3], dir_next[3];
+  float qr, ksi, w, Tr_abs, R, refl, gas_ratio;
+  /* query */
+  int mode, prop;
+  float tc, Ts;
+  double Tmin;
+  int ev;
+  float ev_dst;
+  int32_t ev_layer;
+  uint32_t nsteps;
+  /* segment iterator */
+  int cstate;          /* 0 before grid, 1 in leaf, 2 after, 3 to locate */
+  float tit, tnext;    /* iterator position; grid entry or leaf exit */
+  uint32_t lohi[3];    /* leaf block range lo | hi << 16 */
+  uint32_t leaf;
+  float lmn, lmx;
+  int exit_axis;
+  int32_t j;
+  bool itdone, adv, adv_g, adv_c, adv_end;
+  /* current segment */
+  float s_t1, s_kmin, s_kmax;
+  bool s_cloud;
+  int32_t s_layer;
+  /* chunk accumulator */
+  double sum, sum_c, sum2, sum2_c;
+  float tsum, tsum2;
+  uint32_t n, nfail;
+  /* counters */
+  uint32_t c_leaf, c_seg, c_coll, c_lwc, c_ev;
+};
+
+__device__ __forceinline__ float canonicalf(Lane& L, const uint64_t* key)
 {
-  return HTGX_BAD_OP;
+  if(L.idx == 8) {
+    const uint64_t ctr[4] = {L.block, 0, 0, 0};
+    uint64_t out[4];
+    threefry4x64_20(ctr, key, out);
+    L.b0 = out[0]; L.b1 = out[1]; L.b2 = out[2]; L.b3 = out[3];
+    L.block += 1;
+    L.idx = 0;
+  }
+  const bool low = (L.idx & 1u) != 0;
+  const uint32_t half = low ? (uint32_t)L.b0 : (uint32_t)(L.b0 >> 32);
+  if(low) { L.b0 = L.b1; L.b1 = L.b2; L.b2 = L.b3; }
+  L.idx += 1;
+  return (float)(half >> 8) * 0x1.0p-24f;
+}
+
+__device__ __forceinline__ float ran_expf(Lane& L, const uint64_t* key)
+{
+  return -__logf(1.0f - canonicalf(L, key));
+}
+
+__device__ __forceinline__ void nsum(double& sum, double& c, double x)
+{
+  const double t = sum + x;
+  if(fabs(sum) >= fabs(x)) c += (sum - t) + x;
+  else c += (x - t) + sum;
+  sum = t;
+}
+
+/*******************************************************************************
+ * FP32 geometry helpers
+ ******************************************************************************/
+__device__ __forceinline__ float dot3f(const float a[3], const float b[3])
+{
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+__device__ __forceinline__ void local_to_worldf(const float n[3], float cos_t, float phi, float out[3])
+{
+  const float sign = n[2] >= 0 ? 1.f : -1.f;
+  const float a = -1.f / (sign + n[2]);
+  const float b = n[0] * n[1] * a;
+  const float b1x = 1.f + sign * n[0] * n[0] * a, b1y = sign * b, b1z = -sign * n[0];
+  const float b2x = b, b2y = sign + n[1] * n[1] * a, b2z = -n[1];
+  const float sin_t = sqrtf(fmaxf(0.f, 1.f - cos_t * cos_t));
+  float sp, cp;
+  __sincosf(phi, &sp, &cp);
+  const float cx = cp * sin_t, cy = sp * sin_t;
+  out[0] = b1x * cx + b2x * cy + n[0] * cos_t;
+  out[1] = b1y * cx + b2y * cy + n[1] * cos_t;
+  out[2] = b1z * cx + b2z * cy + n[2] * cos_t;
+}
+
+__device__ __forceinline__ float hg_cosf(float g, float u)
+{
+  if(fabsf(g) < 1e-6f) return 1.f - 2.f * u;
+  const float t = (1.f - g * g) / (1.f - g + 2.f * g * u);
+  const float c = (1.f + g * g - t * t) / (2.f * g);
+  return fminf(1.f, fmaxf(-1.f, c));
+}
+
+__device__ __forceinline__ float hg_evalf(float g, float mu)
+{
+  const float d = 1.f + g * g - 2.f * g * mu;
+  return (1.f - g * g) / (4.f * kPif * d * sqrtf(d));
+}
+
+__device__ __forceinline__ float rayleigh_cosf(float u)
+{
+  const float q = 4.f * u - 2.f;
+  const float A = cbrtf(q + sqrtf(q * q + 1.f));
+  return fminf(1.f, fmaxf(-1.f, A - 1.f / A));
+}
+
+__device__ __forceinline__ float rayleigh_evalf(float mu)
+{
+  return 3.f / (16.f * kPif) * (1.f + mu * mu);
+}
+
+/* Plane ground with the self-hit rules of src/core/htrdr_geometry.c:116-150 */
+__device__ __forceinline__ bool ground_planef(const SceneF& S, const float o[3], const float d[3],
+                                              float r1, bool from_ground, float* t)
+{
+  if(S.ground_type != HTGX_GROUND_PLANE || from_ground || d[2] == 0.f) return false;
+  const float tt = (S.plane_z - o[2]) / d[2];
+  if(!(tt > 0.f && tt < r1)) return false;
+  *t = tt;
+  return true;
+}
+
+/*******************************************************************************
+ * Segment iterator (FP32 version of SegIter in htgx_device.h)
+ ******************************************************************************/
+__device__ __forceinline__ float planef(const SceneF& S, int a, uint32_t k)
+{
+  return S.lower[a] + (float)k * S.bs[a];
+}
+
+template<bool COUNT>
+__device__ __forceinline__ void load_leaf(const SceneF& S, Lane& L, uint32_t cx, uint32_t cy,
+                                          uint32_t cz)
+{
+  const uint32_t word = S.lmap[((size_t)cz * S.dims0[1] + cy) * S.dims0[0] + cx];
+  const uint32_t lvl = word >> kLeafIdBits;
+  L.leaf = word & kLeafIdMask;
+  const float2 b = S.bounds[L.leaf];
+  L.lmn = b.x;
+  L.lmx = b.y;
+  const uint32_t c[3] = {cx, cy, cz};
+  float best = INFINITY;
+  int ax = 0;
+#pragma unroll
+  for(int a = 0; a < 3; ++a) {
+    const uint32_t lo = (c[a] >> lvl) << lvl;
+    const uint32_t hi = min(lo + (1u << lvl), S.dims0[a]);
+    L.lohi[a] = lo | (hi << 16);
+    if(L.dir[a] != 0.f) {
+      const float ta = (planef(S, a, L.dir[a] > 0.f ? hi : lo) - L.pos[a]) / L.dir[a];
+      if(ta < best) { best = ta; ax = a; }
+    }
+  }
+  L.tnext = best;
+  L.exit_axis = ax;
+  if(COUNT) L.c_leaf++;
+}
+
+template<bool COUNT>
+__device__ __forceinline__ void locate_leaf(const SceneF& S, Lane& L, float at)
+{
+  uint32_t c[3];
+#pragma unroll
+  for(int a = 0; a < 3; ++a) {
+    const float p = L.pos[a] + at * L.dir[a];
+    const float f = floorf((p - S.lower[a]) * S.inv_bs[a]);
+    c[a] = f < 0.f ? 0u : f > (float)(S.dims0[a] - 1) ? S.dims0[a] - 1 : (uint32_t)f;
+  }
+  load_leaf<COUNT>(S, L, c[0], c[1], c[2]);
+}
+
+template<bool COUNT>
+__device__ __forceinline__ void step_leaf(const SceneF& S, Lane& L)
+{
+  uint32_t c[3];
+#pragma unroll
+  for(int a = 0; a < 3; ++a) {
+    const uint32_t lo = L.lohi[a] & 0xFFFFu, hi = L.lohi[a] >> 16;
+    if(a == L.exit_axis) {
+      if(L.dir[a] > 0.f) {
+        if(hi >= S.dims0[a]) { L.cstate = 2; return; }
+        c[a] = hi;
+      } else {
+        if(lo == 0) { L.cstate = 2; return; }
+        c[a] = lo - 1;
+      }
+    } else {
+      const float p = L.pos[a] + L.tit * L.dir[a];
+      const float f = floorf((p - S.lower[a]) * S.inv_bs[a]);
+      c[a] = f < (float)lo ? lo : f > (float)(hi - 1) ? hi - 1 : (uint32_t)f;
+    }
+  }
+  load_leaf<COUNT>(S, L, c[0], c[1], c[2]);
+}
+
+/* Start a query over [0, r1] along (pos, dir); no memory access here */
+__device__ __forceinline__ void iter_init(const SceneF& S, Lane& L, float r1)
+{
+  L.tit = 0.f;
+  L.qr = r1;
+  L.itdone = false;
+  L.adv = false;
+  L.cstate = 2;
+  L.nsteps = 0;
+  if(S.has_clouds) {
+    float tn = -INFINITY, tf = INFINITY;
+    bool miss = false;
+#pragma unroll
+    for(int a = 0; a < 3; ++a) {
+      const float plo = planef(S, a, 0), phi = planef(S, a, S.dims0[a]);
+      if(L.dir[a] != 0.f) {
+        const float ta = (plo - L.pos[a]) / L.dir[a];
+        const float tb = (phi - L.pos[a]) / L.dir[a];
+        tn = fmaxf(tn, fminf(ta, tb));
+        tf = fminf(tf, fmaxf(ta, tb));
+      } else if(L.pos[a] < plo || L.pos[a] > phi) {
+        miss = true;
+      }
+    }
+    if(!miss && fmaxf(tn, 0.f) < fminf(tf, r1)) {
+      if(tn > 0.f) { L.cstate = 0; L.tnext = tn; }
+      else L.cstate = 3;
+    }
+  }
+  const float fj = floorf((L.pos[2] - S.zb) * S.inv_dzl);
+  L.j = fj < -1.f ? -1 : fj > (float)S.nlayers ? (int32_t)S.nlayers : (int32_t)fj;
+}
+
+/* Move the iterator past a boundary at t1 (gas layer if adv_g, leaf or grid
+ * entry if adv_c) */
+template<bool COUNT>
+__device__ __forceinline__ void iter_advance(const SceneF& S, Lane& L, float t1, bool adv_g,
+                                             bool adv_c, bool at_end)
+{
+  if(t1 >= L.qr) { L.itdone = true; return; }
+  L.tit = t1;
+  if(adv_g) L.j += L.dir[2] > 0.f ? 1 : -1;
+  if(adv_c) {
+    if(L.cstate == 0) { L.cstate = 1; locate_leaf<COUNT>(S, L, t1); }
+    else if(L.cstate == 1) step_leaf<COUNT>(S, L);
+  }
+  if(at_end) L.itdone = true;
+}
+
+/* Next segment holding some medium into L.s_*; false at the end of the range.
+ * The iterator only moves past an emitted segment at the next call, so that
+ * the leaf range used to clamp the cells of its collisions stays the one of
+ * the segment. */
+template<bool COUNT>
+__device__ __forceinline__ bool iter_next(const SceneF& S, Lane& L, int prop, float* t0)
+{
+  if(L.cstate == 3) { locate_leaf<COUNT>(S, L, L.tit); L.cstate = 1; }
+  if(L.adv) {
+    L.adv = false;
+    iter_advance<COUNT>(S, L, L.s_t1, L.adv_g, L.adv_c, L.adv_end);
+  }
+  const float tmax = L.qr;
+  while(!L.itdone && L.tit < tmax) {
+    const float t = L.tit;
+    const float tc = L.cstate == 0 ? L.tnext : L.cstate == 1 ? fmaxf(L.tnext, t) : INFINITY;
+    float tg;
+    if(L.dir[2] > 0.f && L.j < (int32_t)S.nlayers) tg = (S.zb + (float)(L.j + 1) * S.dzl - L.pos[2]) / L.dir[2];
+    else if(L.dir[2] < 0.f && L.j >= 0) tg = (S.zb + (float)L.j * S.dzl - L.pos[2]) / L.dir[2];
+    else tg = INFINITY;
+    tg = fmaxf(tg, t);
+    const float t1 = fminf(fminf(tc, tg), tmax);
+    const bool at_end = tg == INFINITY && tc == INFINITY;
+    if(t1 > t) {
+      const bool in_gas = L.j >= 0 && L.j < (int32_t)S.nlayers;
+      const bool in_cloud = L.cstate == 1;
+      if(in_gas || in_cloud) {
+        float gk = 0.f, cmn = 0.f, cmx = 0.f;
+        if(in_gas) {
+          const float ka = S.gas_ka[(size_t)(S.quad_off[L.band] + L.quad) * S.nlayers + (uint32_t)L.j];
+          const float ks = S.gas_ks[(size_t)L.band * S.nlayers + (uint32_t)L.j];
+          gk = prop == HTGX_Ka ? ka : prop == HTGX_Ks ? ks : ka + ks;
+        }
+        if(in_cloud) {
+          const float ca = S.cka[L.band], cs = S.cks[L.band];
+          const float c = prop == HTGX_Ka ? ca : prop == HTGX_Ks ? cs : ca + cs;
+          cmn = c * L.lmn;
+          cmx = c * L.lmx;
+        }
+        *t0 = t;
+        L.s_t1 = t1;
+        L.s_kmin = gk + cmn;
+        L.s_kmax = gk + cmx;
+        L.s_cloud = in_cloud;
+        L.s_layer = in_gas ? L.j : -1;
+        L.adv = true;
+        L.adv_g = tg == t1;
+        L.adv_c = tc == t1;
+        L.adv_end = at_end;
+        if(COUNT) L.c_seg++;
+        return true;
+      }
+    }
+    iter_advance<COUNT>(S, L, t1, tg == t1, tc == t1, at_end);
+    if(++L.nsteps > 4096u) { L.itdone = true; break; } /* safety bound */
+  }
+  return false;
+}
+
+/* LWC of the cell holding x, clamped into the current segment's leaf */
+template<bool COUNT>
+__device__ __forceinline__ float seg_lwcf(const SceneF& S, Lane& L, const float x[3])
+{
+  uint32_t ci[3];
+#pragma unroll
+  for(int a = 0; a < 3; ++a) {
+    const uint32_t lo = (L.lohi[a] & 0xFFFFu) * S.B, hi = (L.lohi[a] >> 16) * S.B - 1;
+    const float f = floorf((x[a] - S.lower[a]) * S.inv_cs[a]);
+    ci[a] = f < (float)lo ? lo : f > (float)hi ? hi : (uint32_t)f;
+  }
+  if(COUNT) L.c_lwc++;
+  return S.lwc[((size_t)ci[2] * S.def[1] + ci[1]) * S.def[0] + ci[0]];
+}
+
+/*******************************************************************************
+ * Walk logic (EVENT branch)
+ ******************************************************************************/
+__device__ __forceinline__ void set_query(const SceneF& S, Lane& L, int mode, int prop, float r1)
+{
+  L.mode = mode;
+  L.prop = prop;
+  L.Tmin = 0.0;
+  L.pending = false;
+  L.lstate = LS_SEG;
+  iter_init(S, L, r1);
+}
+
+/* Sample the exponential optical thickness of a new query */
+__device__ __forceinline__ void begin_query(const SceneF& S, Lane& L, const uint64_t* key,
+                                            int mode, int prop, float r1)
+{
+  L.Ts = ran_expf(L, key);
+  set_query(S, L, mode, prop, r1);
+}
+
+__device__ __forceinline__ void sw_scatter_query(const SceneF& S, Lane& L, const uint64_t* key)
+{
+  float t = FLT_MAX;
+  L.gh_hit = ground_planef(S, L.pos, L.dir, FLT_MAX, L.from_ground, &t);
+  L.gh_mtl = S.plane_mtl;
+  begin_query(S, L, key, MODE_SC, HTGX_Ks, L.gh_hit ? t : FLT_MAX);
+}
+
+__device__ __forceinline__ void sw_after_direct(const SceneF& S, Lane& L, const uint64_t* key)
+{
+  if(!(L.mask & HTGX_RADIANCE_DIFFUSE)) { L.stage = ST_WALK_DONE; return; }
+  L.stage = ST_SCATTER;
+  sw_scatter_query(S, L, key);
+}
+
+__device__ __forceinline__ void sw_begin(const SceneF& S, Lane& L, const uint64_t* key, int mask,
+                                         const float p[3], const float d[3])
+{
+  L.mask = mask;
+#pragma unroll
+  for(int a = 0; a < 3; ++a) { L.pos[a] = p[a]; L.dir[a] = d[a]; }
+  L.ksi = 1.f;
+  L.w = 0.f;
+  L.from_ground = false;
+  if((mask & HTGX_RADIANCE_DIRECT) && dot3f(L.dir, S.sun_dir) >= S.sun_cos_half) {
+    float t;
+    if(!ground_planef(S, L.pos, L.dir, FLT_MAX, false, &t)) {
+      L.stage = ST_DIRECT;
+      begin_query(S, L, key, MODE_TR, HTGX_Kext, FLT_MAX);
+      return;
+    }
+  }
+  sw_after_direct(S, L, key);
+}
+
+__device__ __forceinline__ void lw_query(const SceneF& S, Lane& L, const uint64_t* key)
+{
+  float t = FLT_MAX;
+  L.gh_hit = ground_planef(S, L.pos, L.dir, FLT_MAX, L.from_ground, &t);
+  L.gh_mtl = S.plane_mtl;
+  L.stage = ST_LW;
+  begin_query(S, L, key, MODE_LW, HTGX_Kext, L.gh_hit ? t : FLT_MAX);
+}
+
+/* Next-event estimate and Russian roulette closing a SW event
+ * (..._sw.c:494-504); the next scattering query starts from the event */
+__device__ __forceinline__ void sw_accumulate(const SceneF& S, Lane& L, const uint64_t* key, float Tr)
+{
+  L.ksi *= L.Tr_abs;
+  L.w += L.ksi * L.L_sun * Tr * L.R * S.sun_solid_angle;
+  if(L.surface && canonicalf(L, key) >= L.refl) { L.stage = ST_WALK_DONE; return; }
+#pragma unroll
+  for(int a = 0; a < 3; ++a) L.dir[a] = L.dir_next[a];
+  L.stage = ST_SCATTER;
+  sw_scatter_query(S, L, key);
+}
+
+__device__ __forceinline__ float query_tr(const Lane& L)
+{
+  return L.ev == EV_HIT ? 0.f : (L.Tmin != 0.0 ? (float)exp(-L.Tmin) : 1.f);
+}
+
+/* Consume the finished query of the current stage */
+template<bool COUNT>
+__device__ __forceinline__ void consume(const SceneF& S, Lane& L, const uint64_t* key)
+{
+  switch(L.stage) {
+    case ST_DIRECT:
+      L.w = L.L_sun * query_tr(L);
+      sw_after_direct(S, L, key);
+      break;
+
+    case ST_SCATTER: {
+      const bool hit = L.ev == EV_HIT;
+      if(!L.gh_hit && !hit) { L.stage = ST_WALK_DONE; break; }
+      if(COUNT) L.c_ev++;
+      L.surface = !hit;
+      const float dst = L.surface ? L.qr : L.ev_dst;
+      L.from_ground = L.surface;
+      L.stage = ST_ABS;
+      /* the absorption query runs from pos along dir up to the event */
+      begin_query(S, L, key, MODE_TR, HTGX_Ka, dst);
+      L.ev_dst = dst;
+    } break;
+
+    case ST_ABS: {
+      L.Tr_abs = query_tr(L);
+      if(L.Tr_abs <= 0.f) { L.stage = ST_WALK_DONE; break; }
+      const float dst = L.ev_dst;
+      float N[3] = {0.f, 0.f, 1.f};
+      /* move to the event */
+#pragma unroll
+      for(int a = 0; a < 3; ++a) L.pos[a] = L.pos[a] + L.dir[a] * dst;
+      if(L.surface) {
+        if(-(L.dir[2]) < 0.f) N[2] = -1.f; /* plane normal faces the incoming ray */
+        const float albedo = S.mtl_albedo[L.gh_mtl];
+        const float u0 = canonicalf(L, key), u1 = canonicalf(L, key);
+        local_to_worldf(N, sqrtf(1.f - u0), 2.f * kPif * u1, L.dir_next);
+        L.refl = albedo;
+      } else {
+        L.rayleigh = canonicalf(L, key) < L.gas_ratio;
+        const float u0 = canonicalf(L, key), u1 = canonicalf(L, key);
+        const float mu = L.rayleigh ? rayleigh_cosf(u0) : hg_cosf(L.g, u0);
+        local_to_worldf(L.dir, mu, 2.f * kPif * u1, L.dir_next);
+        L.refl = 1.f;
+      }
+      float sun_dir[3];
+      {
+        const float u0 = canonicalf(L, key), u1 = canonicalf(L, key);
+        const float cos_t = 1.f - u0 * (1.f - S.sun_cos_half);
+        local_to_worldf(S.sun_dir, cos_t, 2.f * kPif * u1, sun_dir);
+      }
+      if(L.surface) {
+        const float c = dot3f(N, sun_dir);
+        L.R = c < 0.f ? 0.f : S.mtl_albedo[L.gh_mtl] / kPif * c;
+      } else {
+        const float mu = dot3f(L.dir, sun_dir);
+        L.R = L.rayleigh ? rayleigh_evalf(mu) : hg_evalf(L.g, mu);
+      }
+      float tg;
+      if(L.R <= 0.f || ground_planef(S, L.pos, sun_dir, FLT_MAX, L.from_ground, &tg)) {
+        sw_accumulate(S, L, key, 0.f);
+        break;
+      }
+#pragma unroll
+      for(int a = 0; a < 3; ++a) L.dir[a] = sun_dir[a];
+      L.stage = ST_SUN;
+      begin_query(S, L, key, MODE_TR, HTGX_Kext, FLT_MAX);
+    } break;
+
+    case ST_SUN:
+      sw_accumulate(S, L, key, query_tr(L));
+      break;
+
+    case ST_LW: {
+      if(!L.gh_hit && L.ev == EV_NONE) { L.w = 0.f; L.stage = ST_WALK_DONE; break; }
+      if(COUNT) L.c_ev++;
+      const float dst = L.ev != EV_NONE ? L.ev_dst : L.qr;
+#pragma unroll
+      for(int a = 0; a < 3; ++a) L.pos[a] = L.pos[a] + L.dir[a] * dst;
+      if(L.ev == EV_ABS) {
+        const int32_t nl = (int32_t)S.nlayers;
+        const int32_t jj = L.ev_layer < 0 ? 0 : L.ev_layer >= nl ? nl - 1 : L.ev_layer;
+        L.w = (float)planck_mono((double)L.wlen * 1e-9, (double)S.gas_T[jj]);
+        L.stage = ST_WALK_DONE;
+        break;
+      }
+      float dn[3];
+      if(L.ev == EV_SCA) {
+        const float u0 = canonicalf(L, key), u1 = canonicalf(L, key);
+        local_to_worldf(L.dir, hg_cosf(L.g, u0), 2.f * kPif * u1, dn);
+        L.from_ground = false;
+      } else {
+        float N[3] = {0.f, 0.f, -(L.dir[2]) < 0.f ? -1.f : 1.f};
+        const float u0 = canonicalf(L, key), u1 = canonicalf(L, key);
+        local_to_worldf(N, sqrtf(1.f - u0), 2.f * kPif * u1, dn);
+        if(canonicalf(L, key) >= S.mtl_albedo[L.gh_mtl]) {
+          const float T = S.mtl_T ? S.mtl_T[L.gh_mtl] : 0.f;
+          L.w = T > 0.f ? (float)planck_mono((double)L.wlen * 1e-9, (double)T) : 0.f;
+          L.stage = ST_WALK_DONE;
+          break;
+        }
+        L.from_ground = true;
+      }
+#pragma unroll
+      for(int a = 0; a < 3; ++a) L.dir[a] = dn[a];
+      lw_query(S, L, key);
+    } break;
+
+    default: break;
+  }
+}
+
+/* Sensor ray, wavelength, band and k-point, then the first walk
+ * (..._draw_map.c:348-400 and :233-302) */
+template<int SENSOR, int SPECTRAL>
+__device__ __forceinline__ void real_begin(const SceneF& S, const DevMap& M, Lane& L, const uint64_t* key)
+{
+  L.block = 0;
+  L.idx = 8;
+  const uint32_t px = L.pixel % M.width, py = L.pixel / M.width;
+  float org[3], dir[3];
+  if(SENSOR == HTGX_SENSOR_CAMERA) {
+    const float fx = ((float)px + canonicalf(L, key)) / (float)M.width;
+    const float fy = ((float)py + canonicalf(L, key)) / (float)M.height;
+    (void)canonicalf(L, key); (void)canonicalf(L, key); /* lens sample */
+    const float sx = (2.f * fx - 1.f) * (float)(M.tan_half * M.aspect);
+    const float sy = (1.f - 2.f * fy) * (float)M.tan_half;
+#pragma unroll
+    for(int a = 0; a < 3; ++a) {
+      dir[a] = (float)M.z[a] + (float)M.x[a] * sx + (float)M.y[a] * sy;
+      org[a] = (float)M.pos[a];
+    }
+    const float inv = rsqrtf(dot3f(dir, dir));
+#pragma unroll
+    for(int a = 0; a < 3; ++a) dir[a] *= inv;
+  } else {
+    const float sx = ((float)px + canonicalf(L, key)) / (float)M.width;
+    const float sy = ((float)py + canonicalf(L, key)) / (float)M.height;
+#pragma unroll
+    for(int a = 0; a < 3; ++a)
+      org[a] = (float)M.pos[a] + (float)M.axis_x[a] * (sx * 2.f - 1.f)
+             + (float)M.axis_y[a] * (sy * 2.f - 1.f);
+    const float up[3] = {0.f, 0.f, 1.f};
+    float t;
+    /* an up ray hitting the (upward facing) ground from below starts inside */
+    if(ground_planef(S, org, up, FLT_MAX, false, &t)) { L.stage = ST_REAL_DONE; L.nfail++; L.w = -1.f; return; }
+    const float u0 = canonicalf(L, key), u1 = canonicalf(L, key);
+    const float z[3] = {(float)M.z[0], (float)M.z[1], (float)M.z[2]};
+    local_to_worldf(z, sqrtf(1.f - u0), 2.f * kPif * u1, dir);
+  }
+
+  const double r0 = canonicalf(L, key), r1 = canonicalf(L, key), r2 = canonicalf(L, key);
+  double pdf;
+  const double wlen = sample_wavelength(M, r0, r1, &pdf);
+  L.pdf = (float)(pdf * 1e9);
+  L.wlen = (float)wlen;
+  {
+    uint32_t lo = 0, hi = S.nbands - 1;
+    while(lo < hi) { const uint32_t mid = (lo + hi) >> 1; if(S.band_hi[mid] > wlen) hi = mid; else lo = mid + 1; }
+    L.band = lo;
+    const uint32_t off = S.quad_off[lo], nq = S.quad_off[lo + 1] - off;
+    uint32_t q = 0;
+    for(; q + 1 < nq; ++q) if(S.quad_cdf[off + q] > r2) break;
+    L.quad = q;
+  }
+  L.g = S.has_clouds ? S.hg_g[L.band] : 0.f;
+  L.L_sun = (float)planck_mono(wlen * 1e-9, 5778.0);
+
+  if(SPECTRAL == HTGX_SPECTRAL_LW) {
+    L.phase = 0;
+#pragma unroll
+    for(int a = 0; a < 3; ++a) { L.pos[a] = org[a]; L.dir[a] = dir[a]; }
+    L.from_ground = false;
+    L.w = 0.f;
+    lw_query(S, L, key);
+  } else if(SENSOR == HTGX_SENSOR_CAMERA) {
+    L.phase = 0;
+    sw_begin(S, L, key, HTGX_RADIANCE_ALL, org, dir);
+  } else {
+    float sun_dir[3];
+    const float u0 = canonicalf(L, key), u1 = canonicalf(L, key);
+    local_to_worldf(S.sun_dir, 1.f - u0 * (1.f - S.sun_cos_half), 2.f * kPif * u1, sun_dir);
+    const float z[3] = {(float)M.z[0], (float)M.z[1], (float)M.z[2]};
+    L.cosN = dot3f(z, sun_dir);
+    L.Ld = 0.f;
+#pragma unroll
+    for(int a = 0; a < 3; ++a) { L.org[a] = org[a]; L.dir0[a] = dir[a]; }
+    if(L.cosN > 0.f) { L.phase = 1; sw_begin(S, L, key, HTGX_RADIANCE_DIRECT, org, sun_dir); }
+    else { L.phase = 2; sw_begin(S, L, key, HTGX_RADIANCE_DIFFUSE, org, dir); }
+  }
+}
+
+template<int SENSOR, int SPECTRAL>
+__device__ __forceinline__ void walk_done(const SceneF& S, Lane& L, const uint64_t* key)
+{
+  if(SENSOR == HTGX_SENSOR_CAMERA) {
+    L.stage = ST_REAL_DONE;
+    L.w = L.w / L.pdf;
+  } else if(SPECTRAL == HTGX_SPECTRAL_LW) {
+    L.stage = ST_REAL_DONE;
+    L.w = L.w * kPif / L.pdf;
+  } else if(L.phase == 1) {
+    L.Ld = L.w;
+    L.phase = 2;
+    sw_begin(S, L, key, HTGX_RADIANCE_DIFFUSE, L.org, L.dir0);
+  } else {
+    L.stage = ST_REAL_DONE;
+    L.w = (L.cosN * S.sun_solid_angle * L.Ld + kPif * L.w) / L.pdf;
+  }
+}
+
+/* Next chunk from this XCD's tile range, else from the others */
+__device__ __forceinline__ bool fetch_chunk(const Launch& P, uint32_t xcd, uint32_t* chunk)
+{
+  const uint32_t per_tile = P.nsub * 64u;
+  for(uint32_t k = 0; k < (uint32_t)kXcds; ++k) {
+    const uint32_t x = (xcd + k) % (uint32_t)kXcds;
+    const uint32_t t0 = x * P.xcd_tiles;
+    const uint32_t t1 = min(t0 + P.xcd_tiles, P.ntiles);
+    if(t0 >= t1) continue;
+    const uint32_t n = (t1 - t0) * per_tile;
+    const uint32_t c = atomicAdd(P.counters + x, 1u);
+    if(c < n) { *chunk = t0 * per_tile + c; return true; }
+  }
+  return false;
+}
+
+__device__ __forceinline__ uint64_t wave_sum64(uint64_t v)
+{
+  for(int off = 32; off > 0; off >>= 1) v += (uint64_t)__shfl_xor((unsigned long long)v, off, 64);
+  return v;
+}
+
+template<int SENSOR, int SPECTRAL, bool COUNT>
+__global__ void __launch_bounds__(256)
+wavefront_kernel(SceneF S, DevMap M, Launch P)
+{
+  Lane L;
+  L.lstate = LS_EVENT;
+  L.stage = ST_FETCH;
+  L.pending = false;
+  L.c_leaf = L.c_seg = L.c_coll = L.c_lwc = L.c_ev = 0;
+  L.n = L.nfail = 0;
+  L.sum = L.sum_c = L.sum2 = L.sum2_c = 0.0;
+  L.tsum = L.tsum2 = 0.f;
+  const uint32_t xcd = blockIdx.x % (uint32_t)kXcds;
+  uint64_t key[4] = {M.seed, 0, 0, 0};
+
+  for(;;) {
+    const uint64_t busy = __ballot(L.lstate == LS_SEG || L.lstate == LS_COLL);
+    const uint64_t evts = __ballot(L.lstate == LS_EVENT);
+    if(!(busy | evts)) break;
+
+    /* tentative collision in the current segment */
+    if(L.lstate == LS_COLL) {
+      const float maj = L.mode == MODE_TR ? L.s_kmax - L.s_kmin : L.s_kmax;
+      const float T = (L.s_t1 - L.tc) * maj;
+      if(L.Ts > T || !(maj > 0.f)) {
+        L.Ts -= T;
+        L.lstate = LS_SEG;
+      } else {
+        L.tc += L.Ts / maj;
+        if(L.mode == MODE_SC && L.tc > 1e9f) {
+          L.lstate = LS_SEG;
+        } else {
+          if(COUNT) L.c_coll++;
+          float x[3];
+#pragma unroll
+          for(int a = 0; a < 3; ++a) x[a] = L.pos[a] + L.tc * L.dir[a];
+          const float v = L.s_cloud ? seg_lwcf<COUNT>(S, L, x) : 0.f;
+          const bool gas = L.s_layer >= 0;
+          const float gka = gas ? S.gas_ka[(size_t)(S.quad_off[L.band] + L.quad) * S.nlayers + (uint32_t)L.s_layer] : 0.f;
+          const float gks = gas ? S.gas_ks[(size_t)L.band * S.nlayers + (uint32_t)L.s_layer] : 0.f;
+          const float ka = gka + S.cka[L.band] * v;
+          const float ks = gks + S.cks[L.band] * v;
+          const float u = canonicalf(L, key);
+          if(L.mode == MODE_LW) {
+            const float pa = ka / maj, ps = ks / maj;
+            if(u < pa) { L.ev = EV_ABS; }
+            else if(u < pa + ps) { L.ev = EV_SCA; }
+            else L.ev = EV_NONE;
+          } else {
+            const float k = L.prop == HTGX_Ka ? ka : L.prop == HTGX_Ks ? ks : ka + ks;
+            const float proba = L.mode == MODE_TR ? (k - L.s_kmin) / maj : k / maj;
+            L.ev = u < proba ? EV_HIT : EV_NONE;
+            /* gas vs particle scattering at the event, ..._sw.c:427-438 */
+            if(L.mode == MODE_SC && L.ev == EV_HIT) L.gas_ratio = gks / ks;
+          }
+          if(L.ev != EV_NONE) {
+            L.ev_dst = L.tc;
+            L.ev_layer = L.s_layer;
+            L.pending = true;
+            L.lstate = LS_EVENT;
+          } else if(++L.nsteps > P.max_steps) {
+            /* micro-step budget of a query exhausted: end it without event */
+            L.ev = EV_NONE;
+            L.pending = true;
+            L.lstate = LS_EVENT;
+          } else {
+            L.Ts = ran_expf(L, key);
+          }
+        }
+      }
+    }
+
+    /* next segment of the current query */
+    if(L.lstate == LS_SEG) {
+      float t0;
+      const int sprop = L.mode == MODE_SC ? HTGX_Ks : L.mode == MODE_LW ? HTGX_Kext : L.prop;
+      if(iter_next<COUNT>(S, L, sprop, &t0)) {
+        if(L.mode == MODE_TR) L.Tmin += (double)(L.s_t1 - t0) * (double)L.s_kmin;
+        L.tc = t0;
+        L.lstate = LS_COLL;
+      } else {
+        L.ev = EV_NONE;
+        L.pending = true;
+        L.lstate = LS_EVENT;
+      }
+    }
+
+    /* events, when enough lanes need them or nothing else is left */
+    const uint64_t want = __ballot(L.lstate == LS_EVENT);
+    const uint64_t other = __ballot(L.lstate == LS_SEG || L.lstate == LS_COLL);
+    if(want && (__popcll(want) >= P.ev_thresh || !other)) {
+      if(L.lstate == LS_EVENT) {
+        for(int guard = 0; guard < 64; ++guard) {
+          if(L.stage == ST_FETCH) {
+            uint32_t c;
+            if(!fetch_chunk(P, xcd, &c)) { L.lstate = LS_DONE; break; }
+            L.chunk = c;
+            const uint32_t m = c & 63u, rest = c >> 6;
+            const uint32_t s = rest % P.nsub, t = rest / P.nsub;
+            const uint32_t x = (t % P.ntx) * 8u + ((m & 1u) | ((m >> 1) & 2u) | ((m >> 2) & 4u));
+            const uint32_t y = (t / P.ntx) * 8u + (((m >> 1) & 1u) | ((m >> 2) & 2u) | ((m >> 3) & 4u));
+            if(x >= P.width || y >= P.height) continue; /* outside the image */
+            L.pixel = y * P.width + x;
+            L.r = P.spp_offset + s * P.K;
+            L.r_end = L.r + P.K;
+            L.n = L.nfail = 0;
+            L.sum = L.sum_c = L.sum2 = L.sum2_c = 0.0;
+            L.tsum = L.tsum2 = 0.f;
+            L.stage = ST_BEGIN;
+          }
+          if(L.stage == ST_BEGIN) {
+            key[1] = L.pixel;
+            key[2] = L.r;
+            L.t0 = wall_clock64();
+            real_begin<SENSOR, SPECTRAL>(S, M, L, key);
+          } else if(L.pending) {
+            key[1] = L.pixel;
+            key[2] = L.r;
+            consume<COUNT>(S, L, key);
+          }
+          while(L.stage == ST_WALK_DONE) walk_done<SENSOR, SPECTRAL>(S, L, key);
+          if(L.stage == ST_REAL_DONE) {
+            const uint64_t t1 = wall_clock64();
+            if(L.w >= 0.f) {
+              const double wd = (double)L.w;
+              const float us = (float)((double)(t1 - L.t0) * M.us_per_tick);
+              nsum(L.sum, L.sum_c, wd);
+              nsum(L.sum2, L.sum2_c, wd * wd);
+              L.tsum += us;
+              L.tsum2 += us * us;
+              L.n++;
+            }
+            L.r++;
+            if(L.r < L.r_end) { L.stage = ST_BEGIN; continue; }
+            htgx_mc_accum a;
+            a.sum = L.sum; a.sum_c = L.sum_c; a.sum2 = L.sum2; a.sum2_c = L.sum2_c;
+            a.tsum = L.tsum; a.tsum2 = L.tsum2; a.n = L.n; a.nfail = L.nfail;
+            P.partials[L.chunk] = a;
+            L.stage = ST_FETCH;
+            continue;
+          }
+          break; /* a query stage: tracking resumes next iteration */
+        }
+      }
+    }
+  }
+
+  if(COUNT) {
+    const uint64_t a = wave_sum64(L.c_leaf), b = wave_sum64(L.c_seg), c = wave_sum64(L.c_coll),
+                   d = wave_sum64(L.c_lwc), e = wave_sum64(L.c_ev);
+    if((threadIdx.x & 63u) == 0) {
+      atomicAdd(P.stats + 1, (unsigned long long)a);
+      atomicAdd(P.stats + 2, (unsigned long long)b);
+      atomicAdd(P.stats + 3, (unsigned long long)c);
+      atomicAdd(P.stats + 4, (unsigned long long)d);
+      atomicAdd(P.stats + 5, (unsigned long long)e);
+    }
+  }
+}
+
+/* Fold the chunks of each pixel, in chunk order, into its accumulator */
+__global__ void __launch_bounds__(256)
+wf_merge_kernel(const htgx_mc_accum* partials, uint32_t width, uint32_t height, uint32_t ntx,
+                uint32_t nsub, htgx_mc_accum* accum)
+{
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if(p >= width * height) return;
+  const uint32_t x = p % width, y = p / width;
+  const uint32_t t = (y / 8u) * ntx + (x / 8u);
+  const uint32_t ix = x & 7u, iy = y & 7u;
+  const uint32_t m = (ix & 1u) | ((iy & 1u) << 1) | ((ix & 2u) << 1) | ((iy & 2u) << 2)
+                   | ((ix & 4u) << 2) | ((iy & 4u) << 3);
+  htgx_mc_accum a = accum[p];
+  for(uint32_t s = 0; s < nsub; ++s) {
+    const htgx_mc_accum q = partials[((size_t)t * nsub + s) * 64u + m];
+    nsum(a.sum, a.sum_c, q.sum);
+    a.sum_c += q.sum_c;
+    nsum(a.sum2, a.sum2_c, q.sum2);
+    a.sum2_c += q.sum2_c;
+    a.tsum += q.tsum;
+    a.tsum2 += q.tsum2;
+    a.n += q.n;
+    a.nfail += q.nfail;
+  }
+  accum[p] = a;
+}
+
+__global__ void
+zero_u32_kernel(unsigned int* p, uint32_t n)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < n) p[i] = 0u;
+}
+
+__global__ void
+zero_u64_kernel(unsigned long long* p, uint32_t n)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < n) p[i] = 0ull;
+}
+
+template<class F>
+int launch_variant(F kernel, const SceneF& S, const DevMap& M, const Launch& P, uint32_t grid,
+                   hipStream_t stream)
+{
+  kernel<<<grid, 256, 0, stream>>>(S, M, P);
+  HIP_OK(hipGetLastError());
+  return HTGX_OK;
+}
+
+template<bool COUNT>
+int launch(const SceneF& S, const DevMap& M, const Launch& P, uint32_t grid, hipStream_t stream)
+{
+  if(M.sensor == HTGX_SENSOR_CAMERA) {
+    if(M.spectral_type == HTGX_SPECTRAL_LW)
+      return launch_variant(wavefront_kernel<HTGX_SENSOR_CAMERA, HTGX_SPECTRAL_LW, COUNT>, S, M, P, grid, stream);
+    return launch_variant(wavefront_kernel<HTGX_SENSOR_CAMERA, HTGX_SPECTRAL_SW, COUNT>, S, M, P, grid, stream);
+  }
+  if(M.spectral_type == HTGX_SPECTRAL_LW)
+    return launch_variant(wavefront_kernel<HTGX_SENSOR_RECTANGLE, HTGX_SPECTRAL_LW, COUNT>, S, M, P, grid, stream);
+  return launch_variant(wavefront_kernel<HTGX_SENSOR_RECTANGLE, HTGX_SPECTRAL_SW, COUNT>, S, M, P, grid, stream);
+}
+
+} /* namespace */
+
+/*******************************************************************************
+ * Host side
+ ******************************************************************************/
+struct htgx_wavefront_state {
+  SceneF S;
+  std::vector<void*> bufs;
+  unsigned int* d_counters = nullptr;
+  void* d_partials = nullptr;
+  size_t partials_cap = 0;
+  uint32_t grid = 0;
+  uint32_t K = 4;
+  uint32_t ev_thresh = 16;
+};
+
+template<class T>
+static int upload_f(htgx_wavefront_state* st, const std::vector<T>& v, const T** out)
+{
+  void* d = nullptr;
+  if(v.empty()) { *out = nullptr; return HTGX_OK; }
+  HIP_OK(hipMalloc(&d, v.size() * sizeof(T)));
+  st->bufs.push_back(d);
+  HIP_OK(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  *out = (const T*)d;
+  return HTGX_OK;
+}
+
+template<class T>
+static std::vector<float> dev_to_float(const T* dptr, size_t n)
+{
+  std::vector<T> h(n);
+  std::vector<float> f(n);
+  if(n && hipMemcpy(h.data(), dptr, n * sizeof(T), hipMemcpyDeviceToHost) != hipSuccess) return {};
+  for(size_t i = 0; i < n; ++i) f[i] = (float)h[i];
+  return f;
+}
+
+static int
+wavefront_prepare(htgx_ctx* ctx)
+{
+  if(ctx->wf) return HTGX_OK;
+  htgx_wavefront_state* st = new htgx_wavefront_state();
+  ctx->wf = st;
+  const DevScene& D = ctx->S;
+  SceneF& S = st->S;
+  std::memset(&S, 0, sizeof(S));
+  int res;
+  S.nbands = D.nbands;
+  S.band_hi = D.band_hi;
+  S.quad_off = D.quad_off;
+  S.quad_cdf = D.quad_cdf;
+  S.nlayers = D.nlayers;
+  S.zb = (float)D.zb;
+  S.dzl = (float)D.dzl;
+  S.inv_dzl = (float)D.inv_dzl;
+  std::vector<uint32_t> qoff(D.nbands + 1);
+  HIP_OK(hipMemcpy(qoff.data(), D.quad_off, qoff.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  const uint32_t nq = qoff[D.nbands];
+  if((res = upload_f(st, dev_to_float(D.gas_ka, (size_t)nq * D.nlayers), &S.gas_ka))) return res;
+  if((res = upload_f(st, dev_to_float(D.gas_ks, (size_t)D.nbands * D.nlayers), &S.gas_ks))) return res;
+  if(D.gas_T && (res = upload_f(st, dev_to_float(D.gas_T, D.nlayers), &S.gas_T))) return res;
+  S.has_clouds = D.has_clouds;
+  if(D.has_clouds) {
+    S.B = D.B;
+    for(int a = 0; a < 3; ++a) {
+      S.def[a] = D.def[a];
+      S.dims0[a] = D.dims0[a];
+      S.lower[a] = (float)D.lower[a];
+      S.bs[a] = (float)D.bs[a];
+      S.inv_bs[a] = (float)D.inv_bs[a];
+      S.inv_cs[a] = (float)D.inv_cs[a];
+      if(D.dims0[a] > 0xFFFFu) return HTGX_BAD_ARG;
+    }
+    S.lwc = D.lwc;
+    S.lmap = D.lmap;
+    S.bounds = D.bounds;
+    std::vector<double> ce(D.nbands), ssa(D.nbands), g(D.nbands);
+    HIP_OK(hipMemcpy(ce.data(), D.cext, D.nbands * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(ssa.data(), D.ssa, D.nbands * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(g.data(), D.hg_g, D.nbands * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<float> ca(D.nbands), cs(D.nbands), gf(D.nbands);
+    for(uint32_t b = 0; b < D.nbands; ++b) {
+      ca[b] = (float)(ce[b] * (1.0 - ssa[b]));
+      cs[b] = (float)(ce[b] * ssa[b]);
+      gf[b] = (float)g[b];
+    }
+    if((res = upload_f(st, ca, &S.cka))) return res;
+    if((res = upload_f(st, cs, &S.cks))) return res;
+    if((res = upload_f(st, gf, &S.hg_g))) return res;
+  }
+  S.ground_type = D.ground_type;
+  S.plane_z = (float)D.plane_z;
+  S.plane_mtl = D.plane_mtl;
+  if(D.ground_type == HTGX_GROUND_PLANE) {
+    const uint32_t nm = D.plane_mtl + 1;
+    if((res = upload_f(st, dev_to_float(D.mtl_albedo, nm), &S.mtl_albedo))) return res;
+    if(D.mtl_T && (res = upload_f(st, dev_to_float(D.mtl_T, nm), &S.mtl_T))) return res;
+  }
+  for(int a = 0; a < 3; ++a) S.sun_dir[a] = (float)D.sun_dir[a];
+  S.sun_cos_half = (float)D.sun_cos_half;
+  S.sun_solid_angle = (float)D.sun_solid_angle;
+
+  HIP_OK(hipMalloc(&st->d_counters, kXcds * sizeof(unsigned int)));
+  /* persistent grid: as many resident blocks as the device holds */
+  int dev = 0, ncu = 0, per_cu = 0;
+  HIP_OK(hipGetDevice(&dev));
+  HIP_OK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu,
+    wavefront_kernel<HTGX_SENSOR_CAMERA, HTGX_SPECTRAL_SW, false>, 256, 0));
+  st->grid = (uint32_t)std::max(1, ncu * std::max(1, per_cu));
+  if(const char* e = std::getenv("HTGX_WF_GRID")) st->grid = (uint32_t)std::max(1, atoi(e));
+  if(const char* e = std::getenv("HTGX_WF_EVTHRESH")) st->ev_thresh = (uint32_t)std::max(1, atoi(e));
+  if(const char* e = std::getenv("HTGX_WF_CHUNK")) st->K = (uint32_t)std::max(1, atoi(e));
+  return HTGX_OK;
+}
+
+int htgx_render_wavefront(htgx_ctx* ctx, const DevMap& M, htgx_mc_accum* accum, hipStream_t stream)
+{
+  int res;
+  if((res = wavefront_prepare(ctx)) != HTGX_OK) return res;
+  htgx_wavefront_state* st = ctx->wf;
+  Launch P;
+  std::memset(&P, 0, sizeof(P));
+  P.width = M.width;
+  P.height = M.height;
+  P.ntx = (M.width + 7) / 8;
+  P.ntiles = P.ntx * ((M.height + 7) / 8);
+  /* chunk size: the largest divisor of spp not above K */
+  uint32_t K = std::min(st->K, M.spp);
+  while(M.spp % K) --K;
+  P.K = K;
+  P.nsub = M.spp / K;
+  P.xcd_tiles = (P.ntiles + kXcds - 1) / kXcds;
+  P.spp_offset = M.spp_offset;
+  P.ev_thresh = st->ev_thresh;
+  P.max_steps = 1u << 20;
+  const uint64_t nchunks = (uint64_t)P.ntiles * P.nsub * 64u;
+  if(nchunks > 0xFFFFFFFFull) return HTGX_BAD_ARG;
+  const size_t need = nchunks * sizeof(htgx_mc_accum);
+  if(need > st->partials_cap) {
+    if(st->d_partials) { HIP_OK(hipFree(st->d_partials)); st->d_partials = nullptr; }
+    HIP_OK(hipMalloc(&st->d_partials, need));
+    st->partials_cap = need;
+  }
+  P.counters = st->d_counters;
+  P.partials = (htgx_mc_accum*)st->d_partials;
+  P.stats = ctx->d_counters;
+  zero_u32_kernel<<<1, 64, 0, stream>>>(st->d_counters, kXcds);
+  HIP_OK(hipGetLastError());
+  if(ctx->counting) {
+    zero_u64_kernel<<<1, 64, 0, stream>>>(ctx->d_counters, 8);
+    HIP_OK(hipGetLastError());
+    if((res = launch<true>(st->S, M, P, st->grid, stream))) return res;
+  } else {
+    if((res = launch<false>(st->S, M, P, st->grid, stream))) return res;
+  }
+  const uint32_t npix = M.width * M.height;
+  wf_merge_kernel<<<(npix + 255) / 256, 256, 0, stream>>>(
+    (const htgx_mc_accum*)st->d_partials, M.width, M.height, P.ntx, P.nsub, accum);
+  HIP_OK(hipGetLastError());
+  ctx->last_nrealisations = (uint64_t)npix * M.spp;
+  return HTGX_OK;
 }
 
 int htgx_debug_realisations_fp32(htgx_ctx*, const htgx_map_desc*, const uint64_t*,
@@ -12,4 +1161,12 @@ int htgx_debug_realisations_fp32(htgx_ctx*, const htgx_map_desc*, const uint64_t
   return HTGX_BAD_OP;
 }
 
-void htgx_wavefront_release(htgx_ctx*) {}
+void htgx_wavefront_release(htgx_ctx* ctx)
+{
+  if(!ctx || !ctx->wf) return;
+  for(void* p : ctx->wf->bufs) (void)hipFree(p);
+  if(ctx->wf->d_counters) (void)hipFree(ctx->wf->d_counters);
+  if(ctx->wf->d_partials) (void)hipFree(ctx->wf->d_partials);
+  delete ctx->wf;
+  ctx->wf = nullptr;
+}

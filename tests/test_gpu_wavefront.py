@@ -1,0 +1,65 @@
+"""FP32 batched wavefront solver against the FP64 references.
+
+The FP32 solver draws its own random numbers (two FP32 canonicals per
+Threefry word), so it is compared statistically: per-pixel differences in
+units of the combined Monte Carlo standard error, their mean and spread, and
+the image mean. Tolerances are stated in each test.
+"""
+import numpy as np
+import pytest
+
+from htrdr_kmin_amd import _ffi as F
+from htrdr_kmin_amd import atmosphere as A
+
+pytestmark = pytest.mark.gpu
+
+
+def zscores(a, b):
+    Ea, Sa = a
+    Eb, Sb = b
+    s = np.sqrt(Sa ** 2 + Sb ** 2)
+    m = s > 0
+    z = np.zeros_like(Ea)
+    z[m] = (Ea[m] - Eb[m]) / s[m]
+    return z, m
+
+
+def test_wavefront_vs_plain_cfg0(ctx0):
+    """cfg0 at 64 spp: FP32 wavefront vs FP64 plain. Bars: >= 99% of pixels
+    within 3 combined sigma (heavy-tailed per-pixel estimates at 64 spp), the
+    z-scores have |mean| < 0.1 and std < 1.2, the image means agree within 3
+    sigma of their difference."""
+    d64 = A.map_preset(0, spp=64, solver=F.SOLVER_PLAIN_FP64)
+    d32 = A.map_preset(0, spp=64, solver=F.SOLVER_WAVEFRONT_FP32)
+    p64 = ctx0.draw_map(d64)
+    p32 = ctx0.draw_map(d32)
+    assert np.all(p32["time_n"] == 64)
+    z, m = zscores((p32["radiance_E"], p32["radiance_SE"]), (p64["radiance_E"], p64["radiance_SE"]))
+    within = np.abs(z[m]) <= 3
+    print("within 3 sigma", within.mean(), "z mean", z[m].mean(), "z std", z[m].std())
+    assert within.mean() >= 0.99
+    assert abs(z[m].mean()) < 0.1
+    assert z[m].std() < 1.2
+    d = p32["radiance_E"].mean() - p64["radiance_E"].mean()
+    sd = np.sqrt((p32["radiance_SE"] ** 2).sum() + (p64["radiance_SE"] ** 2).sum()) / p32.size
+    assert abs(d) <= 3 * sd
+
+
+def test_wavefront_slices_deterministic(ctx0):
+    """The chunk partials are merged in a fixed order: two identical runs give
+    bit-identical maps whatever the schedule of the persistent kernel."""
+    d = A.map_preset(0, width=40, height=24, spp=8, solver=F.SOLVER_WAVEFRONT_FP32)
+    a = ctx0.draw_map(d)
+    b = ctx0.draw_map(d)
+    assert np.array_equal(a["radiance_E"], b["radiance_E"])
+    assert np.array_equal(a["radiance_SE"], b["radiance_SE"])
+
+
+def test_wavefront_counters(ctx0):
+    d = A.map_preset(0, width=32, height=32, spp=8, solver=F.SOLVER_WAVEFRONT_FP32)
+    ctx0.set_counting(True)
+    ctx0.draw_map(d)
+    c = ctx0.counters()
+    ctx0.set_counting(False)
+    assert c["nrealisations"] == 32 * 32 * 8
+    assert c["nleaf"] > 0 and c["ncoll"] > 0 and c["nevents"] > 0
