@@ -33,6 +33,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "Mrealisations/s & time-to-0.5%-std-error, FP32 batched vs FP64 plain"
+SOLVER_NAMES = {0: "FP64 plain", 1: "FP32 wavefront"}
 HBM_PEAK_GBS = 8000.0
 
 
@@ -42,13 +43,32 @@ def parse():
     p.add_argument("--steps", type=int, default=16)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--config", type=int, default=1)
-    p.add_argument("--solver", choices=("plain", "wavefront"), default="plain")
+    p.add_argument("--solver", choices=("plain", "wavefront"), default="wavefront")
+    p.add_argument("--fp64-steps", type=int, default=1,
+                   help="steps of the FP64 plain solver timed after the main run (0: skip)")
     p.add_argument("--spp-per-step", type=int, default=16)
     p.add_argument("--width", type=int, default=0, help="override the image width/height")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--seed", type=int, default=0)
+    p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                   help="only run 5 plain device copies of MIB MiB (profiler calibration)")
     return p.parse_args()
+
+
+def calibrate_copy(mib):
+    """Plain copies of a known size: rocprofv3's FETCH_SIZE / WRITE_SIZE of
+    these dispatches give the scale of the traffic counters."""
+    from htrdr_kmin_amd import _ffi as F
+    from htrdr_kmin_amd import hip as H
+    n = mib << 20
+    a, b = H.DeviceBuffer(n), H.DeviceBuffer(n)
+    a.zero()
+    s = H.Stream()
+    for _ in range(5):
+        F.check("htgx_debug_copy", F.load().htgx_debug_copy(b.ptr, a.ptr, n, s.h))
+    s.synchronize()
+    print(json.dumps({"copy_bytes": n, "dispatches": 5}), flush=True)
 
 
 def cpu_baseline(synth, desc, target_s):
@@ -85,6 +105,9 @@ def cpu_baseline(synth, desc, target_s):
 
 def main():
     args = parse()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:
         sys.exit("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -99,7 +122,6 @@ def main():
     desc = A.map_preset(args.config)
     if args.width:
         desc.width = desc.height = args.width
-    desc.solver = F.SOLVER_WAVEFRONT_FP32 if args.solver == "wavefront" else F.SOLVER_PLAIN_FP64
     desc.seed = args.seed
     npix = desc.width * desc.height
     S = args.spp_per_step
@@ -107,58 +129,65 @@ def main():
     acc = H.DeviceBuffer(npix * A.ACCUM_DTYPE.itemsize)
     stream = H.Stream()
 
-    def step(k):
-        desc.spp = S
-        desc.spp_offset = k * S
-        ctx.render_accum(desc, acc.address, stream.h.value)
+    def run(solver, nsteps, acc, count):
+        """Warm-up (optionally counting) then nsteps timed steps into acc."""
+        d = A.map_preset(args.config)
+        for f, _ in d._fields_:
+            setattr(d, f, getattr(desc, f))
+        d.solver = solver
 
-    # Warm-up; the first warm-up step also collects the event counters of the
-    # byte model (a separate, counting build of the kernel).
-    acc.zero()
-    ctx.set_counting(True)
-    t_w = time.perf_counter()
-    step(0)
-    stream.synchronize()
-    counters = ctx.counters()
-    ctx.set_counting(False)
-    print(f"[bench] warm-up step 0 (counting) {time.perf_counter() - t_w:.2f} s", file=sys.stderr,
-          flush=True)
-    for w in range(1, args.warmup):
-        step(w)
+        def step(k):
+            d.spp = S
+            d.spp_offset = k * S
+            ctx.render_accum(d, acc.address, stream.h.value)
+
+        acc.zero()
+        counters = None
+        t_w = time.perf_counter()
+        ctx.set_counting(count)
+        step(0)
         stream.synchronize()
-        print(f"[bench] warm-up step {w} done", file=sys.stderr, flush=True)
+        if count:
+            counters = ctx.counters()
+        ctx.set_counting(False)
+        print(f"[bench] {SOLVER_NAMES[solver]}: warm-up step 0 {time.perf_counter() - t_w:.2f} s",
+              file=sys.stderr, flush=True)
+        for w in range(1, args.warmup):
+            step(w)
+            stream.synchronize()
+        acc.zero()
+        ev = [(H.Event(), H.Event()) for _ in range(nsteps)]
+        H.synchronize()
+        t0 = time.perf_counter()
+        for k in range(nsteps):
+            ev[k][0].record(stream)
+            step(k)
+            ev[k][1].record(stream)
+            ev[k][1].synchronize()
+            print(f"[bench] {SOLVER_NAMES[solver]}: step {k}: {ev[k][0].elapsed_ms(ev[k][1]):.1f} ms",
+                  file=sys.stderr, flush=True)
+        stream.synchronize()
+        H.synchronize()
+        wall = time.perf_counter() - t0
+        total_s = sum(a.elapsed_ms(b) for a, b in ev) / 1e3
+        h = acc.download(np.zeros(npix, A.ACCUM_DTYPE))
+        return total_s, wall, h, counters
 
-    # Timed region
-    acc.zero()
-    ev = [(H.Event(), H.Event()) for _ in range(args.steps)]
-    H.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        ev[k][0].record(stream)
-        step(k)
-        ev[k][1].record(stream)
-        ev[k][1].synchronize()
-        print(f"[bench] step {k}: {ev[k][0].elapsed_ms(ev[k][1]):.1f} ms", file=sys.stderr,
-              flush=True)
-    stream.synchronize()
-    H.synchronize()
-    wall = time.perf_counter() - t0
-    kms = [a.elapsed_ms(b) for a, b in ev]
-    total_s = sum(kms) / 1e3
+    def t05(h, total_s, nreal):
+        """time for 99% of the pixels with E > 0 to reach a 0.5% relative SE"""
+        E, SE = A.accum_estimate(h)
+        n = h["n"].astype(np.float64)
+        pos = E > 0
+        rel_var = (SE[pos] ** 2 * n[pos]) / E[pos] ** 2  # per-realisation (sigma/E)^2
+        nreq = rel_var / 0.005 ** 2
+        spp_req = float(np.quantile(nreq, 0.99)) if nreq.size else float("nan")
+        return spp_req * npix * total_s / nreal, spp_req, float(np.mean(rel_var)), E, SE
 
+    main_solver = F.SOLVER_WAVEFRONT_FP32 if args.solver == "wavefront" else F.SOLVER_PLAIN_FP64
+    total_s, wall, h, counters = run(main_solver, args.steps, acc, True)
     real = npix * S * args.steps
     value = real / total_s / 1e6
-
-    # time to 0.5% relative standard error
-    h = acc.download(np.zeros(npix, A.ACCUM_DTYPE))
-    E, SE = A.accum_estimate(h)
-    n = h["n"].astype(np.float64)
-    pos = E > 0
-    sigma = SE[pos] * np.sqrt(n[pos])
-    nreq = (sigma / (0.005 * E[pos])) ** 2
-    spp_req = float(np.quantile(nreq, 0.99)) if nreq.size else float("nan")
-    t_per_real = total_s / real
-    t05 = spp_req * npix * t_per_real
+    t05_main, spp_req, relvar_main, E_main, SE_main = t05(h, total_s, real)
 
     # roofline of the dominant kernel (one launch = one step)
     creal = max(counters["nrealisations"], 1)
@@ -196,7 +225,7 @@ def main():
             "spp_per_step": S,
             "wall_s": round(wall, 3),
         },
-        "time_to_0p5pct_s": round(t05, 3),
+        "time_to_0p5pct_s": round(t05_main, 3),
         "spp_required_0p5pct_q99": round(spp_req, 1),
         "roofline": {
             "bound": "hbm",
@@ -216,6 +245,33 @@ def main():
         },
         "cpu_baseline": None,
     }
+    if args.fp64_steps > 0 and main_solver == F.SOLVER_WAVEFRONT_FP32:
+        acc64 = H.DeviceBuffer(npix * A.ACCUM_DTYPE.itemsize)
+        s64, _, h64, _ = run(F.SOLVER_PLAIN_FP64, args.fp64_steps, acc64, False)
+        acc64.free()
+        real64 = npix * S * args.fp64_steps
+        t05_64, spp64, relvar_64, E64, SE64 = t05(h64, s64, real64)
+        sig = np.sqrt(SE_main ** 2 + SE64 ** 2)
+        m = sig > 0
+        z = (E_main[m] - E64[m]) / sig[m]
+        # time-to-0.5% ratio from the per-realisation time and the mean
+        # per-realisation relative variance of each solver (robust to the
+        # noise of a q99 taken over few FP64 realisations)
+        t_real_main, t_real_64 = total_s / real, s64 / real64
+        out["fp64_plain"] = {
+            "value": round(real64 / s64 / 1e6, 4), "unit": "Mrealisations/s",
+            "steps": args.fp64_steps, "spp": S * args.fp64_steps,
+            "time_to_0p5pct_s_q99": round(t05_64, 3),
+            "mean_rel_variance_per_realisation": round(relvar_64, 4),
+        }
+        out["fp32_mean_rel_variance_per_realisation"] = round(relvar_main, 4)
+        out["time_to_0p5pct_speedup_fp32_vs_fp64"] = round(
+            (t_real_64 * relvar_64) / (t_real_main * relvar_main), 3)
+        out["fp32_vs_fp64_per_pixel"] = {
+            "within_3_sigma": round(float(np.mean(np.abs(z) <= 3)), 5),
+            "z_mean": round(float(z.mean()), 4), "z_std": round(float(z.std()), 4),
+            "image_mean_rel_diff": round(float((E_main.mean() - E64.mean()) / E64.mean()), 6),
+        }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(synth, desc, args.cpu_seconds)
     stream.destroy()

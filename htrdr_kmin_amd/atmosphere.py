@@ -125,7 +125,9 @@ class Context:
     def counters(self):
         c = F.Counters()
         F.check("htgx_get_counters", self._lib.htgx_get_counters(self._h, C.byref(c)))
-        return {k: getattr(c, k) for k, _ in F.Counters._fields_}
+        out = {k: getattr(c, k) for k, _ in F.Counters._fields_ if k != "diag"}
+        out["diag"] = list(c.diag)
+        return out
 
     def trace_segments(self, rays, prop, iband=0, iquad=0, max_segs=256):
         rays = np.ascontiguousarray(rays, np.float64).reshape(-1, 8)

@@ -224,6 +224,15 @@ sample_dirs_kernel(DevScene S, int kind, double param, double dx, double dy, dou
   }
 }
 
+/* Plain streaming copy, the reference traffic of the HBM counters */
+__global__ void __launch_bounds__(256)
+copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n)
+{
+  for(size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+      i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 template<class T>
 int upload(htgx_ctx* ctx, const T* host, size_t n, const T** out)
 {
@@ -380,8 +389,8 @@ htgx_ctx_init(htgx_ctx* ctx, const htgx_scene_desc& sc)
     S.sun_solid_angle = 2.0 * kPi * (1.0 - S.sun_cos_half);
   }
 
-  HIP_OK(hipMalloc(&ctx->d_counters, 8 * sizeof(unsigned long long)));
-  HIP_OK(hipMemset(ctx->d_counters, 0, 8 * sizeof(unsigned long long)));
+  HIP_OK(hipMalloc(&ctx->d_counters, 16 * sizeof(unsigned long long)));
+  HIP_OK(hipMemset(ctx->d_counters, 0, 16 * sizeof(unsigned long long)));
   return HTGX_OK;
 }
 
@@ -556,7 +565,7 @@ render_plain(htgx_ctx* ctx, const DevMap& M, htgx_mc_accum* accum, hipStream_t s
   const uint64_t nthreads = items_per_slice * nslices;
   const uint32_t nblocks = (uint32_t)((nthreads + 255) / 256);
   if(ctx->counting) {
-    HIP_OK(hipMemsetAsync(ctx->d_counters, 0, 8 * sizeof(unsigned long long), stream));
+    HIP_OK(hipMemsetAsync(ctx->d_counters, 0, 16 * sizeof(unsigned long long), stream));
     plain_kernel<true><<<nblocks, 256, 0, stream>>>(ctx->S, M, ntx, (uint32_t)items_per_slice,
       nslices, spt, (htgx_mc_accum*)ctx->d_partials, ctx->d_counters);
   } else {
@@ -595,7 +604,7 @@ extern "C" int
 htgx_get_counters(htgx_ctx* ctx, htgx_counters* c)
 {
   if(!ctx || !c) return HTGX_BAD_ARG;
-  unsigned long long h[8];
+  unsigned long long h[16];
   HIP_OK(hipDeviceSynchronize());
   HIP_OK(hipMemcpy(h, ctx->d_counters, sizeof(h), hipMemcpyDeviceToHost));
   c->nrealisations = ctx->last_nrealisations;
@@ -606,6 +615,7 @@ htgx_get_counters(htgx_ctx* ctx, htgx_counters* c)
   c->nevents = h[5];
   c->nseg = h[2];
   c->nlwc = h[4];
+  for(int i = 0; i < 7; ++i) c->diag[i] = h[8 + i];
   return HTGX_OK;
 }
 
@@ -807,5 +817,17 @@ htgx_debug_sample_dirs(htgx_ctx* ctx, int kind, double param, const double dir_i
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpy(out, d_o, 3 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
   HIP_OK(hipFree(d_u)); HIP_OK(hipFree(d_o));
+  return HTGX_OK;
+}
+
+extern "C" int
+htgx_debug_copy(void* dst, const void* src, size_t nbytes, void* stream)
+{
+  if(!dst || !src || !nbytes || (nbytes % 16) || ((uintptr_t)dst % 16) || ((uintptr_t)src % 16))
+    return HTGX_BAD_ARG;
+  const size_t n = nbytes / 16;
+  const uint32_t blocks = (uint32_t)std::min<size_t>((n + 255) / 256, 256 * 64);
+  copy_kernel<<<blocks, 256, 0, (hipStream_t)stream>>>((const uint4*)src, (uint4*)dst, n);
+  HIP_OK(hipGetLastError());
   return HTGX_OK;
 }

@@ -68,6 +68,32 @@ HTGX_HD void threefry4x64_20(const uint64_t ctr[4], const uint64_t key[4], uint6
   out[0] = x0; out[1] = x1; out[2] = x2; out[3] = x3;
 }
 
+/* One Threefry-2x32 block with 20 rounds (rotation schedule R_{2x32} of the
+ * paper's table 4, key injection after every 4 rounds). Used by the FP32
+ * solver, which draws one block per pair of FP32 canonicals. */
+HTGX_HD uint32_t rotl32(uint32_t x, unsigned r)
+{
+  return (x << r) | (x >> (32u - r));
+}
+
+HTGX_HD void threefry2x32_20(const uint32_t ctr[2], const uint32_t key[2], uint32_t out[2])
+{
+  const uint32_t ks[3] = {key[0], key[1], 0x1BD11BDAu ^ key[0] ^ key[1]};
+  uint32_t x0 = ctr[0] + ks[0];
+  uint32_t x1 = ctr[1] + ks[1];
+#define HTGX_TF2(r) x0 += x1; x1 = rotl32(x1, r); x1 ^= x0;
+#define HTGX_TF2_INJECT(s) x0 += ks[(s) % 3]; x1 += ks[((s) + 1) % 3] + (uint32_t)(s);
+  HTGX_TF2(13) HTGX_TF2(15) HTGX_TF2(26) HTGX_TF2(6)  HTGX_TF2_INJECT(1)
+  HTGX_TF2(17) HTGX_TF2(29) HTGX_TF2(16) HTGX_TF2(24) HTGX_TF2_INJECT(2)
+  HTGX_TF2(13) HTGX_TF2(15) HTGX_TF2(26) HTGX_TF2(6)  HTGX_TF2_INJECT(3)
+  HTGX_TF2(17) HTGX_TF2(29) HTGX_TF2(16) HTGX_TF2(24) HTGX_TF2_INJECT(4)
+  HTGX_TF2(13) HTGX_TF2(15) HTGX_TF2(26) HTGX_TF2(6)  HTGX_TF2_INJECT(5)
+#undef HTGX_TF2
+#undef HTGX_TF2_INJECT
+  out[0] = x0;
+  out[1] = x1;
+}
+
 /* Stream of one realisation */
 struct Stream {
   uint64_t key[4];

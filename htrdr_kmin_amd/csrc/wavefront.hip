@@ -25,8 +25,9 @@
  *  - results: every chunk writes its own partial accumulator; a merge kernel
  *    folds them per pixel in a fixed order, so a map does not depend on the
  *    schedule.
- * Random numbers: the Threefry-4x64-20 stream of the realisation (key =
- * seed, pixel, realisation), each 64-bit word giving two FP32 canonicals.
+ * Random numbers: Threefry-2x32-20 blocks keyed by (pixel, realisation), with
+ * the block index and the seed as counter; each block gives two FP32
+ * canonicals and every draw site computes its own block.
  */
 #pragma clang fp contract(fast)
 
@@ -120,14 +121,11 @@ struct Lane {
   /* work */
   uint32_t chunk, pixel, r, r_end;
   int lstate;
-  /* random stream: the words of the current Threefry block, consumed from
-   * b0 and shifted down (no dynamic indexing, so the state stays in
-   * registers) */
-  uint64_t b0, b1, b2, b3;
-  uint32_t block, idx;
+  uint32_t pair;        /* next Threefry-2x32 block of the realisation */
   /* realisation */
   float pdf, g, L_sun, wlen;
-  uint32_t band, quad;
+  uint32_t band, gka_off, gks_off; /* band; rows of gas ka (band, k-point) and ks (band) */
+  float cka, cks;       /* particle absorption / scattering per LWC of the band */
   float org[3], dir0[3], cosN, Ld; /* flux SW */
   uint64_t t0;
   /* walk */
@@ -135,13 +133,15 @@ struct Lane {
   bool from_ground, surface, rayleigh, gh_hit, pending;
   uint32_t gh_mtl;
   float pos[3], dir[3], dir_next[3];
-  float qr, ksi, w, Tr_abs, R, refl, gas_ratio;
+  float ksi, w, Tr_abs, R, refl, gas_ratio, ev_dst;
   /* query */
   int mode, prop;
-  float tc, Ts;
+  bool nq, nq_ground;   /* a query is requested; its range comes from a ground trace */
+  float qr;             /* end of the query range */
+  float inv_dir[3];
+  float tc, Ts, maj, inv_maj;
   double Tmin;
   int ev;
-  float ev_dst;
   int32_t ev_layer;
   uint32_t nsteps;
   /* segment iterator */
@@ -165,26 +165,24 @@ struct Lane {
   uint32_t c_leaf, c_seg, c_coll, c_lwc, c_ev;
 };
 
-__device__ __forceinline__ float canonicalf(Lane& L, const uint64_t* key)
+/* Next pair of FP32 canonicals of the realisation: one Threefry-2x32-20 block
+ * with counter (pair index, seed) under the key (pixel, realisation). The
+ * numbers of a realisation depend only on its key and on how many pairs it
+ * drew before, not on the lanes it shares a wavefront with. */
+__device__ __forceinline__ float2 draw2(Lane& L, uint32_t key)
 {
-  if(L.idx == 8) {
-    const uint64_t ctr[4] = {L.block, 0, 0, 0};
-    uint64_t out[4];
-    threefry4x64_20(ctr, key, out);
-    L.b0 = out[0]; L.b1 = out[1]; L.b2 = out[2]; L.b3 = out[3];
-    L.block += 1;
-    L.idx = 0;
-  }
-  const bool low = (L.idx & 1u) != 0;
-  const uint32_t half = low ? (uint32_t)L.b0 : (uint32_t)(L.b0 >> 32);
-  if(low) { L.b0 = L.b1; L.b1 = L.b2; L.b2 = L.b3; }
-  L.idx += 1;
-  return (float)(half >> 8) * 0x1.0p-24f;
+  const uint32_t ctr[2] = {L.pair, key};
+  const uint32_t k[2] = {L.pixel, L.r};
+  uint32_t out[2];
+  threefry2x32_20(ctr, k, out);
+  L.pair += 1;
+  return make_float2((float)(out[0] >> 8) * 0x1.0p-24f, (float)(out[1] >> 8) * 0x1.0p-24f);
 }
 
-__device__ __forceinline__ float ran_expf(Lane& L, const uint64_t* key)
+/* Exp(1) sample from a canonical, ssp_ran_exp(rng, 1) */
+__device__ __forceinline__ float exp1f(float u)
 {
-  return -__logf(1.0f - canonicalf(L, key));
+  return -__logf(1.0f - u);
 }
 
 __device__ __forceinline__ void nsum(double& sum, double& c, double x)
@@ -193,6 +191,16 @@ __device__ __forceinline__ void nsum(double& sum, double& c, double x)
   if(fabs(sum) >= fabs(x)) c += (sum - t) + x;
   else c += (x - t) + sum;
   sum = t;
+}
+
+/* Planck's law in W/m^2/sr/m (src/core/htrdr_spectral.h:102-116) in FP32 */
+__device__ __forceinline__ float planckf(float wlen_nm, float T)
+{
+  const float c1 = 1.191042972e-16f; /* 2 h c^2, W m^2 / sr */
+  const float c2 = 1.438776877e-2f;  /* h c / k, m K */
+  const float l = wlen_nm * 1e-9f;
+  const float l2 = l * l;
+  return c1 / (l2 * l2 * l) / (__expf(c2 / (l * T)) - 1.f);
 }
 
 /*******************************************************************************
@@ -268,7 +276,7 @@ template<bool COUNT>
 __device__ __forceinline__ void load_leaf(const SceneF& S, Lane& L, uint32_t cx, uint32_t cy,
                                           uint32_t cz)
 {
-  const uint32_t word = S.lmap[((size_t)cz * S.dims0[1] + cy) * S.dims0[0] + cx];
+  const uint32_t word = S.lmap[(cz * S.dims0[1] + cy) * S.dims0[0] + cx];
   const uint32_t lvl = word >> kLeafIdBits;
   L.leaf = word & kLeafIdMask;
   const float2 b = S.bounds[L.leaf];
@@ -283,7 +291,7 @@ __device__ __forceinline__ void load_leaf(const SceneF& S, Lane& L, uint32_t cx,
     const uint32_t hi = min(lo + (1u << lvl), S.dims0[a]);
     L.lohi[a] = lo | (hi << 16);
     if(L.dir[a] != 0.f) {
-      const float ta = (planef(S, a, L.dir[a] > 0.f ? hi : lo) - L.pos[a]) / L.dir[a];
+      const float ta = (planef(S, a, L.dir[a] > 0.f ? hi : lo) - L.pos[a]) * L.inv_dir[a];
       if(ta < best) { best = ta; ax = a; }
     }
   }
@@ -329,15 +337,18 @@ __device__ __forceinline__ void step_leaf(const SceneF& S, Lane& L)
   load_leaf<COUNT>(S, L, c[0], c[1], c[2]);
 }
 
-/* Start a query over [0, r1] along (pos, dir); no memory access here */
-__device__ __forceinline__ void iter_init(const SceneF& S, Lane& L, float r1)
+/* Start the iterator of the query over [0, qr] along (pos, dir); no memory
+ * access here */
+__device__ __forceinline__ void iter_init(const SceneF& S, Lane& L)
 {
+  const float r1 = L.qr;
   L.tit = 0.f;
-  L.qr = r1;
   L.itdone = false;
   L.adv = false;
   L.cstate = 2;
   L.nsteps = 0;
+#pragma unroll
+  for(int a = 0; a < 3; ++a) L.inv_dir[a] = 1.f / L.dir[a];
   if(S.has_clouds) {
     float tn = -INFINITY, tf = INFINITY;
     bool miss = false;
@@ -345,8 +356,8 @@ __device__ __forceinline__ void iter_init(const SceneF& S, Lane& L, float r1)
     for(int a = 0; a < 3; ++a) {
       const float plo = planef(S, a, 0), phi = planef(S, a, S.dims0[a]);
       if(L.dir[a] != 0.f) {
-        const float ta = (plo - L.pos[a]) / L.dir[a];
-        const float tb = (phi - L.pos[a]) / L.dir[a];
+        const float ta = (plo - L.pos[a]) * L.inv_dir[a];
+        const float tb = (phi - L.pos[a]) * L.inv_dir[a];
         tn = fmaxf(tn, fminf(ta, tb));
         tf = fminf(tf, fmaxf(ta, tb));
       } else if(L.pos[a] < plo || L.pos[a] > phi) {
@@ -395,8 +406,8 @@ __device__ __forceinline__ bool iter_next(const SceneF& S, Lane& L, int prop, fl
     const float t = L.tit;
     const float tc = L.cstate == 0 ? L.tnext : L.cstate == 1 ? fmaxf(L.tnext, t) : INFINITY;
     float tg;
-    if(L.dir[2] > 0.f && L.j < (int32_t)S.nlayers) tg = (S.zb + (float)(L.j + 1) * S.dzl - L.pos[2]) / L.dir[2];
-    else if(L.dir[2] < 0.f && L.j >= 0) tg = (S.zb + (float)L.j * S.dzl - L.pos[2]) / L.dir[2];
+    if(L.dir[2] > 0.f && L.j < (int32_t)S.nlayers) tg = (S.zb + (float)(L.j + 1) * S.dzl - L.pos[2]) * L.inv_dir[2];
+    else if(L.dir[2] < 0.f && L.j >= 0) tg = (S.zb + (float)L.j * S.dzl - L.pos[2]) * L.inv_dir[2];
     else tg = INFINITY;
     tg = fmaxf(tg, t);
     const float t1 = fminf(fminf(tc, tg), tmax);
@@ -407,13 +418,12 @@ __device__ __forceinline__ bool iter_next(const SceneF& S, Lane& L, int prop, fl
       if(in_gas || in_cloud) {
         float gk = 0.f, cmn = 0.f, cmx = 0.f;
         if(in_gas) {
-          const float ka = S.gas_ka[(size_t)(S.quad_off[L.band] + L.quad) * S.nlayers + (uint32_t)L.j];
-          const float ks = S.gas_ks[(size_t)L.band * S.nlayers + (uint32_t)L.j];
+          const float ka = S.gas_ka[L.gka_off + (uint32_t)L.j];
+          const float ks = S.gas_ks[L.gks_off + (uint32_t)L.j];
           gk = prop == HTGX_Ka ? ka : prop == HTGX_Ks ? ks : ka + ks;
         }
         if(in_cloud) {
-          const float ca = S.cka[L.band], cs = S.cks[L.band];
-          const float c = prop == HTGX_Ka ? ca : prop == HTGX_Ks ? cs : ca + cs;
+          const float c = prop == HTGX_Ka ? L.cka : prop == HTGX_Ks ? L.cks : L.cka + L.cks;
           cmn = c * L.lmn;
           cmx = c * L.lmx;
         }
@@ -449,47 +459,42 @@ __device__ __forceinline__ float seg_lwcf(const SceneF& S, Lane& L, const float 
     ci[a] = f < (float)lo ? lo : f > (float)hi ? hi : (uint32_t)f;
   }
   if(COUNT) L.c_lwc++;
-  return S.lwc[((size_t)ci[2] * S.def[1] + ci[1]) * S.def[0] + ci[0]];
+  return S.lwc[(ci[2] * S.def[1] + ci[1]) * S.def[0] + ci[0]];
 }
 
 /*******************************************************************************
- * Walk logic (EVENT branch)
+ * Walk logic (EVENT branch). Stage functions only request the next query;
+ * the query is set up once at the end of the event branch. `u' holds the
+ * canonicals drawn for this event: slots 0-1 film / position / scattering
+ * direction, 2-3 hemisphere / sun, 4-6 spectral / phase choice / Russian
+ * roulette, 7-8 flux sun, the last one the optical thickness of the query.
  ******************************************************************************/
-__device__ __forceinline__ void set_query(const SceneF& S, Lane& L, int mode, int prop, float r1)
+__device__ __forceinline__ void request(Lane& L, int mode, int prop, float r1)
 {
   L.mode = mode;
   L.prop = prop;
-  L.Tmin = 0.0;
-  L.pending = false;
-  L.lstate = LS_SEG;
-  iter_init(S, L, r1);
+  L.qr = r1;
+  L.nq = true;
+  L.nq_ground = false;
 }
 
-/* Sample the exponential optical thickness of a new query */
-__device__ __forceinline__ void begin_query(const SceneF& S, Lane& L, const uint64_t* key,
-                                            int mode, int prop, float r1)
+__device__ __forceinline__ void request_ground(Lane& L, int mode, int prop)
 {
-  L.Ts = ran_expf(L, key);
-  set_query(S, L, mode, prop, r1);
+  L.mode = mode;
+  L.prop = prop;
+  L.nq = true;
+  L.nq_ground = true;
 }
 
-__device__ __forceinline__ void sw_scatter_query(const SceneF& S, Lane& L, const uint64_t* key)
-{
-  float t = FLT_MAX;
-  L.gh_hit = ground_planef(S, L.pos, L.dir, FLT_MAX, L.from_ground, &t);
-  L.gh_mtl = S.plane_mtl;
-  begin_query(S, L, key, MODE_SC, HTGX_Ks, L.gh_hit ? t : FLT_MAX);
-}
-
-__device__ __forceinline__ void sw_after_direct(const SceneF& S, Lane& L, const uint64_t* key)
+__device__ __forceinline__ void sw_after_direct(Lane& L)
 {
   if(!(L.mask & HTGX_RADIANCE_DIFFUSE)) { L.stage = ST_WALK_DONE; return; }
   L.stage = ST_SCATTER;
-  sw_scatter_query(S, L, key);
+  request_ground(L, MODE_SC, HTGX_Ks);
 }
 
-__device__ __forceinline__ void sw_begin(const SceneF& S, Lane& L, const uint64_t* key, int mask,
-                                         const float p[3], const float d[3])
+__device__ __forceinline__ void sw_begin(const SceneF& S, Lane& L, int mask, const float p[3],
+                                         const float d[3])
 {
   L.mask = mask;
 #pragma unroll
@@ -501,33 +506,24 @@ __device__ __forceinline__ void sw_begin(const SceneF& S, Lane& L, const uint64_
     float t;
     if(!ground_planef(S, L.pos, L.dir, FLT_MAX, false, &t)) {
       L.stage = ST_DIRECT;
-      begin_query(S, L, key, MODE_TR, HTGX_Kext, FLT_MAX);
+      request(L, MODE_TR, HTGX_Kext, FLT_MAX);
       return;
     }
   }
-  sw_after_direct(S, L, key);
-}
-
-__device__ __forceinline__ void lw_query(const SceneF& S, Lane& L, const uint64_t* key)
-{
-  float t = FLT_MAX;
-  L.gh_hit = ground_planef(S, L.pos, L.dir, FLT_MAX, L.from_ground, &t);
-  L.gh_mtl = S.plane_mtl;
-  L.stage = ST_LW;
-  begin_query(S, L, key, MODE_LW, HTGX_Kext, L.gh_hit ? t : FLT_MAX);
+  sw_after_direct(L);
 }
 
 /* Next-event estimate and Russian roulette closing a SW event
  * (..._sw.c:494-504); the next scattering query starts from the event */
-__device__ __forceinline__ void sw_accumulate(const SceneF& S, Lane& L, const uint64_t* key, float Tr)
+__device__ __forceinline__ void sw_accumulate(const SceneF& S, Lane& L, float Tr, float u_rr)
 {
   L.ksi *= L.Tr_abs;
   L.w += L.ksi * L.L_sun * Tr * L.R * S.sun_solid_angle;
-  if(L.surface && canonicalf(L, key) >= L.refl) { L.stage = ST_WALK_DONE; return; }
+  if(L.surface && u_rr >= L.refl) { L.stage = ST_WALK_DONE; return; }
 #pragma unroll
   for(int a = 0; a < 3; ++a) L.dir[a] = L.dir_next[a];
   L.stage = ST_SCATTER;
-  sw_scatter_query(S, L, key);
+  request_ground(L, MODE_SC, HTGX_Ks);
 }
 
 __device__ __forceinline__ float query_tr(const Lane& L)
@@ -537,12 +533,12 @@ __device__ __forceinline__ float query_tr(const Lane& L)
 
 /* Consume the finished query of the current stage */
 template<bool COUNT>
-__device__ __forceinline__ void consume(const SceneF& S, Lane& L, const uint64_t* key)
+__device__ __forceinline__ void consume(const SceneF& S, Lane& L, const float* u)
 {
   switch(L.stage) {
     case ST_DIRECT:
       L.w = L.L_sun * query_tr(L);
-      sw_after_direct(S, L, key);
+      sw_after_direct(L);
       break;
 
     case ST_SCATTER: {
@@ -553,9 +549,9 @@ __device__ __forceinline__ void consume(const SceneF& S, Lane& L, const uint64_t
       const float dst = L.surface ? L.qr : L.ev_dst;
       L.from_ground = L.surface;
       L.stage = ST_ABS;
-      /* the absorption query runs from pos along dir up to the event */
-      begin_query(S, L, key, MODE_TR, HTGX_Ka, dst);
       L.ev_dst = dst;
+      /* the absorption query runs from pos along dir up to the event */
+      request(L, MODE_TR, HTGX_Ka, dst);
     } break;
 
     case ST_ABS: {
@@ -563,48 +559,40 @@ __device__ __forceinline__ void consume(const SceneF& S, Lane& L, const uint64_t
       if(L.Tr_abs <= 0.f) { L.stage = ST_WALK_DONE; break; }
       const float dst = L.ev_dst;
       float N[3] = {0.f, 0.f, 1.f};
-      /* move to the event */
 #pragma unroll
       for(int a = 0; a < 3; ++a) L.pos[a] = L.pos[a] + L.dir[a] * dst;
       if(L.surface) {
-        if(-(L.dir[2]) < 0.f) N[2] = -1.f; /* plane normal faces the incoming ray */
-        const float albedo = S.mtl_albedo[L.gh_mtl];
-        const float u0 = canonicalf(L, key), u1 = canonicalf(L, key);
-        local_to_worldf(N, sqrtf(1.f - u0), 2.f * kPif * u1, L.dir_next);
-        L.refl = albedo;
+        if(-(L.dir[2]) < 0.f) N[2] = -1.f; /* the normal faces the incoming ray */
+        local_to_worldf(N, sqrtf(1.f - u[0]), 2.f * kPif * u[1], L.dir_next);
+        L.refl = S.mtl_albedo[L.gh_mtl];
       } else {
-        L.rayleigh = canonicalf(L, key) < L.gas_ratio;
-        const float u0 = canonicalf(L, key), u1 = canonicalf(L, key);
-        const float mu = L.rayleigh ? rayleigh_cosf(u0) : hg_cosf(L.g, u0);
-        local_to_worldf(L.dir, mu, 2.f * kPif * u1, L.dir_next);
+        L.rayleigh = u[4] < L.gas_ratio;
+        const float mu = L.rayleigh ? rayleigh_cosf(u[0]) : hg_cosf(L.g, u[0]);
+        local_to_worldf(L.dir, mu, 2.f * kPif * u[1], L.dir_next);
         L.refl = 1.f;
       }
       float sun_dir[3];
-      {
-        const float u0 = canonicalf(L, key), u1 = canonicalf(L, key);
-        const float cos_t = 1.f - u0 * (1.f - S.sun_cos_half);
-        local_to_worldf(S.sun_dir, cos_t, 2.f * kPif * u1, sun_dir);
-      }
+      local_to_worldf(S.sun_dir, 1.f - u[2] * (1.f - S.sun_cos_half), 2.f * kPif * u[3], sun_dir);
       if(L.surface) {
         const float c = dot3f(N, sun_dir);
-        L.R = c < 0.f ? 0.f : S.mtl_albedo[L.gh_mtl] / kPif * c;
+        L.R = c < 0.f ? 0.f : L.refl / kPif * c;
       } else {
         const float mu = dot3f(L.dir, sun_dir);
         L.R = L.rayleigh ? rayleigh_evalf(mu) : hg_evalf(L.g, mu);
       }
       float tg;
       if(L.R <= 0.f || ground_planef(S, L.pos, sun_dir, FLT_MAX, L.from_ground, &tg)) {
-        sw_accumulate(S, L, key, 0.f);
+        sw_accumulate(S, L, 0.f, u[5]);
         break;
       }
 #pragma unroll
       for(int a = 0; a < 3; ++a) L.dir[a] = sun_dir[a];
       L.stage = ST_SUN;
-      begin_query(S, L, key, MODE_TR, HTGX_Kext, FLT_MAX);
+      request(L, MODE_TR, HTGX_Kext, FLT_MAX);
     } break;
 
     case ST_SUN:
-      sw_accumulate(S, L, key, query_tr(L));
+      sw_accumulate(S, L, query_tr(L), u[5]);
       break;
 
     case ST_LW: {
@@ -616,22 +604,20 @@ __device__ __forceinline__ void consume(const SceneF& S, Lane& L, const uint64_t
       if(L.ev == EV_ABS) {
         const int32_t nl = (int32_t)S.nlayers;
         const int32_t jj = L.ev_layer < 0 ? 0 : L.ev_layer >= nl ? nl - 1 : L.ev_layer;
-        L.w = (float)planck_mono((double)L.wlen * 1e-9, (double)S.gas_T[jj]);
+        L.w = planckf(L.wlen, S.gas_T[jj]);
         L.stage = ST_WALK_DONE;
         break;
       }
       float dn[3];
       if(L.ev == EV_SCA) {
-        const float u0 = canonicalf(L, key), u1 = canonicalf(L, key);
-        local_to_worldf(L.dir, hg_cosf(L.g, u0), 2.f * kPif * u1, dn);
+        local_to_worldf(L.dir, hg_cosf(L.g, u[0]), 2.f * kPif * u[1], dn);
         L.from_ground = false;
       } else {
         float N[3] = {0.f, 0.f, -(L.dir[2]) < 0.f ? -1.f : 1.f};
-        const float u0 = canonicalf(L, key), u1 = canonicalf(L, key);
-        local_to_worldf(N, sqrtf(1.f - u0), 2.f * kPif * u1, dn);
-        if(canonicalf(L, key) >= S.mtl_albedo[L.gh_mtl]) {
+        local_to_worldf(N, sqrtf(1.f - u[0]), 2.f * kPif * u[1], dn);
+        if(u[6] >= S.mtl_albedo[L.gh_mtl]) {
           const float T = S.mtl_T ? S.mtl_T[L.gh_mtl] : 0.f;
-          L.w = T > 0.f ? (float)planck_mono((double)L.wlen * 1e-9, (double)T) : 0.f;
+          L.w = T > 0.f ? planckf(L.wlen, T) : 0.f;
           L.stage = ST_WALK_DONE;
           break;
         }
@@ -639,7 +625,8 @@ __device__ __forceinline__ void consume(const SceneF& S, Lane& L, const uint64_t
       }
 #pragma unroll
       for(int a = 0; a < 3; ++a) L.dir[a] = dn[a];
-      lw_query(S, L, key);
+      L.stage = ST_LW;
+      request_ground(L, MODE_LW, HTGX_Kext);
     } break;
 
     default: break;
@@ -649,16 +636,14 @@ __device__ __forceinline__ void consume(const SceneF& S, Lane& L, const uint64_t
 /* Sensor ray, wavelength, band and k-point, then the first walk
  * (..._draw_map.c:348-400 and :233-302) */
 template<int SENSOR, int SPECTRAL>
-__device__ __forceinline__ void real_begin(const SceneF& S, const DevMap& M, Lane& L, const uint64_t* key)
+__device__ __forceinline__ void real_begin(const SceneF& S, const DevMap& M, Lane& L, const float* u)
 {
-  L.block = 0;
-  L.idx = 8;
   const uint32_t px = L.pixel % M.width, py = L.pixel / M.width;
   float org[3], dir[3];
   if(SENSOR == HTGX_SENSOR_CAMERA) {
-    const float fx = ((float)px + canonicalf(L, key)) / (float)M.width;
-    const float fy = ((float)py + canonicalf(L, key)) / (float)M.height;
-    (void)canonicalf(L, key); (void)canonicalf(L, key); /* lens sample */
+    /* film sample; a pinhole camera needs no lens sample */
+    const float fx = ((float)px + u[0]) / (float)M.width;
+    const float fy = ((float)py + u[1]) / (float)M.height;
     const float sx = (2.f * fx - 1.f) * (float)(M.tan_half * M.aspect);
     const float sy = (1.f - 2.f * fy) * (float)M.tan_half;
 #pragma unroll
@@ -670,8 +655,8 @@ __device__ __forceinline__ void real_begin(const SceneF& S, const DevMap& M, Lan
 #pragma unroll
     for(int a = 0; a < 3; ++a) dir[a] *= inv;
   } else {
-    const float sx = ((float)px + canonicalf(L, key)) / (float)M.width;
-    const float sy = ((float)py + canonicalf(L, key)) / (float)M.height;
+    const float sx = ((float)px + u[0]) / (float)M.width;
+    const float sy = ((float)py + u[1]) / (float)M.height;
 #pragma unroll
     for(int a = 0; a < 3; ++a)
       org[a] = (float)M.pos[a] + (float)M.axis_x[a] * (sx * 2.f - 1.f)
@@ -680,14 +665,12 @@ __device__ __forceinline__ void real_begin(const SceneF& S, const DevMap& M, Lan
     float t;
     /* an up ray hitting the (upward facing) ground from below starts inside */
     if(ground_planef(S, org, up, FLT_MAX, false, &t)) { L.stage = ST_REAL_DONE; L.nfail++; L.w = -1.f; return; }
-    const float u0 = canonicalf(L, key), u1 = canonicalf(L, key);
     const float z[3] = {(float)M.z[0], (float)M.z[1], (float)M.z[2]};
-    local_to_worldf(z, sqrtf(1.f - u0), 2.f * kPif * u1, dir);
+    local_to_worldf(z, sqrtf(1.f - u[2]), 2.f * kPif * u[3], dir);
   }
 
-  const double r0 = canonicalf(L, key), r1 = canonicalf(L, key), r2 = canonicalf(L, key);
   double pdf;
-  const double wlen = sample_wavelength(M, r0, r1, &pdf);
+  const double wlen = sample_wavelength(M, (double)u[4], (double)u[5], &pdf);
   L.pdf = (float)(pdf * 1e9);
   L.wlen = (float)wlen;
   {
@@ -696,11 +679,14 @@ __device__ __forceinline__ void real_begin(const SceneF& S, const DevMap& M, Lan
     L.band = lo;
     const uint32_t off = S.quad_off[lo], nq = S.quad_off[lo + 1] - off;
     uint32_t q = 0;
-    for(; q + 1 < nq; ++q) if(S.quad_cdf[off + q] > r2) break;
-    L.quad = q;
+    for(; q + 1 < nq; ++q) if(S.quad_cdf[off + q] > (double)u[6]) break;
+    L.gka_off = (off + q) * S.nlayers;
+    L.gks_off = lo * S.nlayers;
   }
   L.g = S.has_clouds ? S.hg_g[L.band] : 0.f;
-  L.L_sun = (float)planck_mono(wlen * 1e-9, 5778.0);
+  L.cka = S.has_clouds ? S.cka[L.band] : 0.f;
+  L.cks = S.has_clouds ? S.cks[L.band] : 0.f;
+  L.L_sun = planckf(L.wlen, 5778.f);
 
   if(SPECTRAL == HTGX_SPECTRAL_LW) {
     L.phase = 0;
@@ -708,26 +694,26 @@ __device__ __forceinline__ void real_begin(const SceneF& S, const DevMap& M, Lan
     for(int a = 0; a < 3; ++a) { L.pos[a] = org[a]; L.dir[a] = dir[a]; }
     L.from_ground = false;
     L.w = 0.f;
-    lw_query(S, L, key);
+    L.stage = ST_LW;
+    request_ground(L, MODE_LW, HTGX_Kext);
   } else if(SENSOR == HTGX_SENSOR_CAMERA) {
     L.phase = 0;
-    sw_begin(S, L, key, HTGX_RADIANCE_ALL, org, dir);
+    sw_begin(S, L, HTGX_RADIANCE_ALL, org, dir);
   } else {
     float sun_dir[3];
-    const float u0 = canonicalf(L, key), u1 = canonicalf(L, key);
-    local_to_worldf(S.sun_dir, 1.f - u0 * (1.f - S.sun_cos_half), 2.f * kPif * u1, sun_dir);
+    local_to_worldf(S.sun_dir, 1.f - u[7] * (1.f - S.sun_cos_half), 2.f * kPif * u[8], sun_dir);
     const float z[3] = {(float)M.z[0], (float)M.z[1], (float)M.z[2]};
     L.cosN = dot3f(z, sun_dir);
     L.Ld = 0.f;
 #pragma unroll
     for(int a = 0; a < 3; ++a) { L.org[a] = org[a]; L.dir0[a] = dir[a]; }
-    if(L.cosN > 0.f) { L.phase = 1; sw_begin(S, L, key, HTGX_RADIANCE_DIRECT, org, sun_dir); }
-    else { L.phase = 2; sw_begin(S, L, key, HTGX_RADIANCE_DIFFUSE, org, dir); }
+    if(L.cosN > 0.f) { L.phase = 1; sw_begin(S, L, HTGX_RADIANCE_DIRECT, org, sun_dir); }
+    else { L.phase = 2; sw_begin(S, L, HTGX_RADIANCE_DIFFUSE, org, dir); }
   }
 }
 
 template<int SENSOR, int SPECTRAL>
-__device__ __forceinline__ void walk_done(const SceneF& S, Lane& L, const uint64_t* key)
+__device__ __forceinline__ void walk_done(const SceneF& S, Lane& L)
 {
   if(SENSOR == HTGX_SENSOR_CAMERA) {
     L.stage = ST_REAL_DONE;
@@ -738,7 +724,7 @@ __device__ __forceinline__ void walk_done(const SceneF& S, Lane& L, const uint64
   } else if(L.phase == 1) {
     L.Ld = L.w;
     L.phase = 2;
-    sw_begin(S, L, key, HTGX_RADIANCE_DIFFUSE, L.org, L.dir0);
+    sw_begin(S, L, HTGX_RADIANCE_DIFFUSE, L.org, L.dir0);
   } else {
     L.stage = ST_REAL_DONE;
     L.w = (L.cosN * S.sun_solid_angle * L.Ld + kPif * L.w) / L.pdf;
@@ -771,82 +757,49 @@ template<int SENSOR, int SPECTRAL, bool COUNT>
 __global__ void __launch_bounds__(256)
 wavefront_kernel(SceneF S, DevMap M, Launch P)
 {
+  /* canonicals per event: 8, plus 2 for the sun of a SW flux map */
+  constexpr int NP = (SENSOR == HTGX_SENSOR_RECTANGLE && SPECTRAL == HTGX_SPECTRAL_SW) ? 5 : 4;
   Lane L;
   L.lstate = LS_EVENT;
   L.stage = ST_FETCH;
   L.pending = false;
+  L.nq = false;
   L.c_leaf = L.c_seg = L.c_coll = L.c_lwc = L.c_ev = 0;
   L.n = L.nfail = 0;
   L.sum = L.sum_c = L.sum2 = L.sum2_c = 0.0;
   L.tsum = L.tsum2 = 0.f;
+  L.pixel = L.r = 0;
   const uint32_t xcd = blockIdx.x % (uint32_t)kXcds;
-  uint64_t key[4] = {M.seed, 0, 0, 0};
+  const uint32_t key = (uint32_t)M.seed ^ (uint32_t)(M.seed >> 32);
+  uint64_t d_it = 0, d_ce = 0, d_cl = 0, d_se = 0, d_sl = 0, d_ee = 0, d_el = 0;
 
   for(;;) {
-    const uint64_t busy = __ballot(L.lstate == LS_SEG || L.lstate == LS_COLL);
-    const uint64_t evts = __ballot(L.lstate == LS_EVENT);
-    if(!(busy | evts)) break;
+    const uint64_t want0 = __ballot(L.lstate == LS_EVENT);
+    const uint64_t busy0 = __ballot(L.lstate == LS_SEG || L.lstate == LS_COLL);
+    if(!(busy0 | want0)) break;
 
-    /* tentative collision in the current segment */
-    if(L.lstate == LS_COLL) {
-      const float maj = L.mode == MODE_TR ? L.s_kmax - L.s_kmin : L.s_kmax;
-      const float T = (L.s_t1 - L.tc) * maj;
-      if(L.Ts > T || !(maj > 0.f)) {
-        L.Ts -= T;
-        L.lstate = LS_SEG;
-      } else {
-        L.tc += L.Ts / maj;
-        if(L.mode == MODE_SC && L.tc > 1e9f) {
-          L.lstate = LS_SEG;
-        } else {
-          if(COUNT) L.c_coll++;
-          float x[3];
-#pragma unroll
-          for(int a = 0; a < 3; ++a) x[a] = L.pos[a] + L.tc * L.dir[a];
-          const float v = L.s_cloud ? seg_lwcf<COUNT>(S, L, x) : 0.f;
-          const bool gas = L.s_layer >= 0;
-          const float gka = gas ? S.gas_ka[(size_t)(S.quad_off[L.band] + L.quad) * S.nlayers + (uint32_t)L.s_layer] : 0.f;
-          const float gks = gas ? S.gas_ks[(size_t)L.band * S.nlayers + (uint32_t)L.s_layer] : 0.f;
-          const float ka = gka + S.cka[L.band] * v;
-          const float ks = gks + S.cks[L.band] * v;
-          const float u = canonicalf(L, key);
-          if(L.mode == MODE_LW) {
-            const float pa = ka / maj, ps = ks / maj;
-            if(u < pa) { L.ev = EV_ABS; }
-            else if(u < pa + ps) { L.ev = EV_SCA; }
-            else L.ev = EV_NONE;
-          } else {
-            const float k = L.prop == HTGX_Ka ? ka : L.prop == HTGX_Ks ? ks : ka + ks;
-            const float proba = L.mode == MODE_TR ? (k - L.s_kmin) / maj : k / maj;
-            L.ev = u < proba ? EV_HIT : EV_NONE;
-            /* gas vs particle scattering at the event, ..._sw.c:427-438 */
-            if(L.mode == MODE_SC && L.ev == EV_HIT) L.gas_ratio = gks / ks;
-          }
-          if(L.ev != EV_NONE) {
-            L.ev_dst = L.tc;
-            L.ev_layer = L.s_layer;
-            L.pending = true;
-            L.lstate = LS_EVENT;
-          } else if(++L.nsteps > P.max_steps) {
-            /* micro-step budget of a query exhausted: end it without event */
-            L.ev = EV_NONE;
-            L.pending = true;
-            L.lstate = LS_EVENT;
-          } else {
-            L.Ts = ran_expf(L, key);
-          }
-        }
-      }
+    /* next segment of the query, and the first tentative collision in it */
+    if(COUNT) {
+      const uint64_t bs = __ballot(L.lstate == LS_SEG);
+      d_it++;
+      d_se += bs != 0;
+      d_sl += __popcll(bs);
     }
-
-    /* next segment of the current query */
     if(L.lstate == LS_SEG) {
       float t0;
       const int sprop = L.mode == MODE_SC ? HTGX_Ks : L.mode == MODE_LW ? HTGX_Kext : L.prop;
       if(iter_next<COUNT>(S, L, sprop, &t0)) {
         if(L.mode == MODE_TR) L.Tmin += (double)(L.s_t1 - t0) * (double)L.s_kmin;
-        L.tc = t0;
-        L.lstate = LS_COLL;
+        const float maj = L.mode == MODE_TR ? L.s_kmax - L.s_kmin : L.s_kmax;
+        const float T = (L.s_t1 - t0) * maj;
+        if(L.Ts > T || !(maj > 0.f)) {
+          L.Ts -= T; /* no collision in this segment */
+        } else {
+          L.maj = maj;
+          L.inv_maj = 1.f / maj;
+          L.tc = t0 + L.Ts * L.inv_maj;
+          L.lstate = LS_COLL;
+        }
       } else {
         L.ev = EV_NONE;
         L.pending = true;
@@ -854,61 +807,133 @@ wavefront_kernel(SceneF S, DevMap M, Launch P)
       }
     }
 
+    /* one tentative collision */
+    if(COUNT) {
+      const uint64_t bc = __ballot(L.lstate == LS_COLL);
+      d_ce += bc != 0;
+      d_cl += __popcll(bc);
+    }
+    if(L.lstate == LS_COLL) {
+      if(L.mode == MODE_SC && L.tc > 1e9f) {
+        L.lstate = LS_SEG; /* ..._sw.c:123 */
+      } else {
+        if(COUNT) L.c_coll++;
+        float x[3];
+#pragma unroll
+        for(int a = 0; a < 3; ++a) x[a] = L.pos[a] + L.tc * L.dir[a];
+        const float v = L.s_cloud ? seg_lwcf<COUNT>(S, L, x) : 0.f;
+        const bool gas = L.s_layer >= 0;
+        const float gka = gas ? S.gas_ka[L.gka_off + (uint32_t)L.s_layer] : 0.f;
+        const float gks = gas ? S.gas_ks[L.gks_off + (uint32_t)L.s_layer] : 0.f;
+        const float ka = gka + L.cka * v;
+        const float ks = gks + L.cks * v;
+        const float2 uu = draw2(L, key);
+        if(L.mode == MODE_LW) {
+          const float pa = ka * L.inv_maj, ps = ks * L.inv_maj;
+          L.ev = uu.x < pa ? EV_ABS : uu.x < pa + ps ? EV_SCA : EV_NONE;
+        } else {
+          const float k = L.prop == HTGX_Ka ? ka : L.prop == HTGX_Ks ? ks : ka + ks;
+          const float proba = (L.mode == MODE_TR ? k - L.s_kmin : k) * L.inv_maj;
+          L.ev = uu.x < proba ? EV_HIT : EV_NONE;
+          /* gas vs particle scattering at the event, ..._sw.c:427-438 */
+          if(L.mode == MODE_SC) L.gas_ratio = gks / ks;
+        }
+        if(L.ev != EV_NONE) {
+          L.ev_dst = L.tc;
+          L.ev_layer = L.s_layer;
+          L.pending = true;
+          L.lstate = LS_EVENT;
+        } else if(++L.nsteps > P.max_steps) {
+          /* micro-step budget of a query exhausted: end it without event */
+          L.pending = true;
+          L.lstate = LS_EVENT;
+        } else {
+          L.Ts = exp1f(uu.y);
+          const float T = (L.s_t1 - L.tc) * L.maj;
+          if(L.Ts > T) { L.Ts -= T; L.lstate = LS_SEG; }
+          else L.tc += L.Ts * L.inv_maj;
+        }
+      }
+    }
+
     /* events, when enough lanes need them or nothing else is left */
     const uint64_t want = __ballot(L.lstate == LS_EVENT);
     const uint64_t other = __ballot(L.lstate == LS_SEG || L.lstate == LS_COLL);
     if(want && (__popcll(want) >= P.ev_thresh || !other)) {
+      if(COUNT) { d_ee++; d_el += __popcll(want); }
       if(L.lstate == LS_EVENT) {
-        for(int guard = 0; guard < 64; ++guard) {
-          if(L.stage == ST_FETCH) {
-            uint32_t c;
-            if(!fetch_chunk(P, xcd, &c)) { L.lstate = LS_DONE; break; }
+        if(L.stage == ST_FETCH) {
+          uint32_t c;
+          if(!fetch_chunk(P, xcd, &c)) {
+            L.lstate = LS_DONE;
+          } else {
             L.chunk = c;
             const uint32_t m = c & 63u, rest = c >> 6;
             const uint32_t s = rest % P.nsub, t = rest / P.nsub;
             const uint32_t x = (t % P.ntx) * 8u + ((m & 1u) | ((m >> 1) & 2u) | ((m >> 2) & 4u));
             const uint32_t y = (t / P.ntx) * 8u + (((m >> 1) & 1u) | ((m >> 2) & 2u) | ((m >> 3) & 4u));
-            if(x >= P.width || y >= P.height) continue; /* outside the image */
-            L.pixel = y * P.width + x;
-            L.r = P.spp_offset + s * P.K;
-            L.r_end = L.r + P.K;
-            L.n = L.nfail = 0;
-            L.sum = L.sum_c = L.sum2 = L.sum2_c = 0.0;
-            L.tsum = L.tsum2 = 0.f;
+            if(x < P.width && y < P.height) {
+              L.pixel = y * P.width + x;
+              L.r = P.spp_offset + s * P.K;
+              L.r_end = L.r + P.K;
+              L.n = L.nfail = 0;
+              L.sum = L.sum_c = L.sum2 = L.sum2_c = 0.0;
+              L.tsum = L.tsum2 = 0.f;
+              L.stage = ST_BEGIN;
+            } /* else: outside the image, fetch again next time */
+          }
+        }
+      }
+      if(L.lstate == LS_EVENT && L.stage != ST_FETCH) {
+        const bool begin = L.stage == ST_BEGIN;
+        if(begin) { L.pair = 0; L.t0 = wall_clock64(); }
+        /* one set of canonicals per event, drawn at a single site */
+        float u[2 * NP];
+#pragma unroll
+        for(int i = 0; i < NP; ++i) { const float2 v = draw2(L, key); u[2 * i] = v.x; u[2 * i + 1] = v.y; }
+        if(begin) {
+          real_begin<SENSOR, SPECTRAL>(S, M, L, u);
+        } else if(L.pending) {
+          L.pending = false;
+          consume<COUNT>(S, L, u);
+        }
+        while(L.stage == ST_WALK_DONE) walk_done<SENSOR, SPECTRAL>(S, L);
+        if(L.stage == ST_REAL_DONE) {
+          const uint64_t t1 = wall_clock64();
+          if(L.w >= 0.f) {
+            const double wd = (double)L.w;
+            const float us = (float)((double)(t1 - L.t0) * M.us_per_tick);
+            nsum(L.sum, L.sum_c, wd);
+            nsum(L.sum2, L.sum2_c, wd * wd);
+            L.tsum += us;
+            L.tsum2 += us * us;
+            L.n++;
+          }
+          L.r++;
+          if(L.r < L.r_end) {
             L.stage = ST_BEGIN;
-          }
-          if(L.stage == ST_BEGIN) {
-            key[1] = L.pixel;
-            key[2] = L.r;
-            L.t0 = wall_clock64();
-            real_begin<SENSOR, SPECTRAL>(S, M, L, key);
-          } else if(L.pending) {
-            key[1] = L.pixel;
-            key[2] = L.r;
-            consume<COUNT>(S, L, key);
-          }
-          while(L.stage == ST_WALK_DONE) walk_done<SENSOR, SPECTRAL>(S, L, key);
-          if(L.stage == ST_REAL_DONE) {
-            const uint64_t t1 = wall_clock64();
-            if(L.w >= 0.f) {
-              const double wd = (double)L.w;
-              const float us = (float)((double)(t1 - L.t0) * M.us_per_tick);
-              nsum(L.sum, L.sum_c, wd);
-              nsum(L.sum2, L.sum2_c, wd * wd);
-              L.tsum += us;
-              L.tsum2 += us * us;
-              L.n++;
-            }
-            L.r++;
-            if(L.r < L.r_end) { L.stage = ST_BEGIN; continue; }
+          } else {
             htgx_mc_accum a;
             a.sum = L.sum; a.sum_c = L.sum_c; a.sum2 = L.sum2; a.sum2_c = L.sum2_c;
             a.tsum = L.tsum; a.tsum2 = L.tsum2; a.n = L.n; a.nfail = L.nfail;
             P.partials[L.chunk] = a;
             L.stage = ST_FETCH;
-            continue;
           }
-          break; /* a query stage: tracking resumes next iteration */
+        }
+        /* set-up of the requested query (single site) */
+        if(L.nq) {
+          L.nq = false;
+          if(L.nq_ground) {
+            float t = FLT_MAX;
+            L.gh_hit = ground_planef(S, L.pos, L.dir, FLT_MAX, L.from_ground, &t);
+            L.gh_mtl = S.plane_mtl;
+            L.qr = L.gh_hit ? t : FLT_MAX;
+          }
+          L.Ts = exp1f(u[2 * NP - 1]);
+          L.Tmin = 0.0;
+          L.pending = false;
+          iter_init(S, L);
+          L.lstate = LS_SEG;
         }
       }
     }
@@ -918,6 +943,13 @@ wavefront_kernel(SceneF S, DevMap M, Launch P)
     const uint64_t a = wave_sum64(L.c_leaf), b = wave_sum64(L.c_seg), c = wave_sum64(L.c_coll),
                    d = wave_sum64(L.c_lwc), e = wave_sum64(L.c_ev);
     if((threadIdx.x & 63u) == 0) {
+      atomicAdd(P.stats + 8, (unsigned long long)d_it);
+      atomicAdd(P.stats + 9, (unsigned long long)d_ce);
+      atomicAdd(P.stats + 10, (unsigned long long)d_cl);
+      atomicAdd(P.stats + 11, (unsigned long long)d_se);
+      atomicAdd(P.stats + 12, (unsigned long long)d_sl);
+      atomicAdd(P.stats + 13, (unsigned long long)d_ee);
+      atomicAdd(P.stats + 14, (unsigned long long)d_el);
       atomicAdd(P.stats + 1, (unsigned long long)a);
       atomicAdd(P.stats + 2, (unsigned long long)b);
       atomicAdd(P.stats + 3, (unsigned long long)c);
@@ -1141,7 +1173,7 @@ int htgx_render_wavefront(htgx_ctx* ctx, const DevMap& M, htgx_mc_accum* accum, 
   zero_u32_kernel<<<1, 64, 0, stream>>>(st->d_counters, kXcds);
   HIP_OK(hipGetLastError());
   if(ctx->counting) {
-    zero_u64_kernel<<<1, 64, 0, stream>>>(ctx->d_counters, 8);
+    zero_u64_kernel<<<1, 64, 0, stream>>>(ctx->d_counters, 16);
     HIP_OK(hipGetLastError());
     if((res = launch<true>(st->S, M, P, st->grid, stream))) return res;
   } else {

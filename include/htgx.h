@@ -285,6 +285,10 @@ struct htgx_counters {
   uint64_t nevents; /* scattering / surface events */
   uint64_t nseg;    /* segments handed to the null-collision loops */
   uint64_t nlwc;    /* cloud LWC fetches (4 B each) */
+  /* Wavefront solver only: loop iterations summed over wavefronts, then for
+   * its collision, segment and event branches the number of wavefront
+   * executions and the number of active lanes summed over them. */
+  uint64_t diag[7];
 };
 int htgx_set_counting(struct htgx_ctx* ctx, int enable);
 int htgx_get_counters(struct htgx_ctx* ctx, struct htgx_counters* counters);
@@ -343,6 +347,11 @@ int htgx_debug_sample_dirs
   (struct htgx_ctx* ctx,
    int kind, double param, const double dir_in[3],
    const double* u, uint32_t n, double* out);
+
+/* Copy nbytes (multiple of 16, 16-byte aligned DEVICE pointers) with a plain
+ * streaming kernel on `stream': the known-size reference used to calibrate
+ * the HBM traffic counters of the profiler. Asynchronous. */
+int htgx_debug_copy(void* dst, const void* src, size_t nbytes, void* stream);
 
 #ifdef __cplusplus
 }
