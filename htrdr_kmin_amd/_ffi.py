@@ -100,7 +100,7 @@ class Segment(C.Structure):
 class Counters(C.Structure):
     _fields_ = [("nrealisations", C.c_uint64), ("nleaf", C.c_uint64), ("ncoll", C.c_uint64),
                 ("nbvh", C.c_uint64), ("ntri", C.c_uint64), ("nevents", C.c_uint64),
-                ("nseg", C.c_uint64), ("nlwc", C.c_uint64), ("diag", C.c_uint64 * 7)]
+                ("nseg", C.c_uint64), ("nlwc", C.c_uint64), ("diag", C.c_uint64 * 16)]
 
 
 class SynthParams(C.Structure):
@@ -135,6 +135,7 @@ SIGNATURES = {
     "htgx_debug_realisations": (C.c_int, [C.c_void_p, C.POINTER(MapDesc), c_uint64_p,
                                           c_uint64_p, C.c_uint32, c_double_p]),
     "htgx_debug_threefry": (C.c_int, [c_uint64_p, c_uint64_p, C.c_uint32, c_uint64_p]),
+    "htgx_debug_threefry2x32": (C.c_int, [c_uint32_p, c_uint32_p, C.c_uint32, c_uint32_p]),
     "htgx_debug_sample_dirs": (C.c_int, [C.c_void_p, C.c_int, C.c_double, c_double_p,
                                          c_double_p, C.c_uint32, c_double_p]),
     "htgx_debug_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),

@@ -188,6 +188,18 @@ def threefry(ctr, key):
     return out
 
 
+def threefry2x32(ctr, key):
+    """Threefry-2x32-20 blocks computed on the device."""
+    lib = F.load()
+    ctr = np.ascontiguousarray(ctr, np.uint32).reshape(-1, 2)
+    key = np.ascontiguousarray(key, np.uint32).reshape(-1, 2)
+    out = np.zeros_like(ctr)
+    F.check("htgx_debug_threefry2x32", lib.htgx_debug_threefry2x32(
+        ctr.ctypes.data_as(F.c_uint32_p), key.ctypes.data_as(F.c_uint32_p), ctr.shape[0],
+        out.ctypes.data_as(F.c_uint32_p)))
+    return out
+
+
 def finalize(desc, accum):
     """Reference pixel structs from a host copy of the device accumulators."""
     lib = F.load()

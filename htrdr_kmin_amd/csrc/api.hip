@@ -205,6 +205,14 @@ threefry_kernel(const uint64_t* ctr, const uint64_t* key, uint32_t n, uint64_t* 
 }
 
 __global__ void
+threefry2x32_kernel(const uint32_t* ctr, const uint32_t* key, uint32_t n, uint32_t* out)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  threefry2x32_20(ctr + 2 * (size_t)i, key + 2 * (size_t)i, out + 2 * (size_t)i);
+}
+
+__global__ void
 sample_dirs_kernel(DevScene S, int kind, double param, double dx, double dy, double dz,
                    const double* u, uint32_t n, double* out)
 {
@@ -389,8 +397,8 @@ htgx_ctx_init(htgx_ctx* ctx, const htgx_scene_desc& sc)
     S.sun_solid_angle = 2.0 * kPi * (1.0 - S.sun_cos_half);
   }
 
-  HIP_OK(hipMalloc(&ctx->d_counters, 16 * sizeof(unsigned long long)));
-  HIP_OK(hipMemset(ctx->d_counters, 0, 16 * sizeof(unsigned long long)));
+  HIP_OK(hipMalloc(&ctx->d_counters, HTGX_NSTATS * sizeof(unsigned long long)));
+  HIP_OK(hipMemset(ctx->d_counters, 0, HTGX_NSTATS * sizeof(unsigned long long)));
   return HTGX_OK;
 }
 
@@ -565,7 +573,7 @@ render_plain(htgx_ctx* ctx, const DevMap& M, htgx_mc_accum* accum, hipStream_t s
   const uint64_t nthreads = items_per_slice * nslices;
   const uint32_t nblocks = (uint32_t)((nthreads + 255) / 256);
   if(ctx->counting) {
-    HIP_OK(hipMemsetAsync(ctx->d_counters, 0, 16 * sizeof(unsigned long long), stream));
+    HIP_OK(hipMemsetAsync(ctx->d_counters, 0, HTGX_NSTATS * sizeof(unsigned long long), stream));
     plain_kernel<true><<<nblocks, 256, 0, stream>>>(ctx->S, M, ntx, (uint32_t)items_per_slice,
       nslices, spt, (htgx_mc_accum*)ctx->d_partials, ctx->d_counters);
   } else {
@@ -604,7 +612,7 @@ extern "C" int
 htgx_get_counters(htgx_ctx* ctx, htgx_counters* c)
 {
   if(!ctx || !c) return HTGX_BAD_ARG;
-  unsigned long long h[16];
+  unsigned long long h[HTGX_NSTATS];
   HIP_OK(hipDeviceSynchronize());
   HIP_OK(hipMemcpy(h, ctx->d_counters, sizeof(h), hipMemcpyDeviceToHost));
   c->nrealisations = ctx->last_nrealisations;
@@ -615,7 +623,7 @@ htgx_get_counters(htgx_ctx* ctx, htgx_counters* c)
   c->nevents = h[5];
   c->nseg = h[2];
   c->nlwc = h[4];
-  for(int i = 0; i < 7; ++i) c->diag[i] = h[8 + i];
+  for(int i = 0; i < 16; ++i) c->diag[i] = h[8 + i];
   return HTGX_OK;
 }
 
@@ -729,12 +737,30 @@ htgx_draw_map(htgx_ctx* ctx, const htgx_map_desc* desc, void* pixels, size_t pit
 /*******************************************************************************
  * Debug entry points
  ******************************************************************************/
+/* Device array owned for the duration of a debug call: freed on every
+ * return path */
 template<class T>
-static int dalloc(T** p, size_t n)
-{
-  HIP_OK(hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T)));
-  return HTGX_OK;
-}
+struct DevArray {
+  T* p = nullptr;
+  ~DevArray() { if(p) (void)hipFree(p); }
+  int alloc(size_t n)
+  {
+    HIP_OK(hipMalloc((void**)&p, std::max<size_t>(n, 1) * sizeof(T)));
+    return HTGX_OK;
+  }
+  int upload(const T* h, size_t n)
+  {
+    int res;
+    if((res = alloc(n))) return res;
+    HIP_OK(hipMemcpy(p, h, n * sizeof(T), hipMemcpyHostToDevice));
+    return HTGX_OK;
+  }
+  int download(T* h, size_t n) const
+  {
+    HIP_OK(hipMemcpy(h, p, n * sizeof(T), hipMemcpyDeviceToHost));
+    return HTGX_OK;
+  }
+};
 
 extern "C" int
 htgx_debug_trace_segments(htgx_ctx* ctx, const double* rays, uint32_t nrays, int prop,
@@ -743,20 +769,18 @@ htgx_debug_trace_segments(htgx_ctx* ctx, const double* rays, uint32_t nrays, int
 {
   if(!ctx || !rays || !segs || !nsegs || !nrays || !max_segs) return HTGX_BAD_ARG;
   if(prop < HTGX_Ka || prop > HTGX_Kext || iband >= ctx->S.nbands) return HTGX_BAD_ARG;
-  double* d_rays; htgx_segment* d_segs; uint32_t* d_n;
+  DevArray<double> d_rays;
+  DevArray<htgx_segment> d_segs;
+  DevArray<uint32_t> d_n;
   int res;
-  if((res = dalloc(&d_rays, (size_t)nrays * 8))) return res;
-  if((res = dalloc(&d_segs, (size_t)nrays * max_segs))) return res;
-  if((res = dalloc(&d_n, nrays))) return res;
-  HIP_OK(hipMemcpy(d_rays, rays, (size_t)nrays * 8 * sizeof(double), hipMemcpyHostToDevice));
-  trace_segments_kernel<<<(nrays + 127) / 128, 128>>>(ctx->S, d_rays, nrays, prop, iband,
-                                                       iquad, d_segs, max_segs, d_n);
+  if((res = d_rays.upload(rays, (size_t)nrays * 8))) return res;
+  if((res = d_segs.alloc((size_t)nrays * max_segs))) return res;
+  if((res = d_n.alloc(nrays))) return res;
+  trace_segments_kernel<<<(nrays + 127) / 128, 128>>>(ctx->S, d_rays.p, nrays, prop, iband,
+                                                       iquad, d_segs.p, max_segs, d_n.p);
   HIP_OK(hipGetLastError());
-  HIP_OK(hipMemcpy(segs, d_segs, (size_t)nrays * max_segs * sizeof(htgx_segment),
-                   hipMemcpyDeviceToHost));
-  HIP_OK(hipMemcpy(nsegs, d_n, nrays * sizeof(uint32_t), hipMemcpyDeviceToHost));
-  HIP_OK(hipFree(d_rays)); HIP_OK(hipFree(d_segs)); HIP_OK(hipFree(d_n));
-  return HTGX_OK;
+  if((res = d_segs.download(segs, (size_t)nrays * max_segs))) return res;
+  return d_n.download(nsegs, nrays);
 }
 
 extern "C" int
@@ -771,35 +795,42 @@ htgx_debug_realisations(htgx_ctx* ctx, const htgx_map_desc* desc, const uint64_t
   DevMap M;
   int res;
   if((res = htgx_map_setup(ctx, desc, &M)) != HTGX_OK) return res;
-  uint64_t *d_p, *d_r; double* d_w;
-  if((res = dalloc(&d_p, n))) return res;
-  if((res = dalloc(&d_r, n))) return res;
-  if((res = dalloc(&d_w, n))) return res;
-  HIP_OK(hipMemcpy(d_p, pix, n * sizeof(uint64_t), hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(d_r, real, n * sizeof(uint64_t), hipMemcpyHostToDevice));
-  realisations_kernel<<<(n + 63) / 64, 64>>>(ctx->S, M, d_p, d_r, n, d_w);
+  DevArray<uint64_t> d_p, d_r;
+  DevArray<double> d_w;
+  if((res = d_p.upload(pix, n))) return res;
+  if((res = d_r.upload(real, n))) return res;
+  if((res = d_w.alloc(n))) return res;
+  realisations_kernel<<<(n + 63) / 64, 64>>>(ctx->S, M, d_p.p, d_r.p, n, d_w.p);
   HIP_OK(hipGetLastError());
-  HIP_OK(hipMemcpy(weights, d_w, n * sizeof(double), hipMemcpyDeviceToHost));
-  HIP_OK(hipFree(d_p)); HIP_OK(hipFree(d_r)); HIP_OK(hipFree(d_w));
-  return HTGX_OK;
+  return d_w.download(weights, n);
 }
 
 extern "C" int
 htgx_debug_threefry(const uint64_t* ctr, const uint64_t* key, uint32_t n, uint64_t* out)
 {
   if(!ctr || !key || !out || !n) return HTGX_BAD_ARG;
-  uint64_t *d_c, *d_k, *d_o;
+  DevArray<uint64_t> d_c, d_k, d_o;
   int res;
-  if((res = dalloc(&d_c, 4 * (size_t)n))) return res;
-  if((res = dalloc(&d_k, 4 * (size_t)n))) return res;
-  if((res = dalloc(&d_o, 4 * (size_t)n))) return res;
-  HIP_OK(hipMemcpy(d_c, ctr, 4 * (size_t)n * sizeof(uint64_t), hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(d_k, key, 4 * (size_t)n * sizeof(uint64_t), hipMemcpyHostToDevice));
-  threefry_kernel<<<(n + 255) / 256, 256>>>(d_c, d_k, n, d_o);
+  if((res = d_c.upload(ctr, 4 * (size_t)n))) return res;
+  if((res = d_k.upload(key, 4 * (size_t)n))) return res;
+  if((res = d_o.alloc(4 * (size_t)n))) return res;
+  threefry_kernel<<<(n + 255) / 256, 256>>>(d_c.p, d_k.p, n, d_o.p);
   HIP_OK(hipGetLastError());
-  HIP_OK(hipMemcpy(out, d_o, 4 * (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToHost));
-  HIP_OK(hipFree(d_c)); HIP_OK(hipFree(d_k)); HIP_OK(hipFree(d_o));
-  return HTGX_OK;
+  return d_o.download(out, 4 * (size_t)n);
+}
+
+extern "C" int
+htgx_debug_threefry2x32(const uint32_t* ctr, const uint32_t* key, uint32_t n, uint32_t* out)
+{
+  if(!ctr || !key || !out || !n) return HTGX_BAD_ARG;
+  DevArray<uint32_t> d_c, d_k, d_o;
+  int res;
+  if((res = d_c.upload(ctr, 2 * (size_t)n))) return res;
+  if((res = d_k.upload(key, 2 * (size_t)n))) return res;
+  if((res = d_o.alloc(2 * (size_t)n))) return res;
+  threefry2x32_kernel<<<(n + 255) / 256, 256>>>(d_c.p, d_k.p, n, d_o.p);
+  HIP_OK(hipGetLastError());
+  return d_o.download(out, 2 * (size_t)n);
 }
 
 extern "C" int
@@ -807,17 +838,14 @@ htgx_debug_sample_dirs(htgx_ctx* ctx, int kind, double param, const double dir_i
                        const double* u, uint32_t n, double* out)
 {
   if(!ctx || !dir_in || !u || !out || !n || kind < 0 || kind > 3) return HTGX_BAD_ARG;
-  double *d_u, *d_o;
+  DevArray<double> d_u, d_o;
   int res;
-  if((res = dalloc(&d_u, 2 * (size_t)n))) return res;
-  if((res = dalloc(&d_o, 3 * (size_t)n))) return res;
-  HIP_OK(hipMemcpy(d_u, u, 2 * (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+  if((res = d_u.upload(u, 2 * (size_t)n))) return res;
+  if((res = d_o.alloc(3 * (size_t)n))) return res;
   sample_dirs_kernel<<<(n + 255) / 256, 256>>>(ctx->S, kind, param, dir_in[0], dir_in[1],
-                                               dir_in[2], d_u, n, d_o);
+                                               dir_in[2], d_u.p, n, d_o.p);
   HIP_OK(hipGetLastError());
-  HIP_OK(hipMemcpy(out, d_o, 3 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
-  HIP_OK(hipFree(d_u)); HIP_OK(hipFree(d_o));
-  return HTGX_OK;
+  return d_o.download(out, 3 * (size_t)n);
 }
 
 extern "C" int

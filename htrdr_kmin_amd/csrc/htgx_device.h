@@ -639,8 +639,9 @@ HTGX_D void cos_hemisphere(const double n[3], double u0, double u1, double out[3
 HTGX_D double layer_temperature(const DevScene& S, int32_t j)
 {
   const int32_t nl = (int32_t)S.nlayers;
+  if(!S.gas_T || nl <= 0) return 0.0; /* no gas layers: no emission */
   j = j < 0 ? 0 : j >= nl ? nl - 1 : j;
-  return S.gas_T ? S.gas_T[j] : 0.0;
+  return S.gas_T[j];
 }
 
 /*******************************************************************************

@@ -10,6 +10,9 @@
 
 struct htgx_wavefront_state;
 
+/* Device counter slots: [1..5] walk counters, [8..23] wavefront diagnostics */
+#define HTGX_NSTATS 32
+
 struct htgx_ctx {
   int device = 0;
   htgx::DevScene S;
@@ -31,7 +34,7 @@ struct htgx_ctx {
   double w_Tref = 0;
   /* event counters */
   int counting = 0;
-  unsigned long long* d_counters = nullptr;
+  unsigned long long* d_counters = nullptr; /* HTGX_NSTATS slots */
   uint64_t last_nrealisations = 0, last_ncell = 0, last_nseg = 0;
   /* FP32 wavefront solver state */
   htgx_wavefront_state* wf = nullptr;

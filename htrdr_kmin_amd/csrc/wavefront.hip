@@ -145,7 +145,7 @@ struct Lane {
   int32_t ev_layer;
   uint32_t nsteps;
   /* segment iterator */
-  int cstate;          /* 0 before grid, 1 in leaf, 2 after, 3 to locate */
+  int cstate;          /* 0 before grid, 1 in leaf, 2 outside, 3 locate at tit, 4 step out of leaf */
   float tit, tnext;    /* iterator position; grid entry or leaf exit */
   uint32_t lohi[3];    /* leaf block range lo | hi << 16 */
   uint32_t leaf;
@@ -154,11 +154,11 @@ struct Lane {
   int32_t j;
   bool itdone, adv, adv_g, adv_c, adv_end;
   /* current segment */
-  float s_t1, s_kmin, s_kmax;
+  float s_t1, s_kmin, s_kmax, s_gka, s_gks; /* gas ka / ks of the segment's layer */
   bool s_cloud;
   int32_t s_layer;
   /* chunk accumulator */
-  double sum, sum_c, sum2, sum2_c;
+  double sum, sum2;     /* plain FP64 sums over the few realisations of a chunk */
   float tsum, tsum2;
   uint32_t n, nfail;
   /* counters */
@@ -300,32 +300,24 @@ __device__ __forceinline__ void load_leaf(const SceneF& S, Lane& L, uint32_t cx,
   if(COUNT) L.c_leaf++;
 }
 
-template<bool COUNT>
-__device__ __forceinline__ void locate_leaf(const SceneF& S, Lane& L, float at)
+/* Cell to load when the iterator is in state 3 (locate the block holding the
+ * point at tit, clamped into the grid) or 4 (step across the exit face of
+ * the current leaf, other axes clamped into its block range); false when the
+ * step leaves the grid. One code path for both, so that a wavefront pays for
+ * a single leaf load per pass whatever mix of states its lanes are in. */
+__device__ __forceinline__ bool next_cell(const SceneF& S, const Lane& L, uint32_t c[3])
 {
-  uint32_t c[3];
+  const bool step = L.cstate == 4;
 #pragma unroll
   for(int a = 0; a < 3; ++a) {
-    const float p = L.pos[a] + at * L.dir[a];
-    const float f = floorf((p - S.lower[a]) * S.inv_bs[a]);
-    c[a] = f < 0.f ? 0u : f > (float)(S.dims0[a] - 1) ? S.dims0[a] - 1 : (uint32_t)f;
-  }
-  load_leaf<COUNT>(S, L, c[0], c[1], c[2]);
-}
-
-template<bool COUNT>
-__device__ __forceinline__ void step_leaf(const SceneF& S, Lane& L)
-{
-  uint32_t c[3];
-#pragma unroll
-  for(int a = 0; a < 3; ++a) {
-    const uint32_t lo = L.lohi[a] & 0xFFFFu, hi = L.lohi[a] >> 16;
-    if(a == L.exit_axis) {
+    const uint32_t lo = step ? L.lohi[a] & 0xFFFFu : 0u;
+    const uint32_t hi = step ? L.lohi[a] >> 16 : S.dims0[a];
+    if(step && a == L.exit_axis) {
       if(L.dir[a] > 0.f) {
-        if(hi >= S.dims0[a]) { L.cstate = 2; return; }
+        if(hi >= S.dims0[a]) return false;
         c[a] = hi;
       } else {
-        if(lo == 0) { L.cstate = 2; return; }
+        if(lo == 0) return false;
         c[a] = lo - 1;
       }
     } else {
@@ -334,7 +326,7 @@ __device__ __forceinline__ void step_leaf(const SceneF& S, Lane& L)
       c[a] = f < (float)lo ? lo : f > (float)(hi - 1) ? hi - 1 : (uint32_t)f;
     }
   }
-  load_leaf<COUNT>(S, L, c[0], c[1], c[2]);
+  return true;
 }
 
 /* Start the iterator of the query over [0, qr] along (pos, dir); no memory
@@ -374,17 +366,17 @@ __device__ __forceinline__ void iter_init(const SceneF& S, Lane& L)
 }
 
 /* Move the iterator past a boundary at t1 (gas layer if adv_g, leaf or grid
- * entry if adv_c) */
-template<bool COUNT>
-__device__ __forceinline__ void iter_advance(const SceneF& S, Lane& L, float t1, bool adv_g,
-                                             bool adv_c, bool at_end)
+ * entry if adv_c). No memory access: a leaf boundary only marks the leaf to
+ * load next (state 3 or 4). */
+__device__ __forceinline__ void iter_advance(Lane& L, float t1, bool adv_g, bool adv_c,
+                                             bool at_end)
 {
   if(t1 >= L.qr) { L.itdone = true; return; }
   L.tit = t1;
   if(adv_g) L.j += L.dir[2] > 0.f ? 1 : -1;
   if(adv_c) {
-    if(L.cstate == 0) { L.cstate = 1; locate_leaf<COUNT>(S, L, t1); }
-    else if(L.cstate == 1) step_leaf<COUNT>(S, L);
+    if(L.cstate == 0) L.cstate = 3;
+    else if(L.cstate == 1) L.cstate = 4;
   }
   if(at_end) L.itdone = true;
 }
@@ -392,17 +384,22 @@ __device__ __forceinline__ void iter_advance(const SceneF& S, Lane& L, float t1,
 /* Next segment holding some medium into L.s_*; false at the end of the range.
  * The iterator only moves past an emitted segment at the next call, so that
  * the leaf range used to clamp the cells of its collisions stays the one of
- * the segment. */
+ * the segment. Each pass of the loop loads at most one leaf, at one site. */
 template<bool COUNT>
 __device__ __forceinline__ bool iter_next(const SceneF& S, Lane& L, int prop, float* t0)
 {
-  if(L.cstate == 3) { locate_leaf<COUNT>(S, L, L.tit); L.cstate = 1; }
   if(L.adv) {
     L.adv = false;
-    iter_advance<COUNT>(S, L, L.s_t1, L.adv_g, L.adv_c, L.adv_end);
+    iter_advance(L, L.s_t1, L.adv_g, L.adv_c, L.adv_end);
   }
   const float tmax = L.qr;
-  while(!L.itdone && L.tit < tmax) {
+  for(;;) {
+    if(L.itdone || !(L.tit < tmax)) return false;
+    if(L.cstate >= 3) {
+      uint32_t c[3];
+      if(next_cell(S, L, c)) { load_leaf<COUNT>(S, L, c[0], c[1], c[2]); L.cstate = 1; }
+      else L.cstate = 2;
+    }
     const float t = L.tit;
     const float tc = L.cstate == 0 ? L.tnext : L.cstate == 1 ? fmaxf(L.tnext, t) : INFINITY;
     float tg;
@@ -412,39 +409,38 @@ __device__ __forceinline__ bool iter_next(const SceneF& S, Lane& L, int prop, fl
     tg = fmaxf(tg, t);
     const float t1 = fminf(fminf(tc, tg), tmax);
     const bool at_end = tg == INFINITY && tc == INFINITY;
-    if(t1 > t) {
-      const bool in_gas = L.j >= 0 && L.j < (int32_t)S.nlayers;
-      const bool in_cloud = L.cstate == 1;
-      if(in_gas || in_cloud) {
-        float gk = 0.f, cmn = 0.f, cmx = 0.f;
-        if(in_gas) {
-          const float ka = S.gas_ka[L.gka_off + (uint32_t)L.j];
-          const float ks = S.gas_ks[L.gks_off + (uint32_t)L.j];
-          gk = prop == HTGX_Ka ? ka : prop == HTGX_Ks ? ks : ka + ks;
-        }
-        if(in_cloud) {
-          const float c = prop == HTGX_Ka ? L.cka : prop == HTGX_Ks ? L.cks : L.cka + L.cks;
-          cmn = c * L.lmn;
-          cmx = c * L.lmx;
-        }
-        *t0 = t;
-        L.s_t1 = t1;
-        L.s_kmin = gk + cmn;
-        L.s_kmax = gk + cmx;
-        L.s_cloud = in_cloud;
-        L.s_layer = in_gas ? L.j : -1;
-        L.adv = true;
-        L.adv_g = tg == t1;
-        L.adv_c = tc == t1;
-        L.adv_end = at_end;
-        if(COUNT) L.c_seg++;
-        return true;
+    const bool in_gas = L.j >= 0 && L.j < (int32_t)S.nlayers;
+    const bool in_cloud = L.cstate == 1;
+    if(t1 > t && (in_gas || in_cloud)) {
+      float gka = 0.f, gks = 0.f, cmn = 0.f, cmx = 0.f;
+      if(in_gas) {
+        gka = S.gas_ka[L.gka_off + (uint32_t)L.j];
+        gks = S.gas_ks[L.gks_off + (uint32_t)L.j];
       }
+      if(in_cloud) {
+        const float c = prop == HTGX_Ka ? L.cka : prop == HTGX_Ks ? L.cks : L.cka + L.cks;
+        cmn = c * L.lmn;
+        cmx = c * L.lmx;
+      }
+      const float gk = prop == HTGX_Ka ? gka : prop == HTGX_Ks ? gks : gka + gks;
+      *t0 = t;
+      L.s_t1 = t1;
+      L.s_kmin = gk + cmn;
+      L.s_kmax = gk + cmx;
+      L.s_gka = gka;
+      L.s_gks = gks;
+      L.s_cloud = in_cloud;
+      L.s_layer = in_gas ? L.j : -1;
+      L.adv = true;
+      L.adv_g = tg == t1;
+      L.adv_c = tc == t1;
+      L.adv_end = at_end;
+      if(COUNT) L.c_seg++;
+      return true;
     }
-    iter_advance<COUNT>(S, L, t1, tg == t1, tc == t1, at_end);
-    if(++L.nsteps > 4096u) { L.itdone = true; break; } /* safety bound */
+    iter_advance(L, t1, tg == t1, tc == t1, at_end);
+    if(++L.nsteps > 4096u) { L.itdone = true; return false; } /* safety bound */
   }
-  return false;
 }
 
 /* LWC of the cell holding x, clamped into the current segment's leaf */
@@ -604,7 +600,8 @@ __device__ __forceinline__ void consume(const SceneF& S, Lane& L, const float* u
       if(L.ev == EV_ABS) {
         const int32_t nl = (int32_t)S.nlayers;
         const int32_t jj = L.ev_layer < 0 ? 0 : L.ev_layer >= nl ? nl - 1 : L.ev_layer;
-        L.w = planckf(L.wlen, S.gas_T[jj]);
+        /* no gas layers: no emission temperature */
+        L.w = S.gas_T && nl > 0 ? planckf(L.wlen, S.gas_T[jj]) : 0.f;
         L.stage = ST_WALK_DONE;
         break;
       }
@@ -766,17 +763,23 @@ wavefront_kernel(SceneF S, DevMap M, Launch P)
   L.nq = false;
   L.c_leaf = L.c_seg = L.c_coll = L.c_lwc = L.c_ev = 0;
   L.n = L.nfail = 0;
-  L.sum = L.sum_c = L.sum2 = L.sum2_c = 0.0;
+  L.sum = L.sum2 = 0.0;
   L.tsum = L.tsum2 = 0.f;
   L.pixel = L.r = 0;
   const uint32_t xcd = blockIdx.x % (uint32_t)kXcds;
   const uint32_t key = (uint32_t)M.seed ^ (uint32_t)(M.seed >> 32);
   uint64_t d_it = 0, d_ce = 0, d_cl = 0, d_se = 0, d_sl = 0, d_ee = 0, d_el = 0;
+  /* shader clock cycles per phase (counting build only) */
+  uint64_t cy[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  uint64_t tk = 0;
+#define WF_TICK(i)                                                             \
+  if(COUNT) { const uint64_t now = clock64(); cy[i] += now - tk; tk = now; }
 
   for(;;) {
     const uint64_t want0 = __ballot(L.lstate == LS_EVENT);
     const uint64_t busy0 = __ballot(L.lstate == LS_SEG || L.lstate == LS_COLL);
     if(!(busy0 | want0)) break;
+    if(COUNT) tk = clock64();
 
     /* next segment of the query, and the first tentative collision in it */
     if(COUNT) {
@@ -806,6 +809,7 @@ wavefront_kernel(SceneF S, DevMap M, Launch P)
         L.lstate = LS_EVENT;
       }
     }
+    WF_TICK(0)
 
     /* one tentative collision */
     if(COUNT) {
@@ -822,9 +826,7 @@ wavefront_kernel(SceneF S, DevMap M, Launch P)
 #pragma unroll
         for(int a = 0; a < 3; ++a) x[a] = L.pos[a] + L.tc * L.dir[a];
         const float v = L.s_cloud ? seg_lwcf<COUNT>(S, L, x) : 0.f;
-        const bool gas = L.s_layer >= 0;
-        const float gka = gas ? S.gas_ka[L.gka_off + (uint32_t)L.s_layer] : 0.f;
-        const float gks = gas ? S.gas_ks[L.gks_off + (uint32_t)L.s_layer] : 0.f;
+        const float gka = L.s_gka, gks = L.s_gks;
         const float ka = gka + L.cka * v;
         const float ks = gks + L.cks * v;
         const float2 uu = draw2(L, key);
@@ -855,6 +857,7 @@ wavefront_kernel(SceneF S, DevMap M, Launch P)
         }
       }
     }
+    WF_TICK(1)
 
     /* events, when enough lanes need them or nothing else is left */
     const uint64_t want = __ballot(L.lstate == LS_EVENT);
@@ -877,13 +880,14 @@ wavefront_kernel(SceneF S, DevMap M, Launch P)
               L.r = P.spp_offset + s * P.K;
               L.r_end = L.r + P.K;
               L.n = L.nfail = 0;
-              L.sum = L.sum_c = L.sum2 = L.sum2_c = 0.0;
+              L.sum = L.sum2 = 0.0;
               L.tsum = L.tsum2 = 0.f;
               L.stage = ST_BEGIN;
             } /* else: outside the image, fetch again next time */
           }
         }
       }
+      WF_TICK(2)
       if(L.lstate == LS_EVENT && L.stage != ST_FETCH) {
         const bool begin = L.stage == ST_BEGIN;
         if(begin) { L.pair = 0; L.t0 = wall_clock64(); }
@@ -891,20 +895,25 @@ wavefront_kernel(SceneF S, DevMap M, Launch P)
         float u[2 * NP];
 #pragma unroll
         for(int i = 0; i < NP; ++i) { const float2 v = draw2(L, key); u[2 * i] = v.x; u[2 * i + 1] = v.y; }
+        WF_TICK(3)
         if(begin) {
           real_begin<SENSOR, SPECTRAL>(S, M, L, u);
-        } else if(L.pending) {
+        }
+        WF_TICK(4)
+        if(!begin && L.pending) {
           L.pending = false;
           consume<COUNT>(S, L, u);
         }
+        WF_TICK(5)
         while(L.stage == ST_WALK_DONE) walk_done<SENSOR, SPECTRAL>(S, L);
+        WF_TICK(6)
         if(L.stage == ST_REAL_DONE) {
           const uint64_t t1 = wall_clock64();
           if(L.w >= 0.f) {
             const double wd = (double)L.w;
             const float us = (float)((double)(t1 - L.t0) * M.us_per_tick);
-            nsum(L.sum, L.sum_c, wd);
-            nsum(L.sum2, L.sum2_c, wd * wd);
+            L.sum += wd;
+            L.sum2 += wd * wd;
             L.tsum += us;
             L.tsum2 += us * us;
             L.n++;
@@ -914,12 +923,13 @@ wavefront_kernel(SceneF S, DevMap M, Launch P)
             L.stage = ST_BEGIN;
           } else {
             htgx_mc_accum a;
-            a.sum = L.sum; a.sum_c = L.sum_c; a.sum2 = L.sum2; a.sum2_c = L.sum2_c;
+            a.sum = L.sum; a.sum_c = 0.0; a.sum2 = L.sum2; a.sum2_c = 0.0;
             a.tsum = L.tsum; a.tsum2 = L.tsum2; a.n = L.n; a.nfail = L.nfail;
             P.partials[L.chunk] = a;
             L.stage = ST_FETCH;
           }
         }
+        WF_TICK(7)
         /* set-up of the requested query (single site) */
         if(L.nq) {
           L.nq = false;
@@ -935,9 +945,11 @@ wavefront_kernel(SceneF S, DevMap M, Launch P)
           iter_init(S, L);
           L.lstate = LS_SEG;
         }
+        WF_TICK(8)
       }
     }
   }
+#undef WF_TICK
 
   if(COUNT) {
     const uint64_t a = wave_sum64(L.c_leaf), b = wave_sum64(L.c_seg), c = wave_sum64(L.c_coll),
@@ -950,6 +962,8 @@ wavefront_kernel(SceneF S, DevMap M, Launch P)
       atomicAdd(P.stats + 12, (unsigned long long)d_sl);
       atomicAdd(P.stats + 13, (unsigned long long)d_ee);
       atomicAdd(P.stats + 14, (unsigned long long)d_el);
+#pragma unroll
+      for(int i = 0; i < 9; ++i) atomicAdd(P.stats + 15 + i, (unsigned long long)cy[i]);
       atomicAdd(P.stats + 1, (unsigned long long)a);
       atomicAdd(P.stats + 2, (unsigned long long)b);
       atomicAdd(P.stats + 3, (unsigned long long)c);
@@ -1135,7 +1149,7 @@ wavefront_prepare(htgx_ctx* ctx)
   st->grid = (uint32_t)std::max(1, ncu * std::max(1, per_cu));
   if(const char* e = std::getenv("HTGX_WF_GRID")) st->grid = (uint32_t)std::max(1, atoi(e));
   if(const char* e = std::getenv("HTGX_WF_EVTHRESH")) st->ev_thresh = (uint32_t)std::max(1, atoi(e));
-  if(const char* e = std::getenv("HTGX_WF_CHUNK")) st->K = (uint32_t)std::max(1, atoi(e));
+  if(const char* e = std::getenv("HTGX_WF_CHUNK")) st->K = (uint32_t)std::min(16, std::max(1, atoi(e)));
   return HTGX_OK;
 }
 
@@ -1173,7 +1187,7 @@ int htgx_render_wavefront(htgx_ctx* ctx, const DevMap& M, htgx_mc_accum* accum, 
   zero_u32_kernel<<<1, 64, 0, stream>>>(st->d_counters, kXcds);
   HIP_OK(hipGetLastError());
   if(ctx->counting) {
-    zero_u64_kernel<<<1, 64, 0, stream>>>(ctx->d_counters, 16);
+    zero_u64_kernel<<<1, 64, 0, stream>>>(ctx->d_counters, HTGX_NSTATS);
     HIP_OK(hipGetLastError());
     if((res = launch<true>(st->S, M, P, st->grid, stream))) return res;
   } else {

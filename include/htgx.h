@@ -285,10 +285,13 @@ struct htgx_counters {
   uint64_t nevents; /* scattering / surface events */
   uint64_t nseg;    /* segments handed to the null-collision loops */
   uint64_t nlwc;    /* cloud LWC fetches (4 B each) */
-  /* Wavefront solver only: loop iterations summed over wavefronts, then for
-   * its collision, segment and event branches the number of wavefront
-   * executions and the number of active lanes summed over them. */
-  uint64_t diag[7];
+  /* Wavefront solver only: [0] loop iterations summed over wavefronts,
+   * [1..6] for its collision, segment and event branches the number of
+   * wavefront executions and the active lanes summed over them, [7..15]
+   * shader clock cycles summed over wavefronts spent in: segment step,
+   * collision step, chunk fetch, event draws, realisation start, event
+   * consumption, end of walk, end of realisation, query set-up. */
+  uint64_t diag[16];
 };
 int htgx_set_counting(struct htgx_ctx* ctx, int enable);
 int htgx_get_counters(struct htgx_ctx* ctx, struct htgx_counters* counters);
@@ -337,6 +340,11 @@ int htgx_debug_realisations
  * threefry(ctr[i][0..3], key[i][0..3]). HOST arrays. */
 int htgx_debug_threefry
   (const uint64_t* ctr, const uint64_t* key, uint32_t n, uint64_t* out);
+
+/* Threefry-2x32-20 blocks computed on the device (the generator of the FP32
+ * solver): out[i][0..1] = threefry(ctr[i][0..1], key[i][0..1]). HOST arrays. */
+int htgx_debug_threefry2x32
+  (const uint32_t* ctr, const uint32_t* key, uint32_t n, uint32_t* out);
 
 /* Direction samplers on the device (phase functions, sun cone, cosine
  * hemisphere). kind: 0 = HG(param) relative to dir_in, 1 = Rayleigh relative

@@ -1355,9 +1355,10 @@ static double
 layer_temperature(const struct oracle_sky* s, int32_t j)
 {
   const int32_t nl = (int32_t)s->scene.gas.nlayers;
+  if(!s->scene.gas.temperature || nl <= 0) return 0.0; /* no gas layers */
   if(j < 0) j = 0;
   if(j >= nl) j = nl - 1;
-  return s->scene.gas.temperature ? s->scene.gas.temperature[j] : 0.0;
+  return s->scene.gas.temperature[j];
 }
 
 /* ..._lw.c:140-279 */

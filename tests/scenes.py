@@ -1,0 +1,178 @@
+"""Hand-built scenes as ``struct htgx_scene_desc``, for tests with closed-form
+answers.
+
+A ``Scene`` keeps the numpy arrays its descriptor points to alive. Units are
+those of include/htgx.h: metres, nm, m^-1, g/m^3, K, degrees.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from htrdr_kmin_amd import _ffi as F
+
+# Constants of Planck's law as the reference writes them
+# (src/core/htrdr_spectral.h:102-116); scipy.constants holds the same
+# SI-defining values.
+H_PLANCK = 6.62607015e-34
+C_LIGHT = 299792458.0
+K_BOLTZ = 1.380649e-23
+SUN_HALF_ANGLE = 4.6524e-3  # rad, src/atmosphere/htrdr_atmosphere_sun.c:84-87
+
+
+def planck(lambda_m, T):
+    """B_lambda(T) in W/m^2/sr/m, numpy, from the physical constants."""
+    lam = np.asarray(lambda_m, np.float64)
+    return 2 * H_PLANCK * C_LIGHT ** 2 / lam ** 5 / np.expm1(H_PLANCK * C_LIGHT / (lam * K_BOLTZ * T))
+
+
+def sun_solid_angle():
+    return 2 * math.pi * (1 - math.cos(SUN_HALF_ANGLE))
+
+
+def sun_dir(azimuth_deg, elevation_deg):
+    az, el = math.radians(azimuth_deg), math.radians(elevation_deg)
+    return np.array([math.cos(el) * math.cos(az), math.cos(el) * math.sin(az), math.sin(el)])
+
+
+class Scene:
+    """Scene from arrays.
+
+    bands: [(lo, hi)] nm; quad_weights: one list per band.
+    ka: [sum nquads][nlayers] m^-1, ks: [nbands][nlayers] m^-1, T: [nlayers] K.
+    lwc: [nz][ny][nx] g/m^3 over [lower, upper]; cext/ssa/g: per band.
+    ground: None, or ("plane", z) with one material (albedo, temperature).
+    """
+
+    def __init__(self, *, bands=((550.0, 550.0),), quad_weights=None, nlayers=0, z_bottom=0.0,
+                 dz=400.0, ka=None, ks=None, T=None, lwc=None, lower=(0, 0, 0), upper=(1, 1, 1),
+                 cext=None, ssa=None, g=None, ground=None, albedo=0.3, ground_T=0.0,
+                 sun=(0.0, 60.0), optical_thickness=1.0, leaf_block=1):
+        nb = len(bands)
+        if quad_weights is None:
+            quad_weights = [[1.0]] * nb
+        self._keep = []
+        keep = self._keep.append
+
+        def arr(a, dt, ptr_t):
+            a = np.ascontiguousarray(a, dt)
+            keep(a)
+            return a.ctypes.data_as(ptr_t)
+
+        d = F.SceneDesc()
+        d.spectral.nbands = nb
+        d.spectral.band_lo = arr([b[0] for b in bands], np.float64, F.c_double_p)
+        d.spectral.band_hi = arr([b[1] for b in bands], np.float64, F.c_double_p)
+        d.spectral.nquads = arr([len(q) for q in quad_weights], np.uint32, F.c_uint32_p)
+        d.spectral.quad_weights = arr(np.concatenate([np.asarray(q, np.float64) for q in quad_weights]),
+                                      np.float64, F.c_double_p)
+        nq = sum(len(q) for q in quad_weights)
+        d.gas.nlayers = nlayers
+        d.gas.z_bottom = z_bottom
+        d.gas.layer_thickness = dz
+        if nlayers:
+            ka = np.zeros((nq, nlayers)) if ka is None else np.broadcast_to(ka, (nq, nlayers))
+            ks = np.zeros((nb, nlayers)) if ks is None else np.broadcast_to(ks, (nb, nlayers))
+            d.gas.ka = arr(ka, np.float64, F.c_double_p)
+            d.gas.ks = arr(ks, np.float64, F.c_double_p)
+            if T is not None:
+                d.gas.temperature = arr(np.broadcast_to(T, (nlayers,)), np.float64, F.c_double_p)
+        if lwc is not None:
+            lwc = np.ascontiguousarray(lwc, np.float32)
+            nz, ny, nx = lwc.shape
+            d.clouds.def_[0], d.clouds.def_[1], d.clouds.def_[2] = nx, ny, nz
+            for a in range(3):
+                d.clouds.lower[a] = lower[a]
+                d.clouds.upper[a] = upper[a]
+            d.clouds.lwc = arr(lwc, np.float32, F.c_float_p)
+            d.clouds.cext = arr(np.broadcast_to(0.15 if cext is None else cext, (nb,)), np.float64, F.c_double_p)
+            d.clouds.ssa = arr(np.broadcast_to(0.999 if ssa is None else ssa, (nb,)), np.float64, F.c_double_p)
+            d.clouds.hg_g = arr(np.broadcast_to(0.85 if g is None else g, (nb,)), np.float64, F.c_double_p)
+        if ground is not None:
+            kind, z = ground
+            assert kind == "plane"
+            d.ground.type = F.GROUND_PLANE
+            d.ground.plane_z = z
+            d.ground.plane_mtl = 0
+            d.ground.nmtls = 1
+            d.ground.mtl_albedo = arr([albedo], np.float64, F.c_double_p)
+            d.ground.mtl_temperature = arr([ground_T], np.float64, F.c_double_p)
+        else:
+            d.ground.type = F.GROUND_NONE
+        d.sun_azimuth, d.sun_elevation = sun
+        d.optical_thickness = optical_thickness
+        d.leaf_block = leaf_block
+        self.desc = d
+        self.lwc = lwc
+        self.lower = np.asarray(lower, np.float64)
+        self.upper = np.asarray(upper, np.float64)
+
+    @property
+    def ptr(self):
+        return C.pointer(self.desc)
+
+
+def camera(width, height, spp, pos, tgt, up=(0, 0, 1), fov=60.0, wlen=(550.0, 550.0),
+           spectral=F.SPECTRAL_SW, Tref=0.0, seed=0, solver=F.SOLVER_PLAIN_FP64):
+    d = F.MapDesc()
+    d.width, d.height, d.spp, d.spp_offset = width, height, spp, 0
+    d.sensor = F.SENSOR_CAMERA
+    for a in range(3):
+        d.pos[a], d.tgt[a], d.up[a] = pos[a], tgt[a], up[a]
+    d.fov_y = fov
+    d.spectral_type = spectral
+    d.wlen_range[0], d.wlen_range[1] = wlen
+    d.ref_temperature = Tref
+    d.seed = seed
+    d.solver = solver
+    return d
+
+
+def rectangle(width, height, spp, pos, normal=(0, 0, 1), up=(0, 1, 0), size=(1000.0, 1000.0),
+              wlen=(550.0, 550.0), spectral=F.SPECTRAL_SW, Tref=0.0, seed=0,
+              solver=F.SOLVER_PLAIN_FP64):
+    d = F.MapDesc()
+    d.width, d.height, d.spp, d.spp_offset = width, height, spp, 0
+    d.sensor = F.SENSOR_RECTANGLE
+    for a in range(3):
+        d.pos[a], d.tgt[a], d.up[a] = pos[a], pos[a] + normal[a], up[a]
+    d.rect_size[0], d.rect_size[1] = size
+    d.spectral_type = spectral
+    d.wlen_range[0], d.wlen_range[1] = wlen
+    d.ref_temperature = Tref
+    d.seed = seed
+    d.solver = solver
+    return d
+
+
+def cells_along(lwc, lower, upper, org, direction, t0, t1):
+    """Exact crossing of a regular grid: [(ta, tb, (ix, iy, iz))] for the
+    cells the segment [t0, t1] of the ray passes through with positive
+    length. Brute force over every cell plane, independent of any octree."""
+    nz, ny, nx = lwc.shape
+    n = (nx, ny, nz)
+    org = np.asarray(org, np.float64)
+    direction = np.asarray(direction, np.float64)
+    ts = [t0, t1]
+    for a in range(3):
+        if direction[a] == 0:
+            continue
+        planes = lower[a] + (upper[a] - lower[a]) * np.arange(n[a] + 1) / n[a]
+        t = (planes - org[a]) / direction[a]
+        ts.extend(t[(t > t0) & (t < t1)].tolist())
+    ts = np.unique(np.asarray(ts))
+    out = []
+    for ta, tb in zip(ts[:-1], ts[1:]):
+        if tb <= ta:
+            continue
+        p = org + 0.5 * (ta + tb) * direction
+        idx = []
+        inside = True
+        for a in range(3):
+            f = math.floor((p[a] - lower[a]) / ((upper[a] - lower[a]) / n[a]))
+            if f < 0 or f >= n[a]:
+                inside = False
+            idx.append(f)
+        if inside:
+            out.append((ta, tb, tuple(idx)))
+    return out

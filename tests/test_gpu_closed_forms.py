@@ -1,0 +1,141 @@
+"""Both HIP solvers on scenes with closed-form answers (the scenes of
+test_oracle_walks.py, seen through the map interface).
+
+Tolerances: deterministic cases 1e-9 relative for the FP64 solver and 2e-5
+for the FP32 one (FP32 geometry and Planck; Tmin stays FP64); the narrow
+camera's film jitter (fov 1e-4 deg) moves the optical path by < 1e-6
+relative. Estimators: 5 standard errors per pixel, 4 on the image mean.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from htrdr_kmin_amd import _ffi as F
+from htrdr_kmin_amd import atmosphere as A
+from scenes import Scene, camera, cells_along, planck, rectangle, sun_dir, sun_solid_angle, SUN_HALF_ANGLE
+
+pytestmark = pytest.mark.gpu
+
+SOLVERS = [F.SOLVER_PLAIN_FP64, F.SOLVER_WAVEFRONT_FP32]
+DET_RTOL = {F.SOLVER_PLAIN_FP64: 1e-9, F.SOLVER_WAVEFRONT_FP32: 2e-5}
+L_SUN_550 = planck(550e-9, 5778.0)
+LW_WLEN = 10000.0
+LW_BANDS = ((9500.0, 10500.0),)
+
+
+def cone_mean_cos(elevation_deg):
+    return math.sin(math.radians(elevation_deg)) * (1 + math.cos(SUN_HALF_ANGLE)) / 2
+
+
+def render(scene, desc):
+    with A.Context(scene.ptr) as ctx:
+        return ctx.draw_map(desc)
+
+
+def check_estimates(E, SE, expect):
+    assert np.all(np.abs(E - expect) <= 5 * SE + 1e-12 * abs(expect))
+    assert abs(E.mean() - expect) <= 4 * math.sqrt((SE ** 2).sum()) / E.size + 1e-12 * abs(expect)
+
+
+def sun_camera(org, s, n, spp, solver, **kw):
+    return camera(n, n, spp, pos=tuple(org), tgt=tuple(np.asarray(org) + s), up=(0, 0, 1),
+                  fov=1e-4, solver=solver, **kw)
+
+
+@pytest.mark.parametrize("solver", SOLVERS)
+def test_vacuum_ground(solver):
+    sc = Scene(ground=("plane", 0.0), albedo=0.3, sun=(0.0, 60.0))
+    p = render(sc, camera(16, 12, 32, pos=(0, 0, 3000), tgt=(0, 1, 2999), fov=40.0, solver=solver))
+    expect = L_SUN_550 * sun_solid_angle() * 0.3 / math.pi * cone_mean_cos(60.0) / 1e9
+    assert np.all(p["time_n"] == 32)
+    check_estimates(p["radiance_E"], p["radiance_SE"], expect)
+
+
+@pytest.mark.parametrize("solver", SOLVERS)
+def test_absorbing_slab_direct_sun(solver):
+    zmid = (np.arange(50) + 0.5) * 400.0
+    ka = 3e-5 * np.exp(-zmid / 8000.0)
+    sc = Scene(nlayers=50, dz=400.0, ka=ka[None, :], sun=(30.0, 50.0))
+    s = sun_dir(30.0, 50.0)
+    p = render(sc, sun_camera((100.0, -50.0, 0.0), s, 4, 8, solver))
+    expect = L_SUN_550 * math.exp(-ka.sum() * 400.0 / s[2]) / 1e9
+    assert np.allclose(p["radiance_E"], expect, rtol=max(DET_RTOL[solver], 2e-6), atol=0)
+
+
+@pytest.mark.parametrize("solver", SOLVERS)
+def test_absorbing_cloud_transmissivity(solver):
+    rng = np.random.default_rng(1)
+    lwc = np.exp(rng.normal(-1.0, 0.8, (4, 8, 8))).astype(np.float32)
+    lwc[rng.random(lwc.shape) < 0.3] = 0.0
+    sc = Scene(lwc=lwc, lower=(0, 0, 1000), upper=(800, 800, 1400), cext=0.02, ssa=0.0,
+               sun=(20.0, 55.0), optical_thickness=1.0, leaf_block=1)
+    s = sun_dir(20.0, 55.0)
+    org = np.array([-400.0, 100.0, 0.0])
+    tau = sum((tb - ta) * 0.02 * float(lwc[iz, iy, ix])
+              for ta, tb, (ix, iy, iz) in cells_along(lwc, sc.lower, sc.upper, org, s, 0.0, 1e5))
+    p = render(sc, sun_camera(org, s, 4, 1024, solver))
+    expect = L_SUN_550 * math.exp(-tau) / 1e9
+    check_estimates(p["radiance_E"], p["radiance_SE"], expect)
+
+
+@pytest.mark.parametrize("solver", SOLVERS)
+def test_lw_isothermal_enclosure(solver):
+    T = 280.0
+    sc = Scene(bands=LW_BANDS, nlayers=50, dz=400.0, ka=np.full((1, 50), 2e-5), T=np.full(50, T),
+               ground=("plane", 0.0), albedo=0.0, ground_T=T)
+    d = camera(8, 8, 16, pos=(0, 0, 5000), tgt=(0.2, -0.1, 4999), fov=50.0,
+               wlen=(LW_WLEN, LW_WLEN), spectral=F.SPECTRAL_LW, Tref=T, solver=solver)
+    p = render(sc, d)
+    assert np.allclose(p["radiance_E"], planck(LW_WLEN * 1e-9, T) / 1e9, rtol=DET_RTOL[solver], atol=0)
+    # brightness temperature of a black body at T is T (..._draw_map.c:425-435)
+    assert np.allclose(p["temperature_E"], T, atol=1e-3)
+
+
+@pytest.mark.parametrize("solver", SOLVERS)
+def test_lw_thick_scattering_enclosure(solver):
+    T = 265.0
+    rng = np.random.default_rng(3)
+    lwc = np.exp(rng.normal(-1.0, 0.8, (4, 8, 8))).astype(np.float32)
+    lwc[rng.random(lwc.shape) < 0.3] = 0.0
+    sc = Scene(bands=LW_BANDS, nlayers=50, dz=400.0, ka=np.full((1, 50), 1e-2), T=np.full(50, T),
+               lwc=lwc, lower=(0, 0, 1000), upper=(800, 800, 1400), cext=0.3, ssa=0.6, g=0.8,
+               ground=("plane", 0.0), albedo=0.0, ground_T=T)
+    d = camera(8, 8, 16, pos=(300, 300, 1200), tgt=(300.1, 300.3, 1201), fov=70.0,
+               wlen=(LW_WLEN, LW_WLEN), spectral=F.SPECTRAL_LW, Tref=T, solver=solver)
+    p = render(sc, d)
+    assert np.allclose(p["radiance_E"], planck(LW_WLEN * 1e-9, T) / 1e9, rtol=DET_RTOL[solver], atol=0)
+
+
+@pytest.mark.parametrize("solver", SOLVERS)
+def test_lw_absorbing_slab_upward(solver):
+    T = 250.0
+    ka = np.full((1, 50), 1.5e-5)
+    sc = Scene(bands=LW_BANDS, nlayers=50, dz=400.0, ka=ka, T=np.full(50, T))
+    d = np.array([0.3, 0.0, 1.0])
+    d /= np.linalg.norm(d)
+    desc = camera(4, 4, 2048, pos=(0, 0, 1.0), tgt=tuple(np.array([0, 0, 1.0]) + d), fov=1e-4,
+                  wlen=(LW_WLEN, LW_WLEN), spectral=F.SPECTRAL_LW, Tref=T, solver=solver)
+    p = render(sc, desc)
+    B = planck(LW_WLEN * 1e-9, T)
+    tau = ka.sum() * 400.0 - 1.5e-5
+    check_estimates(p["radiance_E"], p["radiance_SE"], B * (1 - math.exp(-tau / d[2])) / 1e9)
+
+
+@pytest.mark.parametrize("solver", SOLVERS)
+def test_flux_sw_vacuum(solver):
+    sc = Scene(sun=(45.0, 35.0))
+    p = render(sc, rectangle(8, 8, 64, pos=(0, 0, 2000.0), size=(1000.0, 1000.0), solver=solver))
+    E, SE = A.estimate(p["flux_sum"], p["flux_sum2"], p["flux_n"])
+    assert np.all(p["flux_n"] == 64)
+    check_estimates(E, SE, sun_solid_angle() * L_SUN_550 * cone_mean_cos(35.0) / 1e9)
+
+
+@pytest.mark.parametrize("solver", SOLVERS)
+def test_flux_lw_thick_isothermal(solver):
+    T = 275.0
+    sc = Scene(bands=LW_BANDS, nlayers=50, dz=400.0, ka=np.full((1, 50), 1e-2), T=np.full(50, T))
+    p = render(sc, rectangle(8, 8, 16, pos=(0, 0, 2000.0), wlen=(LW_WLEN, LW_WLEN),
+                             spectral=F.SPECTRAL_LW, Tref=T, solver=solver))
+    E, SE = A.estimate(p["flux_sum"], p["flux_sum2"], p["flux_n"])
+    assert np.allclose(E, math.pi * planck(LW_WLEN * 1e-9, T) / 1e9, rtol=DET_RTOL[solver], atol=0)

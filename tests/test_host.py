@@ -1,0 +1,200 @@
+"""Host-side parts of libhtgx.so that need no GPU: the C-ABI exports, pixel
+finalisation (src/atmosphere/htrdr_atmosphere_draw_map.c:403-435), the
+htrdr-image writer (..._draw_map.c:477-587, doc/htrdr-image.5) and the
+synthetic scene generator."""
+import ctypes as C
+import glob
+import io
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle_ffi as O
+from htrdr_kmin_amd import _ffi as F
+from htrdr_kmin_amd import atmosphere as A
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"(?<!struct )\b(htgx_[a-z0-9_]+)\s*\(", text))
+    return names
+
+
+def test_library_exports_every_declared_function():
+    names = declared_functions()
+    assert len(names) >= 19
+    lib = C.CDLL(F.LIB_PATH)
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing
+    # and the ctypes mirror declares each of them
+    assert names <= set(F.SIGNATURES)
+
+
+def test_load_fails_loudly_without_library(tmp_path, monkeypatch):
+    monkeypatch.setattr(F, "LIB_PATH", str(tmp_path / "missing.so"))
+    monkeypatch.setattr(F, "_lib", None)
+    with pytest.raises(RuntimeError, match="no CPU fallback"):
+        F.load()
+
+
+def test_bad_arguments_rejected_on_host():
+    lib = F.load()
+    p = F.SynthParams()
+    assert lib.htgx_synth_params_preset(99, C.byref(p)) == F.BAD_ARG
+    d = F.MapDesc()
+    assert lib.htgx_synth_map_preset(-1, C.byref(d)) == F.BAD_ARG
+    acc = np.zeros(4, A.ACCUM_DTYPE)
+    out = np.zeros(4, A.XWAVE_DTYPE)
+    d = A.map_preset(0, width=2, height=2)
+    assert lib.htgx_finalize_pixels(None, acc.ctypes.data_as(C.c_void_p),
+                                    out.ctypes.data_as(C.c_void_p), 128) == F.BAD_ARG
+    # pitch smaller than a row
+    assert lib.htgx_finalize_pixels(C.byref(d), acc.ctypes.data_as(C.c_void_p),
+                                    out.ctypes.data_as(C.c_void_p), 64) == F.BAD_ARG
+
+
+def random_accum(n, seed, spp=16):
+    rng = np.random.default_rng(seed)
+    acc = np.zeros(n, A.ACCUM_DTYPE)
+    w = rng.lognormal(-20, 1.0, (n, spp))
+    acc["sum"] = w.sum(1)
+    acc["sum_c"] = rng.normal(0, 1e-25, n)
+    acc["sum2"] = (w * w).sum(1)
+    acc["sum2_c"] = rng.normal(0, 1e-50, n)
+    acc["tsum"] = rng.uniform(10, 100, n)
+    acc["tsum2"] = acc["tsum"] ** 2 / spp * 1.1
+    acc["n"] = spp
+    acc["n"][0] = 0  # an empty pixel
+    return acc
+
+
+def test_finalize_xwave_sw():
+    d = A.map_preset(0, width=4, height=3)
+    acc = random_accum(12, 1)
+    px = A.finalize(d, acc)
+    for i, a in enumerate(acc):
+        E, SE = C.c_double(), C.c_double()
+        O.lib().oracle_accum_estimate(acc[i:i + 1].ctypes.data_as(C.c_void_p), C.byref(E),
+                                      C.byref(SE))
+        p = px.reshape(-1)[i]
+        assert p["radiance_E"] == E.value and p["radiance_SE"] == SE.value
+        n = int(a["n"])
+        S = a["sum"] + a["sum_c"]
+        if n:
+            assert math.isclose(p["radiance_E"], S / n, rel_tol=1e-15)
+        else:
+            assert p["radiance_E"] == 0 and p["radiance_SE"] == 0
+        assert p["temperature_E"] == 0 and p["temperature_SE"] == 0
+        assert p["time_sum"] == a["tsum"] and p["time_n"] == n
+
+
+def test_finalize_xwave_lw_brightness_temperature():
+    """LW: T(E) and SE = T(E + SE) - T(E - SE), ..._draw_map.c:425-435."""
+    d = A.map_preset(2, width=2, height=2)
+    d.sensor = F.SENSOR_CAMERA
+    acc = random_accum(4, 2)
+    acc["n"][0] = 16
+    # radiances of a few hundred kelvin over 4-20 um, in W/m^2/sr/m
+    scale = 8.0 / (acc["sum"] / acc["n"]).mean()
+    for k in ("sum", "sum_c"):
+        acc[k] *= scale
+    for k in ("sum2", "sum2_c"):
+        acc[k] *= scale * scale
+    px = A.finalize(d, acc).reshape(-1)
+    l0, l1 = d.wlen_range[0] * 1e-9, d.wlen_range[1] * 1e-9
+    for p in px:
+        T = O.lib().oracle_radiance_temperature(l0, l1, p["radiance_E"])
+        Tp = O.lib().oracle_radiance_temperature(l0, l1, p["radiance_E"] + p["radiance_SE"])
+        Tm = O.lib().oracle_radiance_temperature(l0, l1, p["radiance_E"] - p["radiance_SE"])
+        assert 150 < T < 400
+        assert abs(p["temperature_E"] - T) < 1e-9
+        assert abs(p["temperature_SE"] - (Tp - Tm)) < 1e-9
+
+
+def test_finalize_flux():
+    d = A.map_preset(2, width=3, height=2)
+    acc = random_accum(6, 3)
+    px = A.finalize(d, acc).reshape(-1)
+    assert px.dtype == A.FLUX_DTYPE
+    assert np.array_equal(px["flux_sum"], acc["sum"] + acc["sum_c"])
+    assert np.array_equal(px["flux_sum2"], acc["sum2"] + acc["sum2_c"])
+    assert np.array_equal(px["flux_n"], acc["n"])
+
+
+def test_dump_buffer_format():
+    """htrdr-image: "W H", then one line of 8 values per pixel, a blank line
+    after each row; xwave columns "%g %g %f %f 0 0" + time "%g %g"."""
+    d = A.map_preset(0, width=2, height=2)
+    px = np.zeros((2, 2), A.XWAVE_DTYPE)
+    px["radiance_E"] = [[1.5, 2.25], [0.125, 3.0]]
+    px["radiance_SE"] = 0.5
+    px["time_sum"] = 8.0
+    px["time_sum2"] = 32.0  # E = 8/4 = 2, SE = sqrt((32/4 - 4)/4) = 1
+    px["time_n"] = [[4, 4], [0, 4]]
+    s = io.StringIO()
+    tacc, _ = A.dump_buffer(px, d, s)
+    lines = s.getvalue().split("\n")
+    assert lines[0] == "2 2"
+    assert lines[1] == "0 0 1.500000 0.500000 0 0 2 1 "
+    assert lines[2] == "0 0 2.250000 0.500000 0 0 2 1 "
+    assert lines[3] == ""
+    assert lines[4] == "0 0 0.125000 0.500000 0 0 0 0 "
+    assert lines[5] == "0 0 3.000000 0.500000 0 0 2 1 "
+    assert lines[6] == "" and lines[7] == ""
+    assert tacc == [24.0, 96.0, 12]
+
+
+def test_dump_buffer_flux():
+    d = A.map_preset(2, width=1, height=1)
+    px = np.zeros((1, 1), A.FLUX_DTYPE)
+    px["flux_sum"], px["flux_sum2"], px["flux_n"] = 10.0, 29.0, 4  # 2.5, sqrt(1/4)
+    s = io.StringIO()
+    _, facc = A.dump_buffer(px, d, s)
+    assert s.getvalue().split("\n")[1] == "2.5 0.5 0 0 0 0 0 0 "
+    assert facc == [10.0, 29.0, 4]
+
+
+def test_synth_cloud_statistics():
+    """The generator's law: 50 % cover, log-normal LWC (mu -1.6, sigma 0.8),
+    exponential gas profile ka0 exp(-z/H) at layer middles."""
+    s = A.Synth(0, seed=0)
+    lwc = s.lwc()
+    assert lwc.shape == (32, 64, 64)
+    cover = (lwc > 0).mean()
+    assert abs(cover - 0.5) < 0.01
+    ln = np.log(lwc[lwc > 0].astype(np.float64))
+    assert abs(ln.mean() + 1.6) < 0.03 and abs(ln.std() - 0.8) < 0.03
+    g = s.scene.gas
+    ka = np.ctypeslib.as_array(g.ka, shape=(g.nlayers,))
+    zmid = g.z_bottom + (np.arange(g.nlayers) + 0.5) * g.layer_thickness
+    assert np.allclose(ka, 1e-5 * np.exp(-zmid / 8000.0), rtol=1e-12)
+    # deterministic in the seed
+    assert np.array_equal(A.Synth(0, seed=0).lwc(), lwc)
+    assert not np.array_equal(A.Synth(0, seed=1).lwc(), lwc)
+
+
+def test_synth_presets():
+    s1 = A.Synth(1, seed=0)
+    c = s1.scene.clouds
+    assert (c.def_[0], c.def_[1], c.def_[2]) == (256, 256, 64)
+    assert (c.upper[0] - c.lower[0], c.upper[2] - c.lower[2]) == (25600.0, 3200.0)
+    s2 = A.Synth(2, seed=0)
+    sp = s2.scene.spectral
+    assert sp.nbands == 8
+    nq = np.ctypeslib.as_array(sp.nquads, shape=(8,))
+    assert np.all(nq == 16)
+    w = np.ctypeslib.as_array(sp.quad_weights, shape=(128,)).reshape(8, 16)
+    assert np.allclose(w.sum(1), 1.0, atol=1e-12)
+    lo = np.ctypeslib.as_array(sp.band_lo, shape=(8,))
+    hi = np.ctypeslib.as_array(sp.band_hi, shape=(8,))
+    assert lo[0] == 4000.0 and hi[-1] == 20000.0 and np.all(lo[1:] == hi[:-1])
+    d = A.map_preset(1)
+    assert (d.width, d.height, d.spp) == (1024, 1024, 256)
