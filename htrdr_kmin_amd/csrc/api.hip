@@ -269,7 +269,8 @@ check_scene(const htgx_scene_desc* s)
     if(!sp.nquads[b] || !(sp.band_hi[b] >= sp.band_lo[b])) return HTGX_BAD_ARG;
     if(b && sp.band_lo[b] < sp.band_hi[b - 1]) return HTGX_BAD_ARG;
   }
-  if(!s->gas.nlayers || !(s->gas.layer_thickness > 0) || !s->gas.ka || !s->gas.ks)
+  /* a scene may have no gas layers (nlayers = 0: tests, cloud-only media) */
+  if(s->gas.nlayers && (!(s->gas.layer_thickness > 0) || !s->gas.ka || !s->gas.ks))
     return HTGX_BAD_ARG;
   const htgx_cloud_desc& c = s->clouds;
   const bool clouds = c.def[0] || c.def[1] || c.def[2];
@@ -278,8 +279,10 @@ check_scene(const htgx_scene_desc* s)
   if(s->ground.type == HTGX_GROUND_PLANE) {
     if(!s->ground.nmtls || s->ground.plane_mtl >= s->ground.nmtls || !s->ground.mtl_albedo)
       return HTGX_BAD_ARG;
+  } else if(s->ground.type == HTGX_GROUND_MESH) {
+    return HTGX_BAD_OP; /* triangle-mesh grounds are not built */
   } else if(s->ground.type != HTGX_GROUND_NONE) {
-    return HTGX_BAD_ARG; /* meshes: see htgx_ctx_create */
+    return HTGX_BAD_ARG;
   }
   if(!(s->optical_thickness >= 0)) return HTGX_BAD_ARG;
   return HTGX_OK;
@@ -339,11 +342,11 @@ htgx_ctx_init(htgx_ctx* ctx, const htgx_scene_desc& sc)
   const uint32_t nl = sc.gas.nlayers;
   S.nlayers = nl;
   S.zb = sc.gas.z_bottom;
-  S.dzl = sc.gas.layer_thickness;
-  S.inv_dzl = 1.0 / sc.gas.layer_thickness;
+  S.dzl = nl ? sc.gas.layer_thickness : 1.0;
+  S.inv_dzl = 1.0 / S.dzl;
   if((res = upload(ctx, sc.gas.ka, (size_t)nq * nl, &S.gas_ka))) return res;
   if((res = upload(ctx, sc.gas.ks, (size_t)nb * nl, &S.gas_ks))) return res;
-  if(sc.gas.temperature) {
+  if(sc.gas.temperature && nl) {
     if((res = upload(ctx, sc.gas.temperature, nl, &S.gas_T))) return res;
     ctx->has_temperature = true;
   }
@@ -453,6 +456,11 @@ htgx_map_setup(htgx_ctx* ctx, const htgx_map_desc* d, DevMap* M)
   if(!d->width || !d->height || !d->spp) return HTGX_BAD_ARG;
   if(d->width > 16384 || d->height > 16384) return HTGX_BAD_ARG; /* htrdr_args.c:61 */
   if(d->sensor != HTGX_SENSOR_CAMERA && d->sensor != HTGX_SENSOR_RECTANGLE) return HTGX_BAD_ARG;
+  if(d->spectral_type == HTGX_SPECTRAL_SW_CIE_XYZ) {
+    /* CIE XYZ flux maps are invalid (src/atmosphere/htrdr_atmosphere.c:189-194);
+     * CIE XYZ images (draw_pixel_image) are not built */
+    return d->sensor == HTGX_SENSOR_RECTANGLE ? HTGX_BAD_ARG : HTGX_BAD_OP;
+  }
   if(d->spectral_type != HTGX_SPECTRAL_SW && d->spectral_type != HTGX_SPECTRAL_LW)
     return HTGX_BAD_ARG;
   if(d->spectral_type == HTGX_SPECTRAL_LW && !ctx->has_temperature) return HTGX_BAD_ARG;

@@ -1064,14 +1064,15 @@ static int upload_f(htgx_wavefront_state* st, const std::vector<T>& v, const T**
   return HTGX_OK;
 }
 
+/* FP32 copy of a device FP64 table, uploaded as a new device table */
 template<class T>
-static std::vector<float> dev_to_float(const T* dptr, size_t n)
+static int upload_as_float(htgx_wavefront_state* st, const T* dptr, size_t n, const float** out)
 {
   std::vector<T> h(n);
   std::vector<float> f(n);
-  if(n && hipMemcpy(h.data(), dptr, n * sizeof(T), hipMemcpyDeviceToHost) != hipSuccess) return {};
+  if(n) HIP_OK(hipMemcpy(h.data(), dptr, n * sizeof(T), hipMemcpyDeviceToHost));
   for(size_t i = 0; i < n; ++i) f[i] = (float)h[i];
-  return f;
+  return upload_f(st, f, out);
 }
 
 static int
@@ -1095,9 +1096,9 @@ wavefront_prepare(htgx_ctx* ctx)
   std::vector<uint32_t> qoff(D.nbands + 1);
   HIP_OK(hipMemcpy(qoff.data(), D.quad_off, qoff.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
   const uint32_t nq = qoff[D.nbands];
-  if((res = upload_f(st, dev_to_float(D.gas_ka, (size_t)nq * D.nlayers), &S.gas_ka))) return res;
-  if((res = upload_f(st, dev_to_float(D.gas_ks, (size_t)D.nbands * D.nlayers), &S.gas_ks))) return res;
-  if(D.gas_T && (res = upload_f(st, dev_to_float(D.gas_T, D.nlayers), &S.gas_T))) return res;
+  if((res = upload_as_float(st, D.gas_ka, (size_t)nq * D.nlayers, &S.gas_ka))) return res;
+  if((res = upload_as_float(st, D.gas_ks, (size_t)D.nbands * D.nlayers, &S.gas_ks))) return res;
+  if(D.gas_T && (res = upload_as_float(st, D.gas_T, D.nlayers, &S.gas_T))) return res;
   S.has_clouds = D.has_clouds;
   if(D.has_clouds) {
     S.B = D.B;
@@ -1132,8 +1133,8 @@ wavefront_prepare(htgx_ctx* ctx)
   S.plane_mtl = D.plane_mtl;
   if(D.ground_type == HTGX_GROUND_PLANE) {
     const uint32_t nm = D.plane_mtl + 1;
-    if((res = upload_f(st, dev_to_float(D.mtl_albedo, nm), &S.mtl_albedo))) return res;
-    if(D.mtl_T && (res = upload_f(st, dev_to_float(D.mtl_T, nm), &S.mtl_T))) return res;
+    if((res = upload_as_float(st, D.mtl_albedo, nm, &S.mtl_albedo))) return res;
+    if(D.mtl_T && (res = upload_as_float(st, D.mtl_T, nm, &S.mtl_T))) return res;
   }
   for(int a = 0; a < 3; ++a) S.sun_dir[a] = (float)D.sun_dir[a];
   S.sun_cos_half = (float)D.sun_cos_half;

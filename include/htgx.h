@@ -54,7 +54,8 @@ enum htgx_res {
 enum htgx_spectral_type {
   HTGX_SPECTRAL_LW = 0,
   HTGX_SPECTRAL_SW = 1,
-  HTGX_SPECTRAL_SW_CIE_XYZ = 2 /* Not supported: rejected with HTGX_BAD_ARG */
+  HTGX_SPECTRAL_SW_CIE_XYZ = 2 /* Not built: images return HTGX_BAD_OP, flux maps
+                                  HTGX_BAD_ARG as in the reference */
 };
 
 /* Radiance components, enum atmosphere_radiance_cpnt_flag
