@@ -116,7 +116,8 @@ class SynthParams(C.Structure):
                 ("ground_albedo", C.c_double), ("ground_temperature", C.c_double),
                 ("sun_azimuth", C.c_double), ("sun_elevation", C.c_double),
                 ("optical_thickness", C.c_double), ("leaf_block", C.c_uint32),
-                ("seed", C.c_uint64)]
+                ("seed", C.c_uint64), ("hf_res", C.c_uint32), ("hf_octaves", C.c_uint32),
+                ("hf_relief", C.c_double), ("hf_hurst", C.c_double)]
 
 
 # Entry points: name -> (restype, argtypes)

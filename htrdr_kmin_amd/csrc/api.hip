@@ -5,6 +5,7 @@
 #include "htgx.h"
 #include "htgx_device.h"
 #include "htgx_internal.h"
+#include "bvh.h"
 #include "svo.h"
 
 #include <hip/hip_runtime.h>
@@ -280,7 +281,13 @@ check_scene(const htgx_scene_desc* s)
     if(!s->ground.nmtls || s->ground.plane_mtl >= s->ground.nmtls || !s->ground.mtl_albedo)
       return HTGX_BAD_ARG;
   } else if(s->ground.type == HTGX_GROUND_MESH) {
-    return HTGX_BAD_OP; /* triangle-mesh grounds are not built */
+    const htgx_ground_desc& g = s->ground;
+    if(!g.nverts || !g.ntris || !g.verts || !g.tris || !g.tri_mtl || !g.nmtls || !g.mtl_albedo)
+      return HTGX_BAD_ARG;
+    for(uint32_t t = 0; t < g.ntris; ++t) {
+      if(g.tri_mtl[t] >= g.nmtls) return HTGX_BAD_ARG;
+      for(int k = 0; k < 3; ++k) if(g.tris[3 * (size_t)t + k] >= g.nverts) return HTGX_BAD_ARG;
+    }
   } else if(s->ground.type != HTGX_GROUND_NONE) {
     return HTGX_BAD_ARG;
   }
@@ -381,12 +388,24 @@ htgx_ctx_init(htgx_ctx* ctx, const htgx_scene_desc& sc)
 
   /* Ground */
   S.ground_type = sc.ground.type;
-  if(sc.ground.type == HTGX_GROUND_PLANE) {
+  if(sc.ground.type == HTGX_GROUND_PLANE || sc.ground.type == HTGX_GROUND_MESH) {
     S.plane_z = sc.ground.plane_z;
     S.plane_mtl = sc.ground.plane_mtl;
     if((res = upload(ctx, sc.ground.mtl_albedo, sc.ground.nmtls, &S.mtl_albedo))) return res;
     if(sc.ground.mtl_temperature)
       if((res = upload(ctx, sc.ground.mtl_temperature, sc.ground.nmtls, &S.mtl_T))) return res;
+    ctx->nmtls = sc.ground.nmtls;
+  }
+  if(sc.ground.type == HTGX_GROUND_MESH) {
+    BvhHost bvh;
+    if((res = bvh_build(sc.ground.verts, sc.ground.nverts, sc.ground.tris, sc.ground.tri_mtl,
+                        sc.ground.ntris, bvh)) != HTGX_OK)
+      return res;
+    S.nbvh = (uint32_t)bvh.nodes.size();
+    if((res = upload(ctx, bvh.nodes.data(), bvh.nodes.size(), &S.bvh))) return res;
+    if((res = upload(ctx, bvh.tris.data(), bvh.tris.size(), &S.tris))) return res;
+    if((res = upload(ctx, bvh.mtl.data(), bvh.mtl.size(), &S.tri_mtl))) return res;
+    ctx->bvh_depth = bvh.depth;
   }
 
   /* Sun: src/atmosphere/htrdr_atmosphere.c:89-111 + ..._sun.c:84-87 */

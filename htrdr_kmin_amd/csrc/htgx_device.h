@@ -22,6 +22,7 @@
 #ifndef HTGX_DEVICE_H
 #define HTGX_DEVICE_H
 
+#include "htgx_mesh.h"
 #include "htgx_rng.h"
 #include "htgx.h"
 
@@ -67,6 +68,11 @@ struct DevScene {
   uint32_t plane_mtl;
   const double* mtl_albedo;
   const double* mtl_T;
+  /* mesh ground: BVH nodes, triangles and their materials in leaf order */
+  const BvhNode* bvh;
+  uint32_t nbvh;
+  const BvhTri* tris;
+  const uint32_t* tri_mtl;
   /* sun */
   double sun_dir[3];
   double sun_cos_half;
@@ -567,28 +573,57 @@ HTGX_D void track(const DevScene& S, Stream& rng, int mode, int prop, uint32_t i
 struct GroundHit {
   bool hit;
   double t;
-  double N[3];
+  double N[3];    /* geometric normal of the hit (unit) */
   uint32_t mtl;
+  int32_t prim;   /* hit primitive: 0 the plane, else the mesh triangle; -1 none */
+  bool edge;      /* the hit lies on an edge or a vertex of its triangle */
 };
 
-/* Plane ground (a flat quad repeated with -R, ..._ground.c:170-217), with the
- * self-hit rules of src/core/htrdr_geometry.c:116-150: hits at or beyond the
- * range bounds and hits on the surface the ray leaves are discarded. */
+/* Closest ground hit in ]r0, r1[, htrdr_atmosphere_ground_trace_ray
+ * (..._ground.c:170-217) with the self-hit rules of
+ * src/core/htrdr_geometry.c:116-150 relative to the previous surface hit
+ * (prev_prim, prev_edge; prev_prim = -1 after a volume event). The plane is
+ * a flat quad repeated to infinity (-R): one primitive, id 0. A mesh is
+ * traced in FP32 as the reference does. With `any', any accepted hit will do
+ * (occlusion queries). */
 HTGX_D GroundHit ground_trace(const DevScene& S, const double o[3], const double d[3],
-                              double r0, double r1, bool from_ground)
+                              double r0, double r1, int32_t prev_prim, bool prev_edge,
+                              bool any = false)
 {
   GroundHit h;
   h.hit = false;
-  h.t = FLT_MAX;
+  h.t = FLT_MAX; /* distance of a missed ray */
   h.N[0] = h.N[1] = h.N[2] = 0;
   h.mtl = 0;
-  if(S.ground_type == HTGX_GROUND_PLANE && !from_ground && d[2] != 0) {
+  h.prim = -1;
+  h.edge = false;
+  if(S.ground_type == HTGX_GROUND_PLANE && prev_prim != 0 && d[2] != 0) {
     const double t = (S.plane_z - o[2]) / d[2];
     if(t > r0 && t < r1) {
       h.hit = true;
       h.t = t;
       h.N[2] = 1;
       h.mtl = S.plane_mtl;
+      h.prim = 0;
+    }
+  } else if(S.ground_type == HTGX_GROUND_MESH) {
+    const float of[3] = {(float)o[0], (float)o[1], (float)o[2]};
+    const float df[3] = {(float)d[0], (float)d[1], (float)d[2]};
+    const float r1f = r1 > (double)FLT_MAX ? INFINITY : (float)r1;
+    const MeshHit m = mesh_trace(S.bvh, S.nbvh, S.tris, of, df, (float)r0, r1f, prev_prim,
+                                 prev_edge, any);
+    if(m.tri >= 0) {
+      const BvhTri& T = S.tris[m.tri];
+      const float n0 = T.e1[1] * T.e2[2] - T.e1[2] * T.e2[1];
+      const float n1 = T.e1[2] * T.e2[0] - T.e1[0] * T.e2[2];
+      const float n2 = T.e1[0] * T.e2[1] - T.e1[1] * T.e2[0];
+      h.hit = true;
+      h.t = (double)m.t;
+      h.N[0] = n0; h.N[1] = n1; h.N[2] = n2;
+      normalize3(h.N);
+      h.mtl = S.tri_mtl[m.tri];
+      h.prim = m.tri;
+      h.edge = mesh_on_edge(m.u, m.v);
     }
   }
   return h;
@@ -673,7 +708,9 @@ struct Path {
   bool rejected;
   /* walk */
   int stage, mask;
-  bool from_ground, surface, rayleigh, gh_hit;
+  bool surface, rayleigh, gh_hit;
+  bool prev_edge, gh_edge;
+  int32_t prev_prim, gh_prim; /* previous surface hit (-1: none) and current hit */
   uint32_t gh_mtl;
   double pos[3], dir[3], pos_next[3], dir_next[3], sun_dir[3], N[3];
   double gh_t, dst, ksi, w, Tr_abs, R, sun_pdf, refl, albedo, gas_ratio;
@@ -681,10 +718,12 @@ struct Path {
 
 HTGX_D void ground_setup(const DevScene& S, Path& P)
 {
-  const GroundHit gh = ground_trace(S, P.pos, P.dir, 0.0, DBL_MAX, P.from_ground);
+  const GroundHit gh = ground_trace(S, P.pos, P.dir, 0.0, DBL_MAX, P.prev_prim, P.prev_edge);
   P.gh_hit = gh.hit;
   P.gh_t = gh.t;
   P.gh_mtl = gh.mtl;
+  P.gh_prim = gh.prim;
+  P.gh_edge = gh.edge;
   P.N[0] = gh.N[0]; P.N[1] = gh.N[1]; P.N[2] = gh.N[2];
 }
 
@@ -702,9 +741,10 @@ HTGX_D void sw_begin(const DevScene& S, Path& P, int mask, const double pos[3], 
   P.dir[0] = dir[0]; P.dir[1] = dir[1]; P.dir[2] = dir[2];
   P.ksi = 1;
   P.w = 0;
-  P.from_ground = false;
+  P.prev_prim = -1;
+  P.prev_edge = false;
   if((mask & HTGX_RADIANCE_DIRECT) && in_solar_cone(S, P.dir)) {
-    const GroundHit h = ground_trace(S, P.pos, P.dir, 0.0, FLT_MAX, false);
+    const GroundHit h = ground_trace(S, P.pos, P.dir, 0.0, FLT_MAX, -1, false, true);
     if(!h.hit) { P.stage = ST_DIRECT; return; }
   }
   sw_after_direct(S, P);
@@ -715,7 +755,8 @@ HTGX_D void lw_begin(const DevScene& S, Path& P, const double pos[3], const doub
   P.pos[0] = pos[0]; P.pos[1] = pos[1]; P.pos[2] = pos[2];
   P.dir[0] = dir[0]; P.dir[1] = dir[1]; P.dir[2] = dir[2];
   P.w = 0;
-  P.from_ground = false;
+  P.prev_prim = -1;
+  P.prev_edge = false;
   ground_setup(S, P);
   P.stage = ST_LW;
 }
@@ -766,7 +807,9 @@ HTGX_D void stage_consume(const DevScene& S, Path& P, Stream& rng, const TrackOu
       P.pos_next[0] = P.pos[0] + P.dir[0] * P.dst;
       P.pos_next[1] = P.pos[1] + P.dir[1] * P.dst;
       P.pos_next[2] = P.pos[2] + P.dir[2] * P.dst;
-      P.from_ground = P.surface;
+      /* ..._sw.c:392: the next traces leave this surface hit, if any */
+      P.prev_prim = P.surface ? P.gh_prim : -1;
+      P.prev_edge = P.surface && P.gh_edge;
       if(!P.surface) {
         /* gas vs particle scattering, ..._sw.c:427-438 */
         const double ks_gas = raw_in_seg(S, out.seg, HTGX_Ks, 1, P.iband, P.iquad, P.pos_next, nullptr);
@@ -804,7 +847,8 @@ HTGX_D void stage_consume(const DevScene& S, Path& P, Stream& rng, const TrackOu
         P.R = P.rayleigh ? rayleigh_eval(mu) : hg_eval(P.g, mu);
       }
       if(P.R <= 0) { sw_accumulate(S, P, rng, 0.0); break; }
-      const GroundHit h = ground_trace(S, P.pos_next, P.sun_dir, 0.0, FLT_MAX, P.from_ground);
+      const GroundHit h = ground_trace(S, P.pos_next, P.sun_dir, 0.0, FLT_MAX, P.prev_prim,
+                                       P.prev_edge, true);
       if(h.hit) { sw_accumulate(S, P, rng, 0.0); break; }
       P.stage = ST_SUN;
     } break;
@@ -828,7 +872,8 @@ HTGX_D void stage_consume(const DevScene& S, Path& P, Stream& rng, const TrackOu
       if(out.event == EV_SCA) {
         const double u0 = rng.canonical(), u1 = rng.canonical();
         local_to_world(P.dir, hg_cos_sample(P.g, u0), 2.0 * kPi * u1, P.dir_next);
-        P.from_ground = false;
+        P.prev_prim = -1; /* ..._lw.c:236 */
+        P.prev_edge = false;
       } else {
         if(-(P.dir[0]*P.N[0] + P.dir[1]*P.N[1] + P.dir[2]*P.N[2]) < 0) {
           P.N[0] = -P.N[0]; P.N[1] = -P.N[1]; P.N[2] = -P.N[2];
@@ -842,7 +887,8 @@ HTGX_D void stage_consume(const DevScene& S, Path& P, Stream& rng, const TrackOu
           P.stage = ST_WALK_DONE;
           break;
         }
-        P.from_ground = true;
+        P.prev_prim = P.gh_prim; /* ..._lw.c:272 */
+        P.prev_edge = P.gh_edge;
       }
       P.pos[0] = P.pos_next[0]; P.pos[1] = P.pos_next[1]; P.pos[2] = P.pos_next[2];
       P.dir[0] = P.dir_next[0]; P.dir[1] = P.dir_next[1]; P.dir[2] = P.dir_next[2];
@@ -883,7 +929,7 @@ HTGX_D void real_begin(const DevScene& S, const DevMap& M, Path& P, Stream& rng,
     for(int a = 0; a < 3; ++a)
       P.org[a] = M.pos[a] + M.axis_x[a] * (sx * 2.0 - 1.0) + M.axis_y[a] * (sy * 2.0 - 1.0);
     const double up[3] = {0, 0, 1};
-    const GroundHit h = ground_trace(S, P.org, up, 0.0, DBL_MAX, false);
+    const GroundHit h = ground_trace(S, P.org, up, 0.0, DBL_MAX, -1, false);
     if(h.hit && dot3(up, h.N) >= 0) { P.rejected = true; P.stage = ST_REAL_DONE; return; }
     const double u0 = rng.canonical(), u1 = rng.canonical();
     cos_hemisphere(M.z, u0, u1, P.dir0);

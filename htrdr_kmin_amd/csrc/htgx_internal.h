@@ -21,6 +21,8 @@ struct htgx_ctx {
   bool has_temperature = false;
   uint32_t nlevels = 0;
   uint64_t nnodes = 0, nleaves = 0;
+  uint32_t nmtls = 0;     /* ground materials */
+  uint32_t bvh_depth = 0; /* mesh ground BVH depth */
   size_t svo_bytes = 0;
   double us_per_tick = 0.01;
   /* per-slice partial accumulators of the plain solver */

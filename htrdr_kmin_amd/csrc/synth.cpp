@@ -19,6 +19,8 @@ struct htgx_synth {
   std::vector<float> lwc;
   std::vector<double> cext, ssa, hg_g;
   std::vector<double> mtl_albedo, mtl_temperature;
+  std::vector<float> verts;
+  std::vector<uint32_t> tris, tri_mtl;
 };
 
 namespace {
@@ -83,6 +85,64 @@ void smooth(std::vector<float>& f, const uint32_t def[3], double sigma)
   }
 }
 
+/* Value of the periodic lattice noise of octave `oct' (n x n points) at
+ * (x, y) in [0, 1)^2: smoothstep interpolation of per-point values uniform in
+ * [-1, 1], drawn from the Threefry stream (seed, 0x6866 "hf", oct) */
+double value_noise(const std::vector<double>& lat, uint32_t n, double x, double y)
+{
+  const double fx = x * n, fy = y * n;
+  const uint32_t i0 = (uint32_t)fx % n, j0 = (uint32_t)fy % n;
+  const uint32_t i1 = (i0 + 1) % n, j1 = (j0 + 1) % n;
+  double tx = fx - std::floor(fx), ty = fy - std::floor(fy);
+  tx = tx * tx * (3 - 2 * tx);
+  ty = ty * ty * (3 - 2 * ty);
+  const double a = lat[(size_t)j0 * n + i0], b = lat[(size_t)j0 * n + i1];
+  const double c = lat[(size_t)j1 * n + i0], d = lat[(size_t)j1 * n + i1];
+  return (a * (1 - tx) + b * tx) * (1 - ty) + (c * (1 - tx) + d * tx) * ty;
+}
+
+/* fBm heightfield mesh over [x0, x1] x [y0, y1] */
+void heightfield(const htgx_synth_params* p, double x0, double x1, double y0, double y1,
+                 htgx_synth* s)
+{
+  const uint32_t R = p->hf_res;
+  std::vector<double> h((size_t)R * R, 0.0);
+  for(uint32_t o = 0; o < p->hf_octaves; ++o) {
+    const uint32_t n = 2u << o; /* lattice points per side */
+    std::vector<double> lat((size_t)n * n);
+    htgx::Stream rng;
+    rng.init(p->seed, 0x6866ULL, o);
+    for(double& v : lat) v = 2.0 * rng.canonical() - 1.0;
+    const double amp = std::pow(2.0, -p->hf_hurst * (double)o);
+    #pragma omp parallel for schedule(static)
+    for(int64_t j = 0; j < (int64_t)R; ++j)
+      for(uint32_t i = 0; i < R; ++i)
+        h[(size_t)j * R + i] += amp * value_noise(lat, n, (double)i / R, (double)j / R);
+  }
+  const auto mm = std::minmax_element(h.begin(), h.end());
+  const double lo = *mm.first, span = *mm.second - *mm.first;
+  s->verts.resize(3 * (size_t)R * R);
+  for(uint32_t j = 0; j < R; ++j)
+    for(uint32_t i = 0; i < R; ++i) {
+      float* v = &s->verts[3 * ((size_t)j * R + i)];
+      v[0] = (float)(x0 + (x1 - x0) * i / (R - 1));
+      v[1] = (float)(y0 + (y1 - y0) * j / (R - 1));
+      v[2] = (float)(span > 0 ? (h[(size_t)j * R + i] - lo) / span * p->hf_relief : 0.0);
+    }
+  for(uint32_t j = 0; j + 1 < R; ++j)
+    for(uint32_t i = 0; i + 1 < R; ++i) {
+      const uint32_t v00 = j * R + i, v10 = v00 + 1, v01 = v00 + R, v11 = v01 + 1;
+      const uint32_t quad[2][3] = {{v00, v10, v11}, {v00, v11, v01}};
+      for(const auto& t : quad) {
+        const double zc = (s->verts[3 * t[0] + 2] + s->verts[3 * t[1] + 2] + s->verts[3 * t[2] + 2]) / 3.0;
+        const uint32_t band = p->hf_relief > 0
+          ? std::min(3u, (uint32_t)(4.0 * zc / p->hf_relief)) : 0u;
+        s->tris.insert(s->tris.end(), {t[0], t[1], t[2]});
+        s->tri_mtl.push_back(band);
+      }
+    }
+}
+
 } /* namespace */
 
 extern "C" int
@@ -137,10 +197,19 @@ htgx_synth_params_preset(int config, htgx_synth_params* p)
       p->wlen_lo = 4000.0;
       p->wlen_hi = 20000.0;
       break;
+    case 4:
+      p->cloud_def[0] = 512; p->cloud_def[1] = 512; p->cloud_def[2] = 128;
+      p->cloud_upper[0] = 51200; p->cloud_upper[1] = 51200;
+      p->ground_type = HTGX_GROUND_MESH;
+      p->hf_res = 1024;
+      p->hf_octaves = 8;
+      p->hf_relief = 800.0;
+      p->hf_hurst = 0.7;
+      break;
     default: return HTGX_BAD_ARG;
   }
   p->cloud_lower[2] = 1000.0;
-  p->cloud_upper[2] = 4200.0;
+  p->cloud_upper[2] = config == 4 ? 7400.0 : 4200.0;
   return HTGX_OK;
 }
 
@@ -182,6 +251,19 @@ htgx_synth_map_preset(int config, htgx_map_desc* d)
       d->wlen_range[0] = 4000.0;
       d->wlen_range[1] = 20000.0;
       d->ref_temperature = 290.0;
+      break;
+    case 4:
+      /* 512^2 SW flux map at 2 km over the whole 51.2 km domain */
+      d->width = d->height = 512;
+      d->spp = 512;
+      d->sensor = HTGX_SENSOR_RECTANGLE;
+      d->pos[0] = 25600.0; d->pos[1] = 25600.0; d->pos[2] = 2000.0;
+      d->tgt[0] = 25600.0; d->tgt[1] = 25600.0; d->tgt[2] = 2001.0;
+      d->up[0] = 0; d->up[1] = 1; d->up[2] = 0;
+      d->rect_size[0] = d->rect_size[1] = 51200.0;
+      d->spectral_type = HTGX_SPECTRAL_SW;
+      d->wlen_range[0] = d->wlen_range[1] = 550.0;
+      d->ref_temperature = 5778.0;
       break;
     default: return HTGX_BAD_ARG;
   }
@@ -269,8 +351,19 @@ htgx_synth_create(const htgx_synth_params* p, htgx_synth** out)
     s->ssa.assign(nb, p->ssa);
     s->hg_g.assign(nb, p->hg_g);
   }
-  s->mtl_albedo.push_back(p->ground_albedo);
-  s->mtl_temperature.push_back(p->ground_temperature);
+  if(p->ground_type == HTGX_GROUND_MESH) {
+    if(p->hf_res < 2 || !p->hf_octaves || p->hf_octaves > 20) { delete s; return HTGX_BAD_ARG; }
+    const double x0 = p->cloud_def[0] ? p->cloud_lower[0] : 0.0;
+    const double x1 = p->cloud_def[0] ? p->cloud_upper[0] : 1.0;
+    const double y0 = p->cloud_def[1] ? p->cloud_lower[1] : 0.0;
+    const double y1 = p->cloud_def[1] ? p->cloud_upper[1] : 1.0;
+    heightfield(p, x0, x1, y0, y1, s);
+    s->mtl_albedo = {0.1, 0.25, 0.4, 0.6};
+    s->mtl_temperature.assign(4, p->ground_temperature);
+  } else {
+    s->mtl_albedo.push_back(p->ground_albedo);
+    s->mtl_temperature.push_back(p->ground_temperature);
+  }
 
   htgx_scene_desc& sc = s->scene;
   sc.spectral.nbands = nb;
@@ -298,7 +391,14 @@ htgx_synth_create(const htgx_synth_params* p, htgx_synth** out)
   sc.ground.type = p->ground_type;
   sc.ground.plane_z = 0.0;
   sc.ground.plane_mtl = 0;
-  sc.ground.nmtls = 1;
+  sc.ground.nmtls = (uint32_t)s->mtl_albedo.size();
+  if(p->ground_type == HTGX_GROUND_MESH) {
+    sc.ground.nverts = (uint32_t)(s->verts.size() / 3);
+    sc.ground.ntris = (uint32_t)(s->tris.size() / 3);
+    sc.ground.verts = s->verts.data();
+    sc.ground.tris = s->tris.data();
+    sc.ground.tri_mtl = s->tri_mtl.data();
+  }
   sc.ground.mtl_albedo = s->mtl_albedo.data();
   sc.ground.mtl_temperature = s->mtl_temperature.data();
   sc.sun_azimuth = p->sun_azimuth;

@@ -41,6 +41,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <new>
 #include <vector>
 
 using namespace htgx;
@@ -77,14 +78,21 @@ struct SceneF {
   uint32_t dims0[3];
   float lower[3], bs[3], inv_bs[3], inv_cs[3];
   const float* lwc;
-  const uint32_t* lmap;
-  const float2* bounds;
+  /* per finest block: x = level-map word (level << 28 | leaf id), y = the
+   * leaf's LWC min (low half) and max (high half) as FP16 multiples of
+   * lwc_scale, rounded outwards */
+  const uint2* leafrec;
+  float lwc_scale;
   const float* cka; /* [nbands] cext*(1-ssa) */
   const float* cks; /* [nbands] cext*ssa */
   const float* hg_g;
   int ground_type;
   float plane_z;
   uint32_t plane_mtl;
+  const BvhNode* bvh;   /* mesh ground (shared with the FP64 solver) */
+  uint32_t nbvh;
+  const BvhTri* tris;
+  const uint32_t* tri_mtl;
   const float* mtl_albedo;
   const float* mtl_T;
   float sun_dir[3];
@@ -130,7 +138,9 @@ struct Lane {
   uint64_t t0;
   /* walk */
   int stage, mask, phase;
-  bool from_ground, surface, rayleigh, gh_hit, pending;
+  bool surface, rayleigh, gh_hit, pending;
+  bool prev_edge, gh_edge;
+  int32_t prev_prim, gh_prim; /* previous surface hit (-1: none) and current hit */
   uint32_t gh_mtl;
   float pos[3], dir[3], dir_next[3];
   float ksi, w, Tr_abs, R, refl, gas_ratio, ev_dst;
@@ -148,8 +158,7 @@ struct Lane {
   int cstate;          /* 0 before grid, 1 in leaf, 2 outside, 3 locate at tit, 4 step out of leaf */
   float tit, tnext;    /* iterator position; grid entry or leaf exit */
   uint32_t lohi[3];    /* leaf block range lo | hi << 16 */
-  uint32_t leaf;
-  float lmn, lmx;
+  float lmn, lmx;       /* LWC bounds of the current leaf */
   int exit_axis;
   int32_t j;
   bool itdone, adv, adv_g, adv_c, adv_end;
@@ -253,20 +262,79 @@ __device__ __forceinline__ float rayleigh_evalf(float mu)
   return 3.f / (16.f * kPif) * (1.f + mu * mu);
 }
 
-/* Plane ground with the self-hit rules of src/core/htrdr_geometry.c:116-150 */
-__device__ __forceinline__ bool ground_planef(const SceneF& S, const float o[3], const float d[3],
-                                              float r1, bool from_ground, float* t)
+/* Ground hit for the FP32 solver: the plane (primitive 0, a quad repeated to
+ * infinity) or the mesh BVH, with the self-hit rules of
+ * src/core/htrdr_geometry.c:116-150 relative to the previous surface hit */
+struct GroundF {
+  bool hit, edge;
+  int32_t prim;
+  uint32_t mtl;
+  float t;
+};
+
+__device__ __forceinline__ GroundF ground_tracef(const SceneF& S, const float o[3], const float d[3],
+                                                 float r1, int32_t prev_prim, bool prev_edge,
+                                                 bool any)
 {
-  if(S.ground_type != HTGX_GROUND_PLANE || from_ground || d[2] == 0.f) return false;
-  const float tt = (S.plane_z - o[2]) / d[2];
-  if(!(tt > 0.f && tt < r1)) return false;
-  *t = tt;
-  return true;
+  GroundF g;
+  g.hit = g.edge = false;
+  g.prim = -1;
+  g.mtl = 0;
+  g.t = FLT_MAX;
+  if(S.ground_type == HTGX_GROUND_PLANE) {
+    if(prev_prim != 0 && d[2] != 0.f) {
+      const float tt = (S.plane_z - o[2]) / d[2];
+      if(tt > 0.f && tt < r1) { g.hit = true; g.t = tt; g.prim = 0; g.mtl = S.plane_mtl; }
+    }
+  } else if(S.ground_type == HTGX_GROUND_MESH) {
+    const MeshHit m = mesh_trace(S.bvh, S.nbvh, S.tris, o, d, 0.f, r1, prev_prim, prev_edge, any);
+    if(m.tri >= 0) {
+      g.hit = true;
+      g.t = m.t;
+      g.prim = m.tri;
+      g.mtl = S.tri_mtl[m.tri];
+      g.edge = mesh_on_edge(m.u, m.v);
+    }
+  }
+  return g;
+}
+
+/* Unit geometric normal of a ground primitive: +z for the plane,
+ * (v1 - v0) x (v2 - v0) for a mesh triangle; its side is the sky side */
+__device__ __forceinline__ void ground_normalf(const SceneF& S, int32_t prim, float N[3])
+{
+  if(S.ground_type == HTGX_GROUND_MESH) {
+    const BvhTri& T = S.tris[prim];
+    N[0] = T.e1[1] * T.e2[2] - T.e1[2] * T.e2[1];
+    N[1] = T.e1[2] * T.e2[0] - T.e1[0] * T.e2[2];
+    N[2] = T.e1[0] * T.e2[1] - T.e1[1] * T.e2[0];
+    const float inv = rsqrtf(dot3f(N, N));
+#pragma unroll
+    for(int a = 0; a < 3; ++a) N[a] *= inv;
+  } else {
+    N[0] = N[1] = 0.f;
+    N[2] = 1.f;
+  }
+}
+
+/* The normal turned to face the incoming direction's origin side
+ * (N . wo >= 0 with wo = -dir, ..._sw.c:402-420) */
+__device__ __forceinline__ void face_forwardf(float N[3], const float dir[3])
+{
+  if(dot3f(N, dir) > 0.f) {
+#pragma unroll
+    for(int a = 0; a < 3; ++a) N[a] = -N[a];
+  }
 }
 
 /*******************************************************************************
  * Segment iterator (FP32 version of SegIter in htgx_device.h)
  ******************************************************************************/
+__device__ __forceinline__ float half_bits_to_float(uint32_t b)
+{
+  return (float)__builtin_bit_cast(_Float16, (uint16_t)b);
+}
+
 __device__ __forceinline__ float planef(const SceneF& S, int a, uint32_t k)
 {
   return S.lower[a] + (float)k * S.bs[a];
@@ -276,12 +344,11 @@ template<bool COUNT>
 __device__ __forceinline__ void load_leaf(const SceneF& S, Lane& L, uint32_t cx, uint32_t cy,
                                           uint32_t cz)
 {
-  const uint32_t word = S.lmap[(cz * S.dims0[1] + cy) * S.dims0[0] + cx];
-  const uint32_t lvl = word >> kLeafIdBits;
-  L.leaf = word & kLeafIdMask;
-  const float2 b = S.bounds[L.leaf];
-  L.lmn = b.x;
-  L.lmx = b.y;
+  /* one 8-byte load: the leaf's level and its LWC bounds */
+  const uint2 rec = S.leafrec[(cz * S.dims0[1] + cy) * S.dims0[0] + cx];
+  const uint32_t lvl = rec.x >> kLeafIdBits;
+  L.lmn = half_bits_to_float(rec.y & 0xFFFFu) * S.lwc_scale;
+  L.lmx = half_bits_to_float(rec.y >> 16) * S.lwc_scale;
   const uint32_t c[3] = {cx, cy, cz};
   float best = INFINITY;
   int ax = 0;
@@ -497,10 +564,10 @@ __device__ __forceinline__ void sw_begin(const SceneF& S, Lane& L, int mask, con
   for(int a = 0; a < 3; ++a) { L.pos[a] = p[a]; L.dir[a] = d[a]; }
   L.ksi = 1.f;
   L.w = 0.f;
-  L.from_ground = false;
+  L.prev_prim = -1;
+  L.prev_edge = false;
   if((mask & HTGX_RADIANCE_DIRECT) && dot3f(L.dir, S.sun_dir) >= S.sun_cos_half) {
-    float t;
-    if(!ground_planef(S, L.pos, L.dir, FLT_MAX, false, &t)) {
+    if(!ground_tracef(S, L.pos, L.dir, FLT_MAX, -1, false, true).hit) {
       L.stage = ST_DIRECT;
       request(L, MODE_TR, HTGX_Kext, FLT_MAX);
       return;
@@ -543,7 +610,9 @@ __device__ __forceinline__ void consume(const SceneF& S, Lane& L, const float* u
       if(COUNT) L.c_ev++;
       L.surface = !hit;
       const float dst = L.surface ? L.qr : L.ev_dst;
-      L.from_ground = L.surface;
+      /* ..._sw.c:392: the next traces leave this surface hit, if any */
+      L.prev_prim = L.surface ? L.gh_prim : -1;
+      L.prev_edge = L.surface && L.gh_edge;
       L.stage = ST_ABS;
       L.ev_dst = dst;
       /* the absorption query runs from pos along dir up to the event */
@@ -558,7 +627,8 @@ __device__ __forceinline__ void consume(const SceneF& S, Lane& L, const float* u
 #pragma unroll
       for(int a = 0; a < 3; ++a) L.pos[a] = L.pos[a] + L.dir[a] * dst;
       if(L.surface) {
-        if(-(L.dir[2]) < 0.f) N[2] = -1.f; /* the normal faces the incoming ray */
+        ground_normalf(S, L.gh_prim, N);
+        face_forwardf(N, L.dir);
         local_to_worldf(N, sqrtf(1.f - u[0]), 2.f * kPif * u[1], L.dir_next);
         L.refl = S.mtl_albedo[L.gh_mtl];
       } else {
@@ -576,8 +646,7 @@ __device__ __forceinline__ void consume(const SceneF& S, Lane& L, const float* u
         const float mu = dot3f(L.dir, sun_dir);
         L.R = L.rayleigh ? rayleigh_evalf(mu) : hg_evalf(L.g, mu);
       }
-      float tg;
-      if(L.R <= 0.f || ground_planef(S, L.pos, sun_dir, FLT_MAX, L.from_ground, &tg)) {
+      if(L.R <= 0.f || ground_tracef(S, L.pos, sun_dir, FLT_MAX, L.prev_prim, L.prev_edge, true).hit) {
         sw_accumulate(S, L, 0.f, u[5]);
         break;
       }
@@ -608,9 +677,12 @@ __device__ __forceinline__ void consume(const SceneF& S, Lane& L, const float* u
       float dn[3];
       if(L.ev == EV_SCA) {
         local_to_worldf(L.dir, hg_cosf(L.g, u[0]), 2.f * kPif * u[1], dn);
-        L.from_ground = false;
+        L.prev_prim = -1; /* ..._lw.c:236 */
+        L.prev_edge = false;
       } else {
-        float N[3] = {0.f, 0.f, -(L.dir[2]) < 0.f ? -1.f : 1.f};
+        float N[3];
+        ground_normalf(S, L.gh_prim, N);
+        face_forwardf(N, L.dir);
         local_to_worldf(N, sqrtf(1.f - u[0]), 2.f * kPif * u[1], dn);
         if(u[6] >= S.mtl_albedo[L.gh_mtl]) {
           const float T = S.mtl_T ? S.mtl_T[L.gh_mtl] : 0.f;
@@ -618,7 +690,8 @@ __device__ __forceinline__ void consume(const SceneF& S, Lane& L, const float* u
           L.stage = ST_WALK_DONE;
           break;
         }
-        L.from_ground = true;
+        L.prev_prim = L.gh_prim; /* ..._lw.c:272 */
+        L.prev_edge = L.gh_edge;
       }
 #pragma unroll
       for(int a = 0; a < 3; ++a) L.dir[a] = dn[a];
@@ -659,9 +732,15 @@ __device__ __forceinline__ void real_begin(const SceneF& S, const DevMap& M, Lan
       org[a] = (float)M.pos[a] + (float)M.axis_x[a] * (sx * 2.f - 1.f)
              + (float)M.axis_y[a] * (sy * 2.f - 1.f);
     const float up[3] = {0.f, 0.f, 1.f};
-    float t;
-    /* an up ray hitting the (upward facing) ground from below starts inside */
-    if(ground_planef(S, org, up, FLT_MAX, false, &t)) { L.stage = ST_REAL_DONE; L.nfail++; L.w = -1.f; return; }
+    /* a sample under the ground (the up ray meets the ground's sky side from
+     * below, ..._draw_map.c:75-100) is rejected */
+    const GroundF gu = ground_tracef(S, org, up, INFINITY, -1, false, false);
+    if(gu.hit) {
+      float N[3];
+      ground_normalf(S, gu.prim, N);
+      /* cos(up, N) >= 0: the up ray travels in the ground's back side */
+      if(N[2] >= 0.f) { L.stage = ST_REAL_DONE; L.nfail++; L.w = -1.f; return; }
+    }
     const float z[3] = {(float)M.z[0], (float)M.z[1], (float)M.z[2]};
     local_to_worldf(z, sqrtf(1.f - u[2]), 2.f * kPif * u[3], dir);
   }
@@ -689,7 +768,8 @@ __device__ __forceinline__ void real_begin(const SceneF& S, const DevMap& M, Lan
     L.phase = 0;
 #pragma unroll
     for(int a = 0; a < 3; ++a) { L.pos[a] = org[a]; L.dir[a] = dir[a]; }
-    L.from_ground = false;
+    L.prev_prim = -1;
+    L.prev_edge = false;
     L.w = 0.f;
     L.stage = ST_LW;
     request_ground(L, MODE_LW, HTGX_Kext);
@@ -934,10 +1014,13 @@ wavefront_kernel(SceneF S, DevMap M, Launch P)
         if(L.nq) {
           L.nq = false;
           if(L.nq_ground) {
-            float t = FLT_MAX;
-            L.gh_hit = ground_planef(S, L.pos, L.dir, FLT_MAX, L.from_ground, &t);
-            L.gh_mtl = S.plane_mtl;
-            L.qr = L.gh_hit ? t : FLT_MAX;
+            const GroundF gh = ground_tracef(S, L.pos, L.dir, INFINITY, L.prev_prim, L.prev_edge,
+                                             false);
+            L.gh_hit = gh.hit;
+            L.gh_mtl = gh.mtl;
+            L.gh_prim = gh.prim;
+            L.gh_edge = gh.edge;
+            L.qr = gh.t;
           }
           L.Ts = exp1f(u[2 * NP - 1]);
           L.Tmin = 0.0;
@@ -1014,6 +1097,33 @@ zero_u64_kernel(unsigned long long* p, uint32_t n)
   if(i < n) p[i] = 0ull;
 }
 
+/* FP16 bits of a bound of v >= 0 in units of scale >= v, rounded down (lower
+ * bound) or up (upper bound) so that the decoded value, FP32
+ * half_bits_to_float(b) * scale as load_leaf computes it, never lies inside
+ * the true bounds */
+__device__ uint32_t bound_half_bits(float v, float scale, bool up)
+{
+  uint32_t b = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)(v / scale));
+  for(int i = 0; i < 4; ++i) {
+    const float d = half_bits_to_float(b) * scale;
+    if(up ? d >= v : d <= v) return b;
+    b = up ? b + 1u : b - 1u;
+  }
+  return up ? 0x3C00u /* 1.0: scale >= v */ : 0u;
+}
+
+__global__ void
+leafrec_kernel(const uint32_t* __restrict__ lmap, const float2* __restrict__ bounds, uint32_t n,
+               float scale, uint2* __restrict__ out)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  const uint32_t word = lmap[i];
+  const float2 b = bounds[word & kLeafIdMask];
+  out[i] = make_uint2(word, bound_half_bits(b.x, scale, false)
+                          | (bound_half_bits(b.y, scale, true) << 16));
+}
+
 template<class F>
 int launch_variant(F kernel, const SceneF& S, const DevMap& M, const Launch& P, uint32_t grid,
                    hipStream_t stream)
@@ -1075,12 +1185,35 @@ static int upload_as_float(htgx_wavefront_state* st, const T* dptr, size_t n, co
   return upload_f(st, f, out);
 }
 
+static void
+wavefront_free(htgx_wavefront_state* st)
+{
+  for(void* p : st->bufs) (void)hipFree(p);
+  if(st->d_counters) (void)hipFree(st->d_counters);
+  if(st->d_partials) (void)hipFree(st->d_partials);
+  delete st;
+}
+
+static int wavefront_build(htgx_ctx* ctx, htgx_wavefront_state* st);
+
 static int
 wavefront_prepare(htgx_ctx* ctx)
 {
   if(ctx->wf) return HTGX_OK;
-  htgx_wavefront_state* st = new htgx_wavefront_state();
-  ctx->wf = st;
+  htgx_wavefront_state* st = new(std::nothrow) htgx_wavefront_state();
+  if(!st) return HTGX_MEM_ERR;
+  const int res = wavefront_build(ctx, st);
+  if(res != HTGX_OK) {
+    wavefront_free(st);
+    return res;
+  }
+  ctx->wf = st; /* published only once complete */
+  return HTGX_OK;
+}
+
+static int
+wavefront_build(htgx_ctx* ctx, htgx_wavefront_state* st)
+{
   const DevScene& D = ctx->S;
   SceneF& S = st->S;
   std::memset(&S, 0, sizeof(S));
@@ -1112,8 +1245,19 @@ wavefront_prepare(htgx_ctx* ctx)
       if(D.dims0[a] > 0xFFFFu) return HTGX_BAD_ARG;
     }
     S.lwc = D.lwc;
-    S.lmap = D.lmap;
-    S.bounds = D.bounds;
+    /* fused leaf records; the scale is the grid's largest LWC (the root's
+     * upper bound, last node of the level-ordered bounds) */
+    float2 root;
+    HIP_OK(hipMemcpy(&root, D.bounds + (ctx->nnodes - 1), sizeof(float2), hipMemcpyDeviceToHost));
+    S.lwc_scale = root.y > 0.f ? root.y : 1.f;
+    const uint32_t nblocks = D.dims0[0] * D.dims0[1] * D.dims0[2];
+    uint2* rec = nullptr;
+    HIP_OK(hipMalloc(&rec, (size_t)nblocks * sizeof(uint2)));
+    st->bufs.push_back(rec);
+    leafrec_kernel<<<(nblocks + 255) / 256, 256>>>(D.lmap, D.bounds, nblocks, S.lwc_scale, rec);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipDeviceSynchronize());
+    S.leafrec = rec;
     std::vector<double> ce(D.nbands), ssa(D.nbands), g(D.nbands);
     HIP_OK(hipMemcpy(ce.data(), D.cext, D.nbands * sizeof(double), hipMemcpyDeviceToHost));
     HIP_OK(hipMemcpy(ssa.data(), D.ssa, D.nbands * sizeof(double), hipMemcpyDeviceToHost));
@@ -1131,8 +1275,12 @@ wavefront_prepare(htgx_ctx* ctx)
   S.ground_type = D.ground_type;
   S.plane_z = (float)D.plane_z;
   S.plane_mtl = D.plane_mtl;
-  if(D.ground_type == HTGX_GROUND_PLANE) {
-    const uint32_t nm = D.plane_mtl + 1;
+  S.bvh = D.bvh;
+  S.nbvh = D.nbvh;
+  S.tris = D.tris;
+  S.tri_mtl = D.tri_mtl;
+  if(D.ground_type == HTGX_GROUND_PLANE || D.ground_type == HTGX_GROUND_MESH) {
+    const uint32_t nm = ctx->nmtls;
     if((res = upload_as_float(st, D.mtl_albedo, nm, &S.mtl_albedo))) return res;
     if(D.mtl_T && (res = upload_as_float(st, D.mtl_T, nm, &S.mtl_T))) return res;
   }
@@ -1211,9 +1359,6 @@ int htgx_debug_realisations_fp32(htgx_ctx*, const htgx_map_desc*, const uint64_t
 void htgx_wavefront_release(htgx_ctx* ctx)
 {
   if(!ctx || !ctx->wf) return;
-  for(void* p : ctx->wf->bufs) (void)hipFree(p);
-  if(ctx->wf->d_counters) (void)hipFree(ctx->wf->d_counters);
-  if(ctx->wf->d_partials) (void)hipFree(ctx->wf->d_partials);
-  delete ctx->wf;
+  wavefront_free(ctx->wf);
   ctx->wf = nullptr;
 }

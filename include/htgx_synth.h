@@ -15,6 +15,15 @@
  * T = T_ground - lapse*z. Longwave bands: per (band, k-point), ka0 is
  * log-uniform in [lw_ka_min, lw_ka_max] and the k-point weights are
  * Dirichlet(1).
+ *
+ * Mesh ground (ground_type = HTGX_GROUND_MESH): a heightfield of hf_res^2
+ * vertices over the cloud grid's XY extent, fractional Brownian motion of
+ * hf_octaves octaves of periodic value noise with amplitude ratio
+ * 2^-hurst per octave, rescaled to [0, hf_relief] m. Each grid quad gives
+ * two triangles wound counter-clockwise seen from above, so that
+ * (v1 - v0) x (v2 - v0), the side of the sky, points up. Triangles take one
+ * of 4 materials by the height band of their centroid (albedo 0.1, 0.25,
+ * 0.4, 0.6, from low to high).
  */
 #ifndef HTGX_SYNTH_H
 #define HTGX_SYNTH_H
@@ -58,12 +67,18 @@ struct htgx_synth_params {
   double optical_thickness;
   uint32_t leaf_block;
   uint64_t seed;
+  /* mesh ground */
+  uint32_t hf_res;          /* vertices per side */
+  uint32_t hf_octaves;
+  double hf_relief;         /* m */
+  double hf_hurst;
 };
 
 struct htgx_synth;
 
 /* Parameters of a benchmark configuration: 0 = 64x64x32 clouds, SW 550 nm;
- * 1 = 256x256x64 clouds, SW; 2 = 256x256x64 clouds, LW 8 bands x 16 k */
+ * 1 = 256x256x64 clouds, SW; 2 = 256x256x64 clouds, LW 8 bands x 16 k;
+ * 4 = 512x512x128 clouds over 51.2 km, SW, 1024^2-vertex fBm ground */
 int htgx_synth_params_preset(int config, struct htgx_synth_params* params);
 
 /* Sensor/spectral settings of a configuration (image or flux map) */

@@ -1140,22 +1140,110 @@ struct ghit {
   double t;
   double N[3];
   uint32_t mtl;
+  int64_t prim;  /* 0 the plane, else the mesh triangle (input order); -1 none */
+  int edge;      /* the hit lies on an edge or vertex of its triangle */
 };
 
+static const struct ghit GHIT_NONE = {0, FLT_MAX, {0, 0, 0}, 0, -1, 0};
+
+/* Self-hit filter of src/core/htrdr_geometry.c:96-150 */
+static int
+on_edge_f(float u, float v)
+{
+  const float eps = 1e-4f;
+  const float w = 1.f - u - v;
+  return fabsf(u) <= eps || fabsf(u - 1.f) <= eps || fabsf(v) <= eps
+      || fabsf(v - 1.f) <= eps || fabsf(w) <= eps || fabsf(w - 1.f) <= eps;
+}
+
+/* Moller-Trumbore in FP32 on (v0, e1 = v1 - v0, e2 = v2 - v0), double-sided */
+static int
+tri_hit_f(const float v0[3], const float e1[3], const float e2[3], const float o[3],
+          const float d[3], float* t, float* u, float* v)
+{
+  const float p0 = d[1]*e2[2] - d[2]*e2[1];
+  const float p1 = d[2]*e2[0] - d[0]*e2[2];
+  const float p2 = d[0]*e2[1] - d[1]*e2[0];
+  const float det = e1[0]*p0 + e1[1]*p1 + e1[2]*p2;
+  float inv, s0, s1, s2, uu, q0, q1, q2, vv;
+  if(det == 0.f) return 0;
+  inv = 1.f / det;
+  s0 = o[0] - v0[0]; s1 = o[1] - v0[1]; s2 = o[2] - v0[2];
+  uu = (s0*p0 + s1*p1 + s2*p2) * inv;
+  if(!(uu >= 0.f && uu <= 1.f)) return 0;
+  q0 = s1*e1[2] - s2*e1[1];
+  q1 = s2*e1[0] - s0*e1[2];
+  q2 = s0*e1[1] - s1*e1[0];
+  vv = (d[0]*q0 + d[1]*q1 + d[2]*q2) * inv;
+  if(!(vv >= 0.f && uu + vv <= 1.f)) return 0;
+  *t = (e2[0]*q0 + e2[1]*q1 + e2[2]*q2) * inv;
+  *u = uu;
+  *v = vv;
+  return 1;
+}
+
+/* Closest ground hit in ]range[0], range[1][ leaving the previous surface
+ * hit `prev' (htrdr_atmosphere_ground_trace_ray, ..._ground.c:170-217). The
+ * plane is the quad repeated to infinity with -R, a single primitive; a mesh
+ * is cast against every triangle in FP32, as the reference converts the ray
+ * (src/core/htrdr_geometry.c:698-700). */
 static struct ghit
 ground_trace(const struct oracle_sky* s, const double o[3], const double d[3],
-             const double range[2], int from_ground)
+             const double range[2], const struct ghit* prev)
 {
-  struct ghit h;
-  h.hit = 0;
-  h.t = FLT_MAX; /* distance of S3D_HIT_NULL */
-  if(s->scene.ground.type == HTGX_GROUND_PLANE && !from_ground && d[2] != 0) {
-    const double t = (s->scene.ground.plane_z - o[2]) / d[2];
+  struct ghit h = GHIT_NONE;
+  const int64_t pprim = prev ? prev->prim : -1;
+  const int pedge = prev ? prev->edge : 0;
+  const struct htgx_ground_desc* g = &s->scene.ground;
+  if(g->type == HTGX_GROUND_PLANE && pprim != 0 && d[2] != 0) {
+    const double t = (g->plane_z - o[2]) / d[2];
     if(t > range[0] && t < range[1]) {
       h.hit = 1;
       h.t = t;
       h.N[0] = 0; h.N[1] = 0; h.N[2] = 1;
-      h.mtl = s->scene.ground.plane_mtl;
+      h.mtl = g->plane_mtl;
+      h.prim = 0;
+    }
+  } else if(g->type == HTGX_GROUND_MESH) {
+    const float of[3] = {(float)o[0], (float)o[1], (float)o[2]};
+    const float df[3] = {(float)d[0], (float)d[1], (float)d[2]};
+    const float r0 = (float)range[0];
+    const float r1 = range[1] > (double)FLT_MAX ? INFINITY : (float)range[1];
+    float best = r1, bu = 0, bv = 0;
+    int64_t bt = -1;
+    uint32_t t;
+    for(t = 0; t < g->ntris; ++t) {
+      const float* p0 = g->verts + 3*(size_t)g->tris[3*(size_t)t + 0];
+      const float* p1 = g->verts + 3*(size_t)g->tris[3*(size_t)t + 1];
+      const float* p2 = g->verts + 3*(size_t)g->tris[3*(size_t)t + 2];
+      float e1[3], e2[3], tt, u, v;
+      int a;
+      for(a = 0; a < 3; ++a) { e1[a] = p1[a] - p0[a]; e2[a] = p2[a] - p0[a]; }
+      if(!tri_hit_f(p0, e1, e2, of, df, &tt, &u, &v)) continue;
+      if(tt >= best) continue;
+      /* self hits, src/core/htrdr_geometry.c:116-150 */
+      if(tt <= r0 || tt >= r1) continue;
+      if((int64_t)t == pprim) continue;
+      if(pprim >= 0 && fabsf(tt) <= 1e-1f && pedge && on_edge_f(u, v)) continue;
+      best = tt; bu = u; bv = v; bt = (int64_t)t;
+    }
+    if(bt >= 0) {
+      const float* p0 = g->verts + 3*(size_t)g->tris[3*(size_t)bt + 0];
+      const float* p1 = g->verts + 3*(size_t)g->tris[3*(size_t)bt + 1];
+      const float* p2 = g->verts + 3*(size_t)g->tris[3*(size_t)bt + 2];
+      float e1[3], e2[3];
+      double n[3];
+      int a;
+      for(a = 0; a < 3; ++a) { e1[a] = p1[a] - p0[a]; e2[a] = p2[a] - p0[a]; }
+      n[0] = (float)(e1[1]*e2[2] - e1[2]*e2[1]);
+      n[1] = (float)(e1[2]*e2[0] - e1[0]*e2[2]);
+      n[2] = (float)(e1[0]*e2[1] - e1[1]*e2[0]);
+      normalize3(h.N, n);
+      h.hit = 1;
+      h.t = (double)best;
+      h.mtl = g->tri_mtl[bt];
+      h.prim = bt;
+      h.edge = on_edge_f(bu, bv);
     }
   }
   return h;
@@ -1262,13 +1350,13 @@ radiance_sw
   const double L_sun = oracle_planck_monochromatic(wlen * 1e-9, 5778.0);
   double pos[3], dir[3];
   double ksi = 1, w = 0;
-  int from_ground = 0;
+  struct ghit hit_prev = GHIT_NONE; /* ..._sw.c:270 */
   memcpy(pos, pos_in, sizeof(pos));
   memcpy(dir, dir_in, sizeof(dir));
 
   if((cpnt_mask & HTGX_RADIANCE_DIRECT) && in_solar_cone(s, dir)) { /* :324-337 */
     const double range[2] = {0, FLT_MAX};
-    const struct ghit h = ground_trace(s, pos, dir, range, 0);
+    const struct ghit h = ground_trace(s, pos, dir, range, NULL);
     if(!h.hit) w = L_sun * transmissivity(s, rng, HTGX_Kext, iband, iquad, pos, dir, range);
   }
   if(!(cpnt_mask & HTGX_RADIANCE_DIFFUSE)) return w;
@@ -1281,7 +1369,7 @@ radiance_sw
     int surface, phase_kind = 0;
     const double r_inf[2] = {0, DBL_MAX};
 
-    gh = ground_trace(s, pos, dir, r_inf, from_ground);   /* :355-356 */
+    gh = ground_trace(s, pos, dir, r_inf, &hit_prev);     /* :355-356 */
     sc.s = s; sc.rng = rng; sc.o = pos; sc.d = dir;
     sc.iband = iband; sc.iquad = iquad;
     sc.Ts = ran_exp(rng);                                 /* :359 */
@@ -1296,7 +1384,7 @@ radiance_sw
     pos_next[0] = pos[0] + dir[0] * dst;                  /* :387-389 */
     pos_next[1] = pos[1] + dir[1] * dst;
     pos_next[2] = pos[2] + dir[2] * dst;
-    from_ground = surface;                                /* :392 */
+    hit_prev = surface ? gh : GHIT_NONE;                  /* :392 */
 
     range[0] = 0; range[1] = dst;                         /* :396-399 */
     Tr_abs = transmissivity(s, rng, HTGX_Ka, iband, iquad, pos, dir, range);
@@ -1335,7 +1423,7 @@ radiance_sw
       Tr = 0;
     } else {
       const double r_sun[2] = {0, FLT_MAX};
-      const struct ghit h = ground_trace(s, pos_next, sun_dir, r_sun, from_ground);
+      const struct ghit h = ground_trace(s, pos_next, sun_dir, r_sun, &hit_prev);
       Tr = h.hit ? 0
         : transmissivity(s, rng, HTGX_Kext, iband, iquad, pos_next, sun_dir, r_sun);
     }
@@ -1370,7 +1458,7 @@ radiance_lw
   const double g = s->scene.clouds.hg_g ? s->scene.clouds.hg_g[iband] : 0.0;
   const double wlen_m = wlen * 1e-9;
   double pos[3], dir[3], w = 0;
-  int from_ground = 0;
+  struct ghit hit_prev = GHIT_NONE; /* ..._lw.c:152 */
   memcpy(pos, pos_in, sizeof(pos));
   memcpy(dir, dir_in, sizeof(dir));
 
@@ -1380,7 +1468,7 @@ radiance_lw
     double range[2], pos_next[3], dir_next[3], dst;
     const double r_inf[2] = {0, DBL_MAX};
 
-    gh = ground_trace(s, pos, dir, r_inf, from_ground);   /* :188-190 */
+    gh = ground_trace(s, pos, dir, r_inf, &hit_prev);     /* :188-190 */
     c.s = s; c.rng = rng; c.o = pos; c.d = dir;
     c.iband = iband; c.iquad = iquad;
     c.Ts = ran_exp(rng);                                  /* :193 */
@@ -1403,7 +1491,7 @@ radiance_lw
     if(c.event == EV_SCA) {                               /* :233-236 */
       const double u0 = canonical(rng), u1 = canonical(rng);
       local_to_world(dir, hg_cos_sample(g, u0), 2.0 * PI * u1, dir_next);
-      from_ground = 0;
+      hit_prev = GHIT_NONE;                               /* :236 */
     } else {                                              /* :239-273 */
       const double wo[3] = {-dir[0], -dir[1], -dir[2]};
       double N[3], u0, u1, refl;
@@ -1418,7 +1506,7 @@ radiance_lw
         w = T > 0 ? oracle_planck_monochromatic(wlen_m, T) : 0;
         break;
       }
-      from_ground = 1;
+      hit_prev = gh;                                      /* :272 */
     }
     memcpy(pos, pos_next, sizeof(pos));
     memcpy(dir, dir_next, sizeof(dir));
@@ -1537,7 +1625,7 @@ realisation
     struct ghit h;
     for(a = 0; a < 3; ++a)
       org[a] = sn->pos[a] + sn->axis_x[a] * (sx*2.0 - 1.0) + sn->axis_y[a] * (sy*2.0 - 1.0);
-    h = ground_trace(s, org, up, r_inf, 0);                   /* :75-100 */
+    h = ground_trace(s, org, up, r_inf, NULL);                /* :75-100 */
     if(h.hit && dot3(up, h.N) >= 0) { *rejected = 1; return 0; }
     {
       const double u0 = canonical(&rng), u1 = canonical(&rng);

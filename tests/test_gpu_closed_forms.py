@@ -33,9 +33,9 @@ def render(scene, desc):
         return ctx.draw_map(desc)
 
 
-def check_estimates(E, SE, expect):
-    assert np.all(np.abs(E - expect) <= 5 * SE + 1e-12 * abs(expect))
-    assert abs(E.mean() - expect) <= 4 * math.sqrt((SE ** 2).sum()) / E.size + 1e-12 * abs(expect)
+def check_estimates(E, SE, expect, rtol=1e-12):
+    assert np.all(np.abs(E - expect) <= 5 * SE + rtol * abs(expect))
+    assert abs(E.mean() - expect) <= 4 * math.sqrt((SE ** 2).sum()) / E.size + rtol * abs(expect)
 
 
 def sun_camera(org, s, n, spp, solver, **kw):
@@ -64,19 +64,30 @@ def test_absorbing_slab_direct_sun(solver):
 
 
 @pytest.mark.parametrize("solver", SOLVERS)
-def test_absorbing_cloud_transmissivity(solver):
+@pytest.mark.parametrize("threshold", [1.0, 1e3])
+def test_absorbing_cloud_transmissivity(solver, threshold):
+    """At threshold 1 the leaves are single cells (k_min = k_max, nearly
+    deterministic); at 1e3 the whole grid is one leaf and the residual
+    tracking carries all the randomness. The pixels' rays sit up to 1e-6 rad
+    off the closed form's ray and cross cell boundaries elsewhere, which
+    moves the optical depth by up to ~1e-4 relative: rtol 2e-4. Checked on
+    the pooled image mean only: with tight bounds a real residual collision
+    (a zero) is a rare event, and a pixel that drew none reports an SE blind
+    to it (the FP32 solver's FP16 leaf bounds are rounded outwards, so its
+    k_min < k_max even for single-cell leaves)."""
     rng = np.random.default_rng(1)
     lwc = np.exp(rng.normal(-1.0, 0.8, (4, 8, 8))).astype(np.float32)
     lwc[rng.random(lwc.shape) < 0.3] = 0.0
     sc = Scene(lwc=lwc, lower=(0, 0, 1000), upper=(800, 800, 1400), cext=0.02, ssa=0.0,
-               sun=(20.0, 55.0), optical_thickness=1.0, leaf_block=1)
+               sun=(20.0, 55.0), optical_thickness=threshold, leaf_block=1)
     s = sun_dir(20.0, 55.0)
     org = np.array([-400.0, 100.0, 0.0])
     tau = sum((tb - ta) * 0.02 * float(lwc[iz, iy, ix])
               for ta, tb, (ix, iy, iz) in cells_along(lwc, sc.lower, sc.upper, org, s, 0.0, 1e5))
     p = render(sc, sun_camera(org, s, 4, 1024, solver))
     expect = L_SUN_550 * math.exp(-tau) / 1e9
-    check_estimates(p["radiance_E"], p["radiance_SE"], expect)
+    E, SE = p["radiance_E"], p["radiance_SE"]
+    assert abs(E.mean() - expect) <= 4 * math.sqrt((SE ** 2).sum()) / E.size + 2e-4 * expect
 
 
 @pytest.mark.parametrize("solver", SOLVERS)
@@ -88,8 +99,21 @@ def test_lw_isothermal_enclosure(solver):
                wlen=(LW_WLEN, LW_WLEN), spectral=F.SPECTRAL_LW, Tref=T, solver=solver)
     p = render(sc, d)
     assert np.allclose(p["radiance_E"], planck(LW_WLEN * 1e-9, T) / 1e9, rtol=DET_RTOL[solver], atol=0)
-    # brightness temperature of a black body at T is T (..._draw_map.c:425-435)
-    assert np.allclose(p["temperature_E"], T, atol=1e-3)
+    # Monochromatic map: the reference inverts the per-nm estimate (pdf 1 nm^-1
+    # = 1e9 m^-1, ..._draw_map.c:379,399) as a per-metre radiance
+    # (:425-435), so T_b is the temperature whose B_lambda is B_lambda(T)/1e9.
+    lam = LW_WLEN * 1e-9
+    E = p["radiance_E"]
+    Tq = 6.62607015e-34 * 299792458.0 / (lam * 1.380649e-23 * np.log1p(
+        2 * 6.62607015e-34 * 299792458.0 ** 2 / (lam ** 5 * E)))
+    assert np.allclose(p["temperature_E"], Tq, atol=2e-3)
+    # Band map with T_ref = T: each realisation is B_lambda(T) / pdf(lambda),
+    # nearly the band integral, and T_b is T (bisection to 1e-4 K; the
+    # 1-nm discrete pdf and the reference's sigma leave < 0.05 K)
+    d = camera(8, 8, 16, pos=(0, 0, 5000), tgt=(0.2, -0.1, 4999), fov=50.0,
+               wlen=LW_BANDS[0], spectral=F.SPECTRAL_LW, Tref=T, solver=solver)
+    p = render(sc, d)
+    assert np.allclose(p["temperature_E"], T, atol=0.05)
 
 
 @pytest.mark.parametrize("solver", SOLVERS)

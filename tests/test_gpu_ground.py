@@ -1,0 +1,140 @@
+"""The mesh ground on the device (the GPU BVH replacing Star-3D/Embree,
+src/core/htrdr_geometry.c:686-718) against the oracle's brute-force ray
+casting, and both solvers on mesh scenes with closed forms.
+
+Bars: FP64 device vs oracle single realisations >= 97% identical to 1e-9
+(BVH and brute force may pick different triangles on exact distance ties);
+flux-map rejected samples equal to within 0.2%; pixels within 3 combined
+sigma >= 99.5%; FP32 vs FP64 z-scores as in test_gpu_wavefront.py; closed
+forms as in test_gpu_closed_forms.py.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_ffi as O
+from htrdr_kmin_amd import _ffi as F
+from htrdr_kmin_amd import atmosphere as A
+from scenes import Scene, camera, planck, rectangle, sun_solid_angle, SUN_HALF_ANGLE
+from test_oracle_ground import flat_quad, terrain, with_mesh
+from test_gpu_parity_maps import flux_estimate, pixel_parity, realisation_parity, zstats
+
+pytestmark = pytest.mark.gpu
+
+SOLVERS = [F.SOLVER_PLAIN_FP64, F.SOLVER_WAVEFRONT_FP32]
+DET_RTOL = {F.SOLVER_PLAIN_FP64: 1e-9, F.SOLVER_WAVEFRONT_FP32: 2e-5}
+
+
+@pytest.fixture(scope="module")
+def hills():
+    """cfg0's clouds over a 33x33-vertex fBm terrain of 400 m relief"""
+    s = A.Synth(0, seed=7, ground_type=F.GROUND_MESH, hf_res=33, hf_octaves=5, hf_relief=400.0,
+                hf_hurst=0.7)
+    ctx = A.Context(s.scene_ptr)
+    sky = O.Sky(s.scene_ptr)
+    yield s, ctx, sky
+    sky.close()
+    ctx.close()
+    s.close()
+
+
+def hills_camera(**kw):
+    args = dict(width=32, height=24, spp=16, pos=(3200.0, -1500.0, 1800.0),
+                tgt=(3200.0, -1499.0, 1799.4))
+    args.update(kw)
+    return A.map_preset(0, **args)
+
+
+def hills_flux(**kw):
+    args = dict(width=24, height=24, spp=16, sensor=F.SENSOR_RECTANGLE, pos=(3200.0, 3200.0, 200.0),
+                tgt=(3200.0, 3200.0, 201.0), up=(0.0, 1.0, 0.0), rect_size=(6000.0, 6000.0))
+    args.update(kw)
+    return A.map_preset(0, **args)
+
+
+def test_mesh_realisations_camera(hills):
+    _, ctx, sky = hills
+    assert realisation_parity(ctx, sky, hills_camera(), n=800) >= 0.97
+
+
+def test_mesh_realisations_flux(hills):
+    _, ctx, sky = hills
+    assert realisation_parity(ctx, sky, hills_flux(), n=800) >= 0.97
+
+
+def test_mesh_flux_map_vs_oracle(hills):
+    _, ctx, sky = hills
+    d = hills_flux()
+    p = ctx.draw_map(d)
+    acc = sky.draw_map(d)
+    # rejected samples: under the terrain (..._draw_map.c:75-100, :251)
+    nf_gpu = d.width * d.height * d.spp - int(p["flux_n"].sum())
+    nf_cpu = int(acc["nfail"].sum())
+    assert nf_cpu > 0
+    assert abs(nf_gpu - nf_cpu) <= 2e-3 * nf_cpu + 2
+    E, SE = flux_estimate(p)
+    Eo, SEo = A.accum_estimate(acc)
+    m = (p["flux_n"] > 0) & (acc["n"] > 0)
+    assert pixel_parity(E[m], SE[m], Eo[m], SEo[m]) >= 0.995
+
+
+def test_mesh_camera_map_vs_oracle(hills):
+    _, ctx, sky = hills
+    d = hills_camera()
+    p = ctx.draw_map(d)
+    Eo, SEo = A.accum_estimate(sky.draw_map(d))
+    assert pixel_parity(p["radiance_E"], p["radiance_SE"], Eo, SEo) >= 0.995
+
+
+def test_mesh_fp32_vs_fp64(hills):
+    _, ctx, _ = hills
+    for make in (hills_camera, hills_flux):
+        d = make(spp=64)
+        p64 = ctx.draw_map(d)
+        d.solver = F.SOLVER_WAVEFRONT_FP32
+        p32 = ctx.draw_map(d)
+        if d.sensor == F.SENSOR_RECTANGLE:
+            # the same samples are rejected up to FP32 origins within a few mm
+            # of the terrain; pixels with few accepted samples have no usable
+            # standard error
+            n32, n64 = p32["flux_n"].astype(np.int64), p64["flux_n"].astype(np.int64)
+            assert abs(n32.sum() - n64.sum()) <= 5e-3 * n64.sum()
+            keep = (n32 >= 16) & (n64 >= 16)
+            a = tuple(x[keep] for x in flux_estimate(p32))
+            b = tuple(x[keep] for x in flux_estimate(p64))
+        else:
+            a, b = (p32["radiance_E"], p32["radiance_SE"]), (p64["radiance_E"], p64["radiance_SE"])
+        within, zm, zs = zstats(*a, *b)
+        assert within >= 0.99 and abs(zm) < 0.1 and zs < 1.2
+
+
+@pytest.mark.parametrize("solver", SOLVERS)
+def test_flat_mesh_vacuum_ground(solver):
+    v, t = flat_quad()
+    sc = with_mesh(Scene(sun=(0.0, 60.0)), v, t, [0, 0], [0.3], [0.0])
+    with A.Context(sc.ptr) as ctx:
+        p = ctx.draw_map(camera(16, 12, 32, pos=(0, 0, 3000), tgt=(0, 1, 2999), fov=40.0,
+                                solver=solver))
+    L_sun = planck(550e-9, 5778.0)
+    expect = L_sun * sun_solid_angle() * 0.3 / math.pi * math.sin(math.radians(60)) * (
+        1 + math.cos(SUN_HALF_ANGLE)) / 2 / 1e9
+    E, SE = p["radiance_E"], p["radiance_SE"]
+    assert np.all(np.abs(E - expect) <= 5 * SE)
+    assert abs(E.mean() - expect) <= 4 * math.sqrt((SE ** 2).sum()) / E.size
+
+
+@pytest.mark.parametrize("solver", SOLVERS)
+def test_lw_isothermal_heightfield(solver):
+    """Thick isothermal gas over a reflecting heightfield at the same T: every
+    realisation is B(T) (a leak through the mesh would score 0)."""
+    T = 270.0
+    _, v, t, _ = terrain(res=17, relief=300.0, seed=5)
+    sc = Scene(bands=((9500.0, 10500.0),), nlayers=50, dz=400.0, ka=np.full((1, 50), 1e-2),
+               T=np.full(50, T))
+    sc = with_mesh(sc, v, t, np.zeros(len(t)), [0.6], [T])
+    d = camera(16, 16, 16, pos=(3200.0, 3200.0, 450.0), tgt=(3200.3, 3200.2, 449.0), fov=60.0,
+               wlen=(10000.0, 10000.0), spectral=F.SPECTRAL_LW, Tref=T, solver=solver)
+    with A.Context(sc.ptr) as ctx:
+        p = ctx.draw_map(d)
+    assert np.allclose(p["radiance_E"], planck(10e-6, T) / 1e9, rtol=DET_RTOL[solver], atol=0)

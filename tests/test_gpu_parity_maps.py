@@ -1,0 +1,129 @@
+"""Parity of the flux-map and longwave paths (draw_pixel_flux,
+src/atmosphere/htrdr_atmosphere_draw_map.c:210-322, and the LW walk,
+..._compute_radiance_lw.c) with the oracle, and of the FP32 solver with the
+FP64 one on them.
+
+Scenes: the LW configuration (4-20 um, 8 bands x 16 k-points, 16000 one-nm
+CDF bands, temperature lapse, log-uniform gas ka) on a 64x64x32 cloud grid so
+that the oracle stays fast, and the cfg0 SW scene. Bars:
+* FP64 device vs oracle, single realisations on the same keys: >= 97%
+  identical to 1e-9 (the rest differ through last-ulp transcendental
+  differences that flip a branch);
+* FP64 device vs oracle per pixel: >= 99.5% within 3 combined sigma;
+* FP32 vs FP64 per pixel: >= 99% within 3 sigma, |mean z| < 0.1, z std < 1.2.
+"""
+import numpy as np
+import pytest
+
+import oracle_ffi as O
+from htrdr_kmin_amd import _ffi as F
+from htrdr_kmin_amd import atmosphere as A
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def lw():
+    s = A.Synth(2, seed=3, cloud_def=(64, 64, 32), cloud_upper=(6400.0, 6400.0, 4200.0))
+    ctx = A.Context(s.scene_ptr)
+    sky = O.Sky(s.scene_ptr)
+    yield s, ctx, sky
+    sky.close()
+    ctx.close()
+    s.close()
+
+
+def lw_flux(**kw):
+    d = A.map_preset(2, width=32, height=32, spp=16, pos=(3200.0, 3200.0, 2000.0),
+                     tgt=(3200.0, 3200.0, 2001.0), rect_size=(6400.0, 6400.0))
+    for k, v in kw.items():
+        setattr(d, k, v)
+    return d
+
+
+def flux_estimate(p):
+    return A.estimate(p["flux_sum"], p["flux_sum2"], p["flux_n"])
+
+
+def realisation_parity(ctx, sky, desc, n=1000, seed=4):
+    rng = np.random.default_rng(seed)
+    pix = rng.integers(0, desc.width * desc.height, size=n).astype(np.uint64)
+    real = rng.integers(0, 1 << 20, size=n).astype(np.uint64)
+    g = ctx.realisations(desc, pix, real)
+    o = np.array([sky.realisation(desc, int(p), int(r)) for p, r in zip(pix, real)])
+    both_nan = np.isnan(g) & np.isnan(o)
+    same = both_nan | np.isclose(g, o, rtol=1e-9, atol=0)
+    print("identical realisations", same.mean())
+    return same.mean()
+
+
+def pixel_parity(E, SE, E2, SE2):
+    dE = np.abs(E - E2)
+    sig = np.sqrt(SE ** 2 + SE2 ** 2)
+    ok = (dE <= 3 * sig) | (dE <= 1e-12 * np.abs(E2))
+    print("pixels within 3 sigma", ok.mean())
+    return ok.mean()
+
+
+def zstats(E, SE, E2, SE2):
+    s = np.sqrt(SE ** 2 + SE2 ** 2)
+    m = s > 0
+    z = (E[m] - E2[m]) / s[m]
+    print("within 3 sigma", np.mean(np.abs(z) <= 3), "z mean", z.mean(), "z std", z.std())
+    return np.mean(np.abs(z) <= 3), z.mean(), z.std()
+
+
+def test_lw_flux_realisations(lw):
+    _, ctx, sky = lw
+    assert realisation_parity(ctx, sky, lw_flux()) >= 0.97
+
+
+def test_lw_flux_map_vs_oracle(lw):
+    _, ctx, sky = lw
+    d = lw_flux()
+    p = ctx.draw_map(d)
+    E, SE = flux_estimate(p)
+    acc = sky.draw_map(d)
+    Eo, SEo = A.accum_estimate(acc)
+    assert np.all(p["flux_n"] == acc["n"])
+    assert pixel_parity(E, SE, Eo, SEo) >= 0.995
+    assert np.all(E > 0)
+
+
+def test_lw_flux_fp32_vs_fp64(lw):
+    _, ctx, _ = lw
+    p64 = ctx.draw_map(lw_flux(spp=64))
+    p32 = ctx.draw_map(lw_flux(spp=64, solver=F.SOLVER_WAVEFRONT_FP32))
+    within, zm, zs = zstats(*flux_estimate(p32), *flux_estimate(p64))
+    assert within >= 0.99 and abs(zm) < 0.1 and zs < 1.2
+
+
+def test_lw_camera_brightness_temperature(lw):
+    """LW camera: realisation parity, and the pixel brightness temperatures
+    of both solvers are physical (between the coldest layer and the ground)."""
+    _, ctx, sky = lw
+    d = A.map_preset(0, width=24, height=16, spp=16, spectral_type=F.SPECTRAL_LW,
+                     wlen_range=(4000.0, 20000.0), ref_temperature=290.0)
+    assert realisation_parity(ctx, sky, d, n=600) >= 0.97
+    for solver in (F.SOLVER_PLAIN_FP64, F.SOLVER_WAVEFRONT_FP32):
+        d.solver = solver
+        p = ctx.draw_map(d)
+        assert np.all((p["temperature_E"] > 150) & (p["temperature_E"] < 300))
+
+
+def test_sw_flux_realisations_and_map(synth0, ctx0, sky0):
+    d = A.map_preset(0, width=24, height=24, spp=16, sensor=F.SENSOR_RECTANGLE,
+                     pos=(3200.0, 3200.0, 2600.0), tgt=(3200.0, 3200.0, 2601.0), up=(0.0, 1.0, 0.0),
+                     rect_size=(6400.0, 6400.0))
+    assert realisation_parity(ctx0, sky0, d, n=800) >= 0.97
+    p = ctx0.draw_map(d)
+    acc = sky0.draw_map(d)
+    Eo, SEo = A.accum_estimate(acc)
+    E, SE = flux_estimate(p)
+    assert pixel_parity(E, SE, Eo, SEo) >= 0.995
+    d.spp = 64
+    p64 = ctx0.draw_map(d)
+    d.solver = F.SOLVER_WAVEFRONT_FP32
+    p32 = ctx0.draw_map(d)
+    within, zm, zs = zstats(*flux_estimate(p32), *flux_estimate(p64))
+    assert within >= 0.99 and abs(zm) < 0.1 and zs < 1.2
