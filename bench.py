@@ -251,9 +251,21 @@ def main():
         acc64.free()
         real64 = npix * S * args.fp64_steps
         E64, SE64 = A.accum_estimate(h64)
-        sig = np.sqrt(SE_main ** 2 + SE64 ** 2)
+        # per-pixel agreement at equal sample counts: an FP32 pass (not timed)
+        # over as many realisations as the FP64 leg, beyond the timed ones
+        d32 = A.map_preset(args.config)
+        for f, _ in d32._fields_:
+            setattr(d32, f, getattr(desc, f))
+        d32.solver = F.SOLVER_WAVEFRONT_FP32
+        d32.spp = S * args.fp64_steps
+        d32.spp_offset = S * (args.steps + args.warmup)
+        acc.zero()
+        ctx.render_accum(d32, acc.address, stream.h.value)
+        stream.synchronize()
+        E32, SE32 = A.accum_estimate(acc.download(np.zeros(npix, A.ACCUM_DTYPE)))
+        sig = np.sqrt(SE32 ** 2 + SE64 ** 2)
         m = sig > 0
-        z = (E_main[m] - E64[m]) / sig[m]
+        z = (E32[m] - E64[m]) / sig[m]
         # Per-realisation variance of each solver pooled over the pixels and
         # normalised by the same precise means (the FP32 run's, many more
         # realisations): sum_p s2_p / sum_p E_p^2 with s2_p the unbiased
@@ -279,6 +291,7 @@ def main():
         out["fp32_pooled_rel_variance_per_realisation"] = round(relvar_main, 4)
         out["time_to_0p5pct_speedup_fp32_vs_fp64"] = round(speedup, 3)
         out["fp32_vs_fp64_per_pixel"] = {
+            "spp_each": S * args.fp64_steps,
             "within_3_sigma": round(float(np.mean(np.abs(z) <= 3)), 5),
             "z_mean": round(float(z.mean()), 4), "z_std": round(float(z.std()), 4),
             "image_mean_rel_diff": round(float((E_main.mean() - E64.mean()) / E64.mean()), 6),

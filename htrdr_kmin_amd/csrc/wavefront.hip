@@ -830,8 +830,10 @@ __device__ __forceinline__ uint64_t wave_sum64(uint64_t v)
   return v;
 }
 
+/* 3 waves per SIMD: the register cap (168) spills a little but hides more
+ * latency than 2 waves without spills (measured 14% faster at 1024^2) */
 template<int SENSOR, int SPECTRAL, bool COUNT>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 wavefront_kernel(SceneF S, DevMap M, Launch P)
 {
   /* canonicals per event: 8, plus 2 for the sun of a SW flux map */
@@ -1159,7 +1161,7 @@ struct htgx_wavefront_state {
   size_t partials_cap = 0;
   uint32_t grid = 0;
   uint32_t K = 4;
-  uint32_t ev_thresh = 16;
+  uint32_t ev_thresh = 24;
 };
 
 template<class T>
