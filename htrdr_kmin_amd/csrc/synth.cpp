@@ -353,10 +353,10 @@ htgx_synth_create(const htgx_synth_params* p, htgx_synth** out)
   }
   if(p->ground_type == HTGX_GROUND_MESH) {
     if(p->hf_res < 2 || !p->hf_octaves || p->hf_octaves > 20) { delete s; return HTGX_BAD_ARG; }
-    const double x0 = p->cloud_def[0] ? p->cloud_lower[0] : 0.0;
-    const double x1 = p->cloud_def[0] ? p->cloud_upper[0] : 1.0;
-    const double y0 = p->cloud_def[1] ? p->cloud_lower[1] : 0.0;
-    const double y1 = p->cloud_def[1] ? p->cloud_upper[1] : 1.0;
+    /* over the configured cloud extent (also when the clouds are off) */
+    const bool ext = p->cloud_upper[0] > p->cloud_lower[0] && p->cloud_upper[1] > p->cloud_lower[1];
+    const double x0 = ext ? p->cloud_lower[0] : 0.0, x1 = ext ? p->cloud_upper[0] : 1.0;
+    const double y0 = ext ? p->cloud_lower[1] : 0.0, y1 = ext ? p->cloud_upper[1] : 1.0;
     heightfield(p, x0, x1, y0, y1, s);
     s->mtl_albedo = {0.1, 0.25, 0.4, 0.6};
     s->mtl_temperature.assign(4, p->ground_temperature);

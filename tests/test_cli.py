@@ -1,0 +1,107 @@
+"""The C command line `htrdr-atmosphere` (htrdr_kmin_amd/cli): the
+reference's options (src/atmosphere/htrdr_atmosphere_args.c:124-255) and its
+htrdr-image output (doc/htrdr-image.5). CPU tests cover argument handling;
+the GPU test renders through the CLI and compares its output with the
+Python mirror's dump of the same map."""
+import io
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from htrdr_kmin_amd import _ffi as F
+from htrdr_kmin_amd import atmosphere as A
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "htrdr_kmin_amd", "htrdr-atmosphere")
+
+
+def run(*args, env=None):
+    if not os.path.exists(CLI):
+        pytest.skip("htrdr_kmin_amd/htrdr-atmosphere not built")
+    return subprocess.run([CLI, *args], capture_output=True, text=True, timeout=120, env=env)
+
+
+def test_help():
+    r = run("-h")
+    assert r.returncode == 0 and "Usage: htrdr-atmosphere" in r.stdout
+
+
+@pytest.mark.parametrize("args,msg", [
+    ((), "option '-a'"),
+    (("-a", "gas.nc"), "htgop files cannot be read"),
+    (("-a", "synth:0", "-c", "clouds.nc"), "Mie's data"),
+    (("-a", "synth:0", "-c", "clouds.nc", "-m", "mie.nc"), "htcp files cannot be read"),
+    (("-a", "synth:0", "-D", "400,10"), "invalid option argument"),
+    (("-a", "synth:0", "-D", "10,95"), "invalid option argument"),
+    (("-a", "synth:0", "-i", "def=0x4"), "invalid option argument"),
+    (("-a", "synth:0", "-C", "fov=180"), "invalid option argument"),
+    (("-a", "synth:0", "-s", "sw=600,500"), "invalid option argument"),
+    (("-a", "synth:0", "-T", "-1"), "invalid option argument"),
+    (("-a", "synth:0", "-d"), "not built"),
+    (("-a", "synth:0", "-P", "pos=0,0,0"), "not built"),
+    (("-a", "synth:0", "-r"), "not built"),
+    (("-a", "synth:9", "-s", "sw=550,550"), "unknown synthetic configuration"),
+])
+def test_bad_arguments(args, msg):
+    r = run(*args)
+    assert r.returncode != 0
+    assert msg in r.stderr
+
+
+def test_no_overwrite_without_force(tmp_path):
+    out = tmp_path / "img.txt"
+    out.write_text("keep")
+    r = run("-a", "synth:0", "-s", "sw=550,550", "-o", str(out))
+    assert r.returncode != 0 and "use -f" in r.stderr
+    assert out.read_text() == "keep"
+
+
+def parse_image(text):
+    lines = text.split("\n")
+    w, h = map(int, lines[0].split())
+    rows = []
+    for y in range(h):
+        base = 1 + y * (w + 1)
+        rows.append([list(map(float, lines[base + x].split())) for x in range(w)])
+        assert lines[base + w] == ""
+    return w, h, np.array(rows)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["image", "flux"])
+def test_cli_matches_library(tmp_path, kind):
+    """Same scene and map through the CLI and through Context.draw_map +
+    dump_buffer: identical columns 1-6 (the time columns are wall clocks)."""
+    out = tmp_path / "out.txt"
+    common = ["-a", "synth:0,3", "-c", "synth", "-g", "plane", "-R", "-D", "0,60", "-f",
+              "-o", str(out), "-i", "def=24x16:spp=8"]
+    if kind == "image":
+        extra = ["-s", "sw=550,550", "-C", "pos=3200,-2000,5000:tgt=3200,-1999.5,4999.133974596:up=0,0,1:fov=60"]
+    else:
+        extra = ["-s", "sw=550,550", "-p", "pos=3200,3200,2000:tgt=3200,3200,2001:up=0,1,0:sz=6400,6400"]
+    r = run(*(common + extra))
+    assert r.returncode == 0, r.stderr
+    assert "Time per radiative path" in r.stderr
+    w, h, cli = parse_image(out.read_text())
+    assert (w, h) == (24, 16)
+
+    s = A.Synth(0, seed=3, sun_azimuth=0.0, sun_elevation=60.0)
+    d = A.map_preset(0, width=24, height=16, spp=8, solver=F.SOLVER_WAVEFRONT_FP32)
+    if kind == "image":
+        d.pos[:] = (3200, -2000, 5000)
+        d.tgt[:] = (3200, -1999.5, 4999.133974596)
+        d.fov_y = 60.0
+    else:
+        d.sensor = F.SENSOR_RECTANGLE
+        d.pos[:] = (3200, 3200, 2000)
+        d.tgt[:] = (3200, 3200, 2001)
+        d.up[:] = (0, 1, 0)
+        d.rect_size[:] = (6400.0, 6400.0)
+    with A.Context(s.scene_ptr) as ctx:
+        px = ctx.draw_map(d)
+    buf = io.StringIO()
+    A.dump_buffer(px, d, buf)
+    _, _, lib = parse_image(buf.getvalue())
+    assert np.array_equal(cli[..., :6], lib[..., :6])
