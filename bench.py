@@ -19,9 +19,11 @@ Mrealisations/s; `time_to_0p5pct_s` is the time the solver needs, at the
 measured rate, to bring 99% of the pixels with E > 0 to a 0.5% relative
 standard error (N_req(p) = (sigma_p / (0.005 E_p))^2 from the measured
 per-pixel variance). `roofline` prices the dominant kernel's algorithmic
-bytes against the 8 TB/s HBM peak: per realisation 12 B per octree-leaf
-lookup (4 B level-map word + 8 B LWC bounds), 8 B per tentative collision
-(4 B LWC + 4 B gas coefficient) and 64 B per pixel per launch;
+bytes against the 8 TB/s HBM peak: per realisation one leaf record per
+octree-leaf lookup (8 B for the FP32 kernel: level word + FP16 bounds; 12 B
+for the FP64 kernel: level-map word + FP32 bounds), 8 B per tentative
+collision (4 B LWC + 4 B gas coefficient), 32 B per BVH node and 36 B per
+triangle of a mesh ground, and 64 B per pixel per launch;
 `cpu_baseline` is the CPU oracle timed on the host cores over a bounded
 pixel sample.
 """
@@ -194,7 +196,9 @@ def main():
     # roofline of the dominant kernel (one launch = one step): algorithmic
     # bytes per realisation from the kernel's own event counters
     creal = max(counters["nrealisations"], 1)
-    bytes_real = (12.0 * counters["nleaf"] + 8.0 * counters["ncoll"]) / creal + 64.0 / S
+    leaf_bytes = 8.0 if args.solver == "wavefront" else 12.0
+    bytes_real = (leaf_bytes * counters["nleaf"] + 8.0 * counters["ncoll"] + 32.0 * counters["nbvh"]
+                  + 36.0 * counters["ntri"]) / creal + 64.0 / S
     avg_launch_s = total_s / args.steps
     achieved = bytes_real * npix * S / avg_launch_s / 1e9
     traffic = None

@@ -645,8 +645,8 @@ htgx_get_counters(htgx_ctx* ctx, htgx_counters* c)
   c->nrealisations = ctx->last_nrealisations;
   c->nleaf = h[1];
   c->ncoll = h[3];
-  c->nbvh = 0;
-  c->ntri = 0;
+  c->nbvh = h[6]; /* counted by the FP32 solver */
+  c->ntri = h[7];
   c->nevents = h[5];
   c->nseg = h[2];
   c->nlwc = h[4];

@@ -10,7 +10,8 @@
 
 struct htgx_wavefront_state;
 
-/* Device counter slots: [1..5] walk counters, [8..23] wavefront diagnostics */
+/* Device counter slots: [1..5] walk counters, [6..7] BVH nodes and triangles,
+ * [8..23] wavefront diagnostics */
 #define HTGX_NSTATS 32
 
 struct htgx_ctx {

@@ -25,6 +25,7 @@ namespace htgx {
 struct MeshHit {
   float t, u, v;
   int32_t tri;   /* leaf-order triangle index; -1: no hit */
+  uint32_t nodes, ntris; /* BVH nodes and triangles visited */
 };
 
 HTGX_HD bool mesh_on_edge(float u, float v)
@@ -79,12 +80,14 @@ HTGX_HD MeshHit mesh_trace(const BvhNode* nodes, uint32_t nnodes, const BvhTri* 
   h.t = r1;
   h.u = h.v = 0.f;
   h.tri = -1;
+  h.nodes = h.ntris = 0;
   float inv[3];
 #pragma unroll
   for(int a = 0; a < 3; ++a) inv[a] = 1.f / d[a];
   uint32_t i = 0;
   while(i < nnodes) {
     const BvhNode& N = nodes[i];
+    h.nodes++;
     float tn = r0, tf = h.t;
 #pragma unroll
     for(int a = 0; a < 3; ++a) {
@@ -99,6 +102,7 @@ HTGX_HD MeshHit mesh_trace(const BvhNode* nodes, uint32_t nnodes, const BvhTri* 
     for(uint32_t k = 0; k < count; ++k) {
       float t, u, v;
       const int32_t tri = (int32_t)(first + k);
+      h.ntris++;
       if(!mesh_tri_hit(tris[first + k], o, d, &t, &u, &v)) continue;
       if(t >= h.t || mesh_self_hit(t, u, v, tri, r0, r1, prev_tri, prev_edge)) continue;
       h.t = t;

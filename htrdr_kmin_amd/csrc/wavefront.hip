@@ -171,7 +171,7 @@ struct Lane {
   float tsum, tsum2;
   uint32_t n, nfail;
   /* counters */
-  uint32_t c_leaf, c_seg, c_coll, c_lwc, c_ev;
+  uint32_t c_leaf, c_seg, c_coll, c_lwc, c_ev, c_bvh, c_tri;
 };
 
 /* Next pair of FP32 canonicals of the realisation: one Threefry-2x32-20 block
@@ -270,6 +270,7 @@ struct GroundF {
   int32_t prim;
   uint32_t mtl;
   float t;
+  uint32_t nodes, ntris; /* BVH work of the trace (counting build) */
 };
 
 __device__ __forceinline__ GroundF ground_tracef(const SceneF& S, const float o[3], const float d[3],
@@ -281,6 +282,7 @@ __device__ __forceinline__ GroundF ground_tracef(const SceneF& S, const float o[
   g.prim = -1;
   g.mtl = 0;
   g.t = FLT_MAX;
+  g.nodes = g.ntris = 0;
   if(S.ground_type == HTGX_GROUND_PLANE) {
     if(prev_prim != 0 && d[2] != 0.f) {
       const float tt = (S.plane_z - o[2]) / d[2];
@@ -288,6 +290,8 @@ __device__ __forceinline__ GroundF ground_tracef(const SceneF& S, const float o[
     }
   } else if(S.ground_type == HTGX_GROUND_MESH) {
     const MeshHit m = mesh_trace(S.bvh, S.nbvh, S.tris, o, d, 0.f, r1, prev_prim, prev_edge, any);
+    g.nodes = m.nodes;
+    g.ntris = m.ntris;
     if(m.tri >= 0) {
       g.hit = true;
       g.t = m.t;
@@ -556,6 +560,13 @@ __device__ __forceinline__ void sw_after_direct(Lane& L)
   request_ground(L, MODE_SC, HTGX_Ks);
 }
 
+template<bool COUNT>
+__device__ __forceinline__ void count_ground(Lane& L, const GroundF& g)
+{
+  if(COUNT) { L.c_bvh += g.nodes; L.c_tri += g.ntris; }
+}
+
+template<bool COUNT>
 __device__ __forceinline__ void sw_begin(const SceneF& S, Lane& L, int mask, const float p[3],
                                          const float d[3])
 {
@@ -567,7 +578,9 @@ __device__ __forceinline__ void sw_begin(const SceneF& S, Lane& L, int mask, con
   L.prev_prim = -1;
   L.prev_edge = false;
   if((mask & HTGX_RADIANCE_DIRECT) && dot3f(L.dir, S.sun_dir) >= S.sun_cos_half) {
-    if(!ground_tracef(S, L.pos, L.dir, FLT_MAX, -1, false, true).hit) {
+    const GroundF g = ground_tracef(S, L.pos, L.dir, FLT_MAX, -1, false, true);
+    count_ground<COUNT>(L, g);
+    if(!g.hit) {
       L.stage = ST_DIRECT;
       request(L, MODE_TR, HTGX_Kext, FLT_MAX);
       return;
@@ -646,7 +659,13 @@ __device__ __forceinline__ void consume(const SceneF& S, Lane& L, const float* u
         const float mu = dot3f(L.dir, sun_dir);
         L.R = L.rayleigh ? rayleigh_evalf(mu) : hg_evalf(L.g, mu);
       }
-      if(L.R <= 0.f || ground_tracef(S, L.pos, sun_dir, FLT_MAX, L.prev_prim, L.prev_edge, true).hit) {
+      bool shadowed = L.R <= 0.f;
+      if(!shadowed) {
+        const GroundF g = ground_tracef(S, L.pos, sun_dir, FLT_MAX, L.prev_prim, L.prev_edge, true);
+        count_ground<COUNT>(L, g);
+        shadowed = g.hit;
+      }
+      if(shadowed) {
         sw_accumulate(S, L, 0.f, u[5]);
         break;
       }
@@ -705,7 +724,7 @@ __device__ __forceinline__ void consume(const SceneF& S, Lane& L, const float* u
 
 /* Sensor ray, wavelength, band and k-point, then the first walk
  * (..._draw_map.c:348-400 and :233-302) */
-template<int SENSOR, int SPECTRAL>
+template<int SENSOR, int SPECTRAL, bool COUNT>
 __device__ __forceinline__ void real_begin(const SceneF& S, const DevMap& M, Lane& L, const float* u)
 {
   const uint32_t px = L.pixel % M.width, py = L.pixel / M.width;
@@ -735,6 +754,7 @@ __device__ __forceinline__ void real_begin(const SceneF& S, const DevMap& M, Lan
     /* a sample under the ground (the up ray meets the ground's sky side from
      * below, ..._draw_map.c:75-100) is rejected */
     const GroundF gu = ground_tracef(S, org, up, INFINITY, -1, false, false);
+    count_ground<COUNT>(L, gu);
     if(gu.hit) {
       float N[3];
       ground_normalf(S, gu.prim, N);
@@ -775,7 +795,7 @@ __device__ __forceinline__ void real_begin(const SceneF& S, const DevMap& M, Lan
     request_ground(L, MODE_LW, HTGX_Kext);
   } else if(SENSOR == HTGX_SENSOR_CAMERA) {
     L.phase = 0;
-    sw_begin(S, L, HTGX_RADIANCE_ALL, org, dir);
+    sw_begin<COUNT>(S, L, HTGX_RADIANCE_ALL, org, dir);
   } else {
     float sun_dir[3];
     local_to_worldf(S.sun_dir, 1.f - u[7] * (1.f - S.sun_cos_half), 2.f * kPif * u[8], sun_dir);
@@ -784,12 +804,12 @@ __device__ __forceinline__ void real_begin(const SceneF& S, const DevMap& M, Lan
     L.Ld = 0.f;
 #pragma unroll
     for(int a = 0; a < 3; ++a) { L.org[a] = org[a]; L.dir0[a] = dir[a]; }
-    if(L.cosN > 0.f) { L.phase = 1; sw_begin(S, L, HTGX_RADIANCE_DIRECT, org, sun_dir); }
-    else { L.phase = 2; sw_begin(S, L, HTGX_RADIANCE_DIFFUSE, org, dir); }
+    if(L.cosN > 0.f) { L.phase = 1; sw_begin<COUNT>(S, L, HTGX_RADIANCE_DIRECT, org, sun_dir); }
+    else { L.phase = 2; sw_begin<COUNT>(S, L, HTGX_RADIANCE_DIFFUSE, org, dir); }
   }
 }
 
-template<int SENSOR, int SPECTRAL>
+template<int SENSOR, int SPECTRAL, bool COUNT>
 __device__ __forceinline__ void walk_done(const SceneF& S, Lane& L)
 {
   if(SENSOR == HTGX_SENSOR_CAMERA) {
@@ -801,7 +821,7 @@ __device__ __forceinline__ void walk_done(const SceneF& S, Lane& L)
   } else if(L.phase == 1) {
     L.Ld = L.w;
     L.phase = 2;
-    sw_begin(S, L, HTGX_RADIANCE_DIFFUSE, L.org, L.dir0);
+    sw_begin<COUNT>(S, L, HTGX_RADIANCE_DIFFUSE, L.org, L.dir0);
   } else {
     L.stage = ST_REAL_DONE;
     L.w = (L.cosN * S.sun_solid_angle * L.Ld + kPif * L.w) / L.pdf;
@@ -843,7 +863,7 @@ wavefront_kernel(SceneF S, DevMap M, Launch P)
   L.stage = ST_FETCH;
   L.pending = false;
   L.nq = false;
-  L.c_leaf = L.c_seg = L.c_coll = L.c_lwc = L.c_ev = 0;
+  L.c_leaf = L.c_seg = L.c_coll = L.c_lwc = L.c_ev = L.c_bvh = L.c_tri = 0;
   L.n = L.nfail = 0;
   L.sum = L.sum2 = 0.0;
   L.tsum = L.tsum2 = 0.f;
@@ -979,7 +999,7 @@ wavefront_kernel(SceneF S, DevMap M, Launch P)
         for(int i = 0; i < NP; ++i) { const float2 v = draw2(L, key); u[2 * i] = v.x; u[2 * i + 1] = v.y; }
         WF_TICK(3)
         if(begin) {
-          real_begin<SENSOR, SPECTRAL>(S, M, L, u);
+          real_begin<SENSOR, SPECTRAL, COUNT>(S, M, L, u);
         }
         WF_TICK(4)
         if(!begin && L.pending) {
@@ -987,7 +1007,7 @@ wavefront_kernel(SceneF S, DevMap M, Launch P)
           consume<COUNT>(S, L, u);
         }
         WF_TICK(5)
-        while(L.stage == ST_WALK_DONE) walk_done<SENSOR, SPECTRAL>(S, L);
+        while(L.stage == ST_WALK_DONE) walk_done<SENSOR, SPECTRAL, COUNT>(S, L);
         WF_TICK(6)
         if(L.stage == ST_REAL_DONE) {
           const uint64_t t1 = wall_clock64();
@@ -1018,6 +1038,7 @@ wavefront_kernel(SceneF S, DevMap M, Launch P)
           if(L.nq_ground) {
             const GroundF gh = ground_tracef(S, L.pos, L.dir, INFINITY, L.prev_prim, L.prev_edge,
                                              false);
+            count_ground<COUNT>(L, gh);
             L.gh_hit = gh.hit;
             L.gh_mtl = gh.mtl;
             L.gh_prim = gh.prim;
@@ -1038,7 +1059,8 @@ wavefront_kernel(SceneF S, DevMap M, Launch P)
 
   if(COUNT) {
     const uint64_t a = wave_sum64(L.c_leaf), b = wave_sum64(L.c_seg), c = wave_sum64(L.c_coll),
-                   d = wave_sum64(L.c_lwc), e = wave_sum64(L.c_ev);
+                   d = wave_sum64(L.c_lwc), e = wave_sum64(L.c_ev), f = wave_sum64(L.c_bvh),
+                   g = wave_sum64(L.c_tri);
     if((threadIdx.x & 63u) == 0) {
       atomicAdd(P.stats + 8, (unsigned long long)d_it);
       atomicAdd(P.stats + 9, (unsigned long long)d_ce);
@@ -1054,6 +1076,8 @@ wavefront_kernel(SceneF S, DevMap M, Launch P)
       atomicAdd(P.stats + 3, (unsigned long long)c);
       atomicAdd(P.stats + 4, (unsigned long long)d);
       atomicAdd(P.stats + 5, (unsigned long long)e);
+      atomicAdd(P.stats + 6, (unsigned long long)f);
+      atomicAdd(P.stats + 7, (unsigned long long)g);
     }
   }
 }

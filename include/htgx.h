@@ -281,8 +281,8 @@ struct htgx_counters {
   uint64_t nrealisations;
   uint64_t nleaf;   /* octree leaf x gas layer segments visited */
   uint64_t ncoll;   /* tentative collisions (property fetches) */
-  uint64_t nbvh;    /* BVH nodes visited */
-  uint64_t ntri;    /* triangles tested */
+  uint64_t nbvh;    /* BVH nodes visited (FP32 solver only) */
+  uint64_t ntri;    /* triangles tested (FP32 solver only) */
   uint64_t nevents; /* scattering / surface events */
   uint64_t nseg;    /* segments handed to the null-collision loops */
   uint64_t nlwc;    /* cloud LWC fetches (4 B each) */

@@ -138,3 +138,15 @@ def test_lw_isothermal_heightfield(solver):
     with A.Context(sc.ptr) as ctx:
         p = ctx.draw_map(d)
     assert np.allclose(p["radiance_E"], planck(10e-6, T) / 1e9, rtol=DET_RTOL[solver], atol=0)
+
+
+def test_mesh_counters(hills):
+    """The FP32 solver counts BVH nodes and triangles (the 32 B and 36 B
+    terms of the byte model); the plane scenes count none."""
+    _, ctx, _ = hills
+    d = hills_flux(solver=F.SOLVER_WAVEFRONT_FP32)
+    ctx.set_counting(True)
+    ctx.draw_map(d)
+    c = ctx.counters()
+    ctx.set_counting(False)
+    assert c["nbvh"] > c["nrealisations"] and c["ntri"] > 0
