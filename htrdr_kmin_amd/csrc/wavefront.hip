@@ -125,25 +125,20 @@ enum {
 /*******************************************************************************
  * Lane state
  ******************************************************************************/
+/* Per-lane state the tracking steps use every iteration; it stays in
+ * registers */
 struct Lane {
   /* work */
-  uint32_t chunk, pixel, r, r_end;
+  uint32_t pixel, r;    /* key of the realisation's random stream */
   int lstate;
   uint32_t pair;        /* next Threefry-2x32 block of the realisation */
   /* realisation */
-  float pdf, g, L_sun, wlen;
-  uint32_t band, gka_off, gks_off; /* band; rows of gas ka (band, k-point) and ks (band) */
+  uint32_t gka_off, gks_off; /* rows of gas ka (band, k-point) and ks (band) */
   float cka, cks;       /* particle absorption / scattering per LWC of the band */
-  float org[3], dir0[3], cosN, Ld; /* flux SW */
-  uint64_t t0;
   /* walk */
-  int stage, mask, phase;
   bool surface, rayleigh, gh_hit, pending;
   bool prev_edge, gh_edge;
-  int32_t prev_prim, gh_prim; /* previous surface hit (-1: none) and current hit */
-  uint32_t gh_mtl;
-  float pos[3], dir[3], dir_next[3];
-  float ksi, w, Tr_abs, R, refl, gas_ratio, ev_dst;
+  float pos[3], dir[3];
   /* query */
   int mode, prop;
   bool nq, nq_ground;   /* a query is requested; its range comes from a ground trace */
@@ -152,7 +147,6 @@ struct Lane {
   float tc, Ts, maj, inv_maj;
   double Tmin;
   int ev;
-  int32_t ev_layer;
   uint32_t nsteps;
   /* segment iterator */
   int cstate;          /* 0 before grid, 1 in leaf, 2 outside, 3 locate at tit, 4 step out of leaf */
@@ -166,13 +160,33 @@ struct Lane {
   float s_t1, s_kmin, s_kmax, s_gka, s_gks; /* gas ka / ks of the segment's layer */
   bool s_cloud;
   int32_t s_layer;
-  /* chunk accumulator */
-  double sum, sum2;     /* plain FP64 sums over the few realisations of a chunk */
-  float tsum, tsum2;
-  uint32_t n, nfail;
   /* counters */
   uint32_t c_leaf, c_seg, c_coll, c_lwc, c_ev, c_bvh, c_tri;
 };
+
+/* Per-lane state used only by the event step (realisation constants, walk
+ * weights, ground-hit records, chunk sums), kept in LDS as arrays over the
+ * block's lanes so that it takes no registers across the tracking steps.
+ * Lane t touches element t of each array: no bank conflicts. */
+constexpr int kBlock = 256;
+struct LaneCold {
+  double sum[kBlock], sum2[kBlock]; /* plain FP64 sums over the few realisations of a chunk */
+  float pdf[kBlock], g[kBlock], L_sun[kBlock], wlen[kBlock];
+  float org[3][kBlock], dir0[3][kBlock], cosN[kBlock], Ld[kBlock]; /* flux SW */
+  uint32_t t0[kBlock];  /* realisation start, low bits of the wall clock */
+  int stage[kBlock], mask[kBlock], phase[kBlock];
+  float dir_next[3][kBlock];
+  float ksi[kBlock], w[kBlock], Tr_abs[kBlock], R[kBlock], refl[kBlock], gas_ratio[kBlock],
+        ev_dst[kBlock];
+  int32_t ev_layer[kBlock];
+  int32_t prev_prim[kBlock], gh_prim[kBlock]; /* previous surface hit (-1: none), current hit */
+  uint32_t gh_mtl[kBlock];
+  float tsum[kBlock], tsum2[kBlock];
+  uint32_t n[kBlock], nfail[kBlock], chunk[kBlock], r_end[kBlock];
+};
+__shared__ LaneCold g_cold;
+#define LC(f) (g_cold.f[threadIdx.x])
+#define LCA(f, a) (g_cold.f[a][threadIdx.x])
 
 /* Next pair of FP32 canonicals of the realisation: one Threefry-2x32-20 block
  * with counter (pair index, seed) under the key (pixel, realisation). The
@@ -555,8 +569,8 @@ __device__ __forceinline__ void request_ground(Lane& L, int mode, int prop)
 
 __device__ __forceinline__ void sw_after_direct(Lane& L)
 {
-  if(!(L.mask & HTGX_RADIANCE_DIFFUSE)) { L.stage = ST_WALK_DONE; return; }
-  L.stage = ST_SCATTER;
+  if(!(LC(mask) & HTGX_RADIANCE_DIFFUSE)) { LC(stage) = ST_WALK_DONE; return; }
+  LC(stage) = ST_SCATTER;
   request_ground(L, MODE_SC, HTGX_Ks);
 }
 
@@ -570,18 +584,18 @@ template<bool COUNT>
 __device__ __forceinline__ void sw_begin(const SceneF& S, Lane& L, int mask, const float p[3],
                                          const float d[3])
 {
-  L.mask = mask;
+  LC(mask) = mask;
 #pragma unroll
   for(int a = 0; a < 3; ++a) { L.pos[a] = p[a]; L.dir[a] = d[a]; }
-  L.ksi = 1.f;
-  L.w = 0.f;
-  L.prev_prim = -1;
+  LC(ksi) = 1.f;
+  LC(w) = 0.f;
+  LC(prev_prim) = -1;
   L.prev_edge = false;
   if((mask & HTGX_RADIANCE_DIRECT) && dot3f(L.dir, S.sun_dir) >= S.sun_cos_half) {
     const GroundF g = ground_tracef(S, L.pos, L.dir, FLT_MAX, -1, false, true);
     count_ground<COUNT>(L, g);
     if(!g.hit) {
-      L.stage = ST_DIRECT;
+      LC(stage) = ST_DIRECT;
       request(L, MODE_TR, HTGX_Kext, FLT_MAX);
       return;
     }
@@ -593,12 +607,12 @@ __device__ __forceinline__ void sw_begin(const SceneF& S, Lane& L, int mask, con
  * (..._sw.c:494-504); the next scattering query starts from the event */
 __device__ __forceinline__ void sw_accumulate(const SceneF& S, Lane& L, float Tr, float u_rr)
 {
-  L.ksi *= L.Tr_abs;
-  L.w += L.ksi * L.L_sun * Tr * L.R * S.sun_solid_angle;
-  if(L.surface && u_rr >= L.refl) { L.stage = ST_WALK_DONE; return; }
+  LC(ksi) *= LC(Tr_abs);
+  LC(w) += LC(ksi) * LC(L_sun) * Tr * LC(R) * S.sun_solid_angle;
+  if(L.surface && u_rr >= LC(refl)) { LC(stage) = ST_WALK_DONE; return; }
 #pragma unroll
-  for(int a = 0; a < 3; ++a) L.dir[a] = L.dir_next[a];
-  L.stage = ST_SCATTER;
+  for(int a = 0; a < 3; ++a) L.dir[a] = LCA(dir_next, a);
+  LC(stage) = ST_SCATTER;
   request_ground(L, MODE_SC, HTGX_Ks);
 }
 
@@ -611,57 +625,63 @@ __device__ __forceinline__ float query_tr(const Lane& L)
 template<bool COUNT>
 __device__ __forceinline__ void consume(const SceneF& S, Lane& L, const float* u)
 {
-  switch(L.stage) {
+  switch(LC(stage)) {
     case ST_DIRECT:
-      L.w = L.L_sun * query_tr(L);
+      LC(w) = LC(L_sun) * query_tr(L);
       sw_after_direct(L);
       break;
 
     case ST_SCATTER: {
       const bool hit = L.ev == EV_HIT;
-      if(!L.gh_hit && !hit) { L.stage = ST_WALK_DONE; break; }
+      if(!L.gh_hit && !hit) { LC(stage) = ST_WALK_DONE; break; }
       if(COUNT) L.c_ev++;
       L.surface = !hit;
-      const float dst = L.surface ? L.qr : L.ev_dst;
+      const float dst = L.surface ? L.qr : LC(ev_dst);
       /* ..._sw.c:392: the next traces leave this surface hit, if any */
-      L.prev_prim = L.surface ? L.gh_prim : -1;
+      LC(prev_prim) = L.surface ? LC(gh_prim) : -1;
       L.prev_edge = L.surface && L.gh_edge;
-      L.stage = ST_ABS;
-      L.ev_dst = dst;
+      LC(stage) = ST_ABS;
+      LC(ev_dst) = dst;
       /* the absorption query runs from pos along dir up to the event */
       request(L, MODE_TR, HTGX_Ka, dst);
     } break;
 
     case ST_ABS: {
-      L.Tr_abs = query_tr(L);
-      if(L.Tr_abs <= 0.f) { L.stage = ST_WALK_DONE; break; }
-      const float dst = L.ev_dst;
+      LC(Tr_abs) = query_tr(L);
+      if(LC(Tr_abs) <= 0.f) { LC(stage) = ST_WALK_DONE; break; }
+      const float dst = LC(ev_dst);
       float N[3] = {0.f, 0.f, 1.f};
 #pragma unroll
       for(int a = 0; a < 3; ++a) L.pos[a] = L.pos[a] + L.dir[a] * dst;
       if(L.surface) {
-        ground_normalf(S, L.gh_prim, N);
+        ground_normalf(S, LC(gh_prim), N);
         face_forwardf(N, L.dir);
-        local_to_worldf(N, sqrtf(1.f - u[0]), 2.f * kPif * u[1], L.dir_next);
-        L.refl = S.mtl_albedo[L.gh_mtl];
+        float dn[3];
+        local_to_worldf(N, sqrtf(1.f - u[0]), 2.f * kPif * u[1], dn);
+#pragma unroll
+        for(int a = 0; a < 3; ++a) LCA(dir_next, a) = dn[a];
+        LC(refl) = S.mtl_albedo[LC(gh_mtl)];
       } else {
-        L.rayleigh = u[4] < L.gas_ratio;
-        const float mu = L.rayleigh ? rayleigh_cosf(u[0]) : hg_cosf(L.g, u[0]);
-        local_to_worldf(L.dir, mu, 2.f * kPif * u[1], L.dir_next);
-        L.refl = 1.f;
+        L.rayleigh = u[4] < LC(gas_ratio);
+        const float mu = L.rayleigh ? rayleigh_cosf(u[0]) : hg_cosf(LC(g), u[0]);
+        float dn[3];
+        local_to_worldf(L.dir, mu, 2.f * kPif * u[1], dn);
+#pragma unroll
+        for(int a = 0; a < 3; ++a) LCA(dir_next, a) = dn[a];
+        LC(refl) = 1.f;
       }
       float sun_dir[3];
       local_to_worldf(S.sun_dir, 1.f - u[2] * (1.f - S.sun_cos_half), 2.f * kPif * u[3], sun_dir);
       if(L.surface) {
         const float c = dot3f(N, sun_dir);
-        L.R = c < 0.f ? 0.f : L.refl / kPif * c;
+        LC(R) = c < 0.f ? 0.f : LC(refl) / kPif * c;
       } else {
         const float mu = dot3f(L.dir, sun_dir);
-        L.R = L.rayleigh ? rayleigh_evalf(mu) : hg_evalf(L.g, mu);
+        LC(R) = L.rayleigh ? rayleigh_evalf(mu) : hg_evalf(LC(g), mu);
       }
-      bool shadowed = L.R <= 0.f;
+      bool shadowed = LC(R) <= 0.f;
       if(!shadowed) {
-        const GroundF g = ground_tracef(S, L.pos, sun_dir, FLT_MAX, L.prev_prim, L.prev_edge, true);
+        const GroundF g = ground_tracef(S, L.pos, sun_dir, FLT_MAX, LC(prev_prim), L.prev_edge, true);
         count_ground<COUNT>(L, g);
         shadowed = g.hit;
       }
@@ -671,7 +691,7 @@ __device__ __forceinline__ void consume(const SceneF& S, Lane& L, const float* u
       }
 #pragma unroll
       for(int a = 0; a < 3; ++a) L.dir[a] = sun_dir[a];
-      L.stage = ST_SUN;
+      LC(stage) = ST_SUN;
       request(L, MODE_TR, HTGX_Kext, FLT_MAX);
     } break;
 
@@ -680,41 +700,41 @@ __device__ __forceinline__ void consume(const SceneF& S, Lane& L, const float* u
       break;
 
     case ST_LW: {
-      if(!L.gh_hit && L.ev == EV_NONE) { L.w = 0.f; L.stage = ST_WALK_DONE; break; }
+      if(!L.gh_hit && L.ev == EV_NONE) { LC(w) = 0.f; LC(stage) = ST_WALK_DONE; break; }
       if(COUNT) L.c_ev++;
-      const float dst = L.ev != EV_NONE ? L.ev_dst : L.qr;
+      const float dst = L.ev != EV_NONE ? LC(ev_dst) : L.qr;
 #pragma unroll
       for(int a = 0; a < 3; ++a) L.pos[a] = L.pos[a] + L.dir[a] * dst;
       if(L.ev == EV_ABS) {
         const int32_t nl = (int32_t)S.nlayers;
-        const int32_t jj = L.ev_layer < 0 ? 0 : L.ev_layer >= nl ? nl - 1 : L.ev_layer;
+        const int32_t jj = LC(ev_layer) < 0 ? 0 : LC(ev_layer) >= nl ? nl - 1 : LC(ev_layer);
         /* no gas layers: no emission temperature */
-        L.w = S.gas_T && nl > 0 ? planckf(L.wlen, S.gas_T[jj]) : 0.f;
-        L.stage = ST_WALK_DONE;
+        LC(w) = S.gas_T && nl > 0 ? planckf(LC(wlen), S.gas_T[jj]) : 0.f;
+        LC(stage) = ST_WALK_DONE;
         break;
       }
       float dn[3];
       if(L.ev == EV_SCA) {
-        local_to_worldf(L.dir, hg_cosf(L.g, u[0]), 2.f * kPif * u[1], dn);
-        L.prev_prim = -1; /* ..._lw.c:236 */
+        local_to_worldf(L.dir, hg_cosf(LC(g), u[0]), 2.f * kPif * u[1], dn);
+        LC(prev_prim) = -1; /* ..._lw.c:236 */
         L.prev_edge = false;
       } else {
         float N[3];
-        ground_normalf(S, L.gh_prim, N);
+        ground_normalf(S, LC(gh_prim), N);
         face_forwardf(N, L.dir);
         local_to_worldf(N, sqrtf(1.f - u[0]), 2.f * kPif * u[1], dn);
-        if(u[6] >= S.mtl_albedo[L.gh_mtl]) {
-          const float T = S.mtl_T ? S.mtl_T[L.gh_mtl] : 0.f;
-          L.w = T > 0.f ? planckf(L.wlen, T) : 0.f;
-          L.stage = ST_WALK_DONE;
+        if(u[6] >= S.mtl_albedo[LC(gh_mtl)]) {
+          const float T = S.mtl_T ? S.mtl_T[LC(gh_mtl)] : 0.f;
+          LC(w) = T > 0.f ? planckf(LC(wlen), T) : 0.f;
+          LC(stage) = ST_WALK_DONE;
           break;
         }
-        L.prev_prim = L.gh_prim; /* ..._lw.c:272 */
+        LC(prev_prim) = LC(gh_prim); /* ..._lw.c:272 */
         L.prev_edge = L.gh_edge;
       }
 #pragma unroll
       for(int a = 0; a < 3; ++a) L.dir[a] = dn[a];
-      L.stage = ST_LW;
+      LC(stage) = ST_LW;
       request_ground(L, MODE_LW, HTGX_Kext);
     } break;
 
@@ -759,7 +779,7 @@ __device__ __forceinline__ void real_begin(const SceneF& S, const DevMap& M, Lan
       float N[3];
       ground_normalf(S, gu.prim, N);
       /* cos(up, N) >= 0: the up ray travels in the ground's back side */
-      if(N[2] >= 0.f) { L.stage = ST_REAL_DONE; L.nfail++; L.w = -1.f; return; }
+      if(N[2] >= 0.f) { LC(stage) = ST_REAL_DONE; LC(nfail)++; LC(w) = -1.f; return; }
     }
     const float z[3] = {(float)M.z[0], (float)M.z[1], (float)M.z[2]};
     local_to_worldf(z, sqrtf(1.f - u[2]), 2.f * kPif * u[3], dir);
@@ -767,45 +787,46 @@ __device__ __forceinline__ void real_begin(const SceneF& S, const DevMap& M, Lan
 
   double pdf;
   const double wlen = sample_wavelength(M, (double)u[4], (double)u[5], &pdf);
-  L.pdf = (float)(pdf * 1e9);
-  L.wlen = (float)wlen;
+  LC(pdf) = (float)(pdf * 1e9);
+  LC(wlen) = (float)wlen;
+  uint32_t band;
   {
     uint32_t lo = 0, hi = S.nbands - 1;
     while(lo < hi) { const uint32_t mid = (lo + hi) >> 1; if(S.band_hi[mid] > wlen) hi = mid; else lo = mid + 1; }
-    L.band = lo;
+    band = lo;
     const uint32_t off = S.quad_off[lo], nq = S.quad_off[lo + 1] - off;
     uint32_t q = 0;
     for(; q + 1 < nq; ++q) if(S.quad_cdf[off + q] > (double)u[6]) break;
     L.gka_off = (off + q) * S.nlayers;
     L.gks_off = lo * S.nlayers;
   }
-  L.g = S.has_clouds ? S.hg_g[L.band] : 0.f;
-  L.cka = S.has_clouds ? S.cka[L.band] : 0.f;
-  L.cks = S.has_clouds ? S.cks[L.band] : 0.f;
-  L.L_sun = planckf(L.wlen, 5778.f);
+  LC(g) = S.has_clouds ? S.hg_g[band] : 0.f;
+  L.cka = S.has_clouds ? S.cka[band] : 0.f;
+  L.cks = S.has_clouds ? S.cks[band] : 0.f;
+  LC(L_sun) = planckf(LC(wlen), 5778.f);
 
   if(SPECTRAL == HTGX_SPECTRAL_LW) {
-    L.phase = 0;
+    LC(phase) = 0;
 #pragma unroll
     for(int a = 0; a < 3; ++a) { L.pos[a] = org[a]; L.dir[a] = dir[a]; }
-    L.prev_prim = -1;
+    LC(prev_prim) = -1;
     L.prev_edge = false;
-    L.w = 0.f;
-    L.stage = ST_LW;
+    LC(w) = 0.f;
+    LC(stage) = ST_LW;
     request_ground(L, MODE_LW, HTGX_Kext);
   } else if(SENSOR == HTGX_SENSOR_CAMERA) {
-    L.phase = 0;
+    LC(phase) = 0;
     sw_begin<COUNT>(S, L, HTGX_RADIANCE_ALL, org, dir);
   } else {
     float sun_dir[3];
     local_to_worldf(S.sun_dir, 1.f - u[7] * (1.f - S.sun_cos_half), 2.f * kPif * u[8], sun_dir);
     const float z[3] = {(float)M.z[0], (float)M.z[1], (float)M.z[2]};
-    L.cosN = dot3f(z, sun_dir);
-    L.Ld = 0.f;
+    LC(cosN) = dot3f(z, sun_dir);
+    LC(Ld) = 0.f;
 #pragma unroll
-    for(int a = 0; a < 3; ++a) { L.org[a] = org[a]; L.dir0[a] = dir[a]; }
-    if(L.cosN > 0.f) { L.phase = 1; sw_begin<COUNT>(S, L, HTGX_RADIANCE_DIRECT, org, sun_dir); }
-    else { L.phase = 2; sw_begin<COUNT>(S, L, HTGX_RADIANCE_DIFFUSE, org, dir); }
+    for(int a = 0; a < 3; ++a) { LCA(org, a) = org[a]; LCA(dir0, a) = dir[a]; }
+    if(LC(cosN) > 0.f) { LC(phase) = 1; sw_begin<COUNT>(S, L, HTGX_RADIANCE_DIRECT, org, sun_dir); }
+    else { LC(phase) = 2; sw_begin<COUNT>(S, L, HTGX_RADIANCE_DIFFUSE, org, dir); }
   }
 }
 
@@ -813,18 +834,21 @@ template<int SENSOR, int SPECTRAL, bool COUNT>
 __device__ __forceinline__ void walk_done(const SceneF& S, Lane& L)
 {
   if(SENSOR == HTGX_SENSOR_CAMERA) {
-    L.stage = ST_REAL_DONE;
-    L.w = L.w / L.pdf;
+    LC(stage) = ST_REAL_DONE;
+    LC(w) = LC(w) / LC(pdf);
   } else if(SPECTRAL == HTGX_SPECTRAL_LW) {
-    L.stage = ST_REAL_DONE;
-    L.w = L.w * kPif / L.pdf;
-  } else if(L.phase == 1) {
-    L.Ld = L.w;
-    L.phase = 2;
-    sw_begin<COUNT>(S, L, HTGX_RADIANCE_DIFFUSE, L.org, L.dir0);
+    LC(stage) = ST_REAL_DONE;
+    LC(w) = LC(w) * kPif / LC(pdf);
+  } else if(LC(phase) == 1) {
+    LC(Ld) = LC(w);
+    LC(phase) = 2;
+    float o[3], d[3];
+#pragma unroll
+    for(int a = 0; a < 3; ++a) { o[a] = LCA(org, a); d[a] = LCA(dir0, a); }
+    sw_begin<COUNT>(S, L, HTGX_RADIANCE_DIFFUSE, o, d);
   } else {
-    L.stage = ST_REAL_DONE;
-    L.w = (L.cosN * S.sun_solid_angle * L.Ld + kPif * L.w) / L.pdf;
+    LC(stage) = ST_REAL_DONE;
+    LC(w) = (LC(cosN) * S.sun_solid_angle * LC(Ld) + kPif * LC(w)) / LC(pdf);
   }
 }
 
@@ -860,13 +884,13 @@ wavefront_kernel(SceneF S, DevMap M, Launch P)
   constexpr int NP = (SENSOR == HTGX_SENSOR_RECTANGLE && SPECTRAL == HTGX_SPECTRAL_SW) ? 5 : 4;
   Lane L;
   L.lstate = LS_EVENT;
-  L.stage = ST_FETCH;
+  LC(stage) = ST_FETCH;
   L.pending = false;
   L.nq = false;
   L.c_leaf = L.c_seg = L.c_coll = L.c_lwc = L.c_ev = L.c_bvh = L.c_tri = 0;
-  L.n = L.nfail = 0;
-  L.sum = L.sum2 = 0.0;
-  L.tsum = L.tsum2 = 0.f;
+  LC(n) = LC(nfail) = 0;
+  LC(sum) = LC(sum2) = 0.0;
+  LC(tsum) = LC(tsum2) = 0.f;
   L.pixel = L.r = 0;
   const uint32_t xcd = blockIdx.x % (uint32_t)kXcds;
   const uint32_t key = (uint32_t)M.seed ^ (uint32_t)(M.seed >> 32);
@@ -940,11 +964,11 @@ wavefront_kernel(SceneF S, DevMap M, Launch P)
           const float proba = (L.mode == MODE_TR ? k - L.s_kmin : k) * L.inv_maj;
           L.ev = uu.x < proba ? EV_HIT : EV_NONE;
           /* gas vs particle scattering at the event, ..._sw.c:427-438 */
-          if(L.mode == MODE_SC) L.gas_ratio = gks / ks;
+          if(L.mode == MODE_SC) LC(gas_ratio) = gks / ks;
         }
         if(L.ev != EV_NONE) {
-          L.ev_dst = L.tc;
-          L.ev_layer = L.s_layer;
+          LC(ev_dst) = L.tc;
+          LC(ev_layer) = L.s_layer;
           L.pending = true;
           L.lstate = LS_EVENT;
         } else if(++L.nsteps > P.max_steps) {
@@ -967,12 +991,12 @@ wavefront_kernel(SceneF S, DevMap M, Launch P)
     if(want && (__popcll(want) >= P.ev_thresh || !other)) {
       if(COUNT) { d_ee++; d_el += __popcll(want); }
       if(L.lstate == LS_EVENT) {
-        if(L.stage == ST_FETCH) {
+        if(LC(stage) == ST_FETCH) {
           uint32_t c;
           if(!fetch_chunk(P, xcd, &c)) {
             L.lstate = LS_DONE;
           } else {
-            L.chunk = c;
+            LC(chunk) = c;
             const uint32_t m = c & 63u, rest = c >> 6;
             const uint32_t s = rest % P.nsub, t = rest / P.nsub;
             const uint32_t x = (t % P.ntx) * 8u + ((m & 1u) | ((m >> 1) & 2u) | ((m >> 2) & 4u));
@@ -980,19 +1004,19 @@ wavefront_kernel(SceneF S, DevMap M, Launch P)
             if(x < P.width && y < P.height) {
               L.pixel = y * P.width + x;
               L.r = P.spp_offset + s * P.K;
-              L.r_end = L.r + P.K;
-              L.n = L.nfail = 0;
-              L.sum = L.sum2 = 0.0;
-              L.tsum = L.tsum2 = 0.f;
-              L.stage = ST_BEGIN;
+              LC(r_end) = L.r + P.K;
+              LC(n) = LC(nfail) = 0;
+              LC(sum) = LC(sum2) = 0.0;
+              LC(tsum) = LC(tsum2) = 0.f;
+              LC(stage) = ST_BEGIN;
             } /* else: outside the image, fetch again next time */
           }
         }
       }
       WF_TICK(2)
-      if(L.lstate == LS_EVENT && L.stage != ST_FETCH) {
-        const bool begin = L.stage == ST_BEGIN;
-        if(begin) { L.pair = 0; L.t0 = wall_clock64(); }
+      if(L.lstate == LS_EVENT && LC(stage) != ST_FETCH) {
+        const bool begin = LC(stage) == ST_BEGIN;
+        if(begin) { L.pair = 0; LC(t0) = (uint32_t)wall_clock64(); }
         /* one set of canonicals per event, drawn at a single site */
         float u[2 * NP];
 #pragma unroll
@@ -1007,28 +1031,28 @@ wavefront_kernel(SceneF S, DevMap M, Launch P)
           consume<COUNT>(S, L, u);
         }
         WF_TICK(5)
-        while(L.stage == ST_WALK_DONE) walk_done<SENSOR, SPECTRAL, COUNT>(S, L);
+        while(LC(stage) == ST_WALK_DONE) walk_done<SENSOR, SPECTRAL, COUNT>(S, L);
         WF_TICK(6)
-        if(L.stage == ST_REAL_DONE) {
+        if(LC(stage) == ST_REAL_DONE) {
           const uint64_t t1 = wall_clock64();
-          if(L.w >= 0.f) {
-            const double wd = (double)L.w;
-            const float us = (float)((double)(t1 - L.t0) * M.us_per_tick);
-            L.sum += wd;
-            L.sum2 += wd * wd;
-            L.tsum += us;
-            L.tsum2 += us * us;
-            L.n++;
+          if(LC(w) >= 0.f) {
+            const double wd = (double)LC(w);
+            const float us = (float)((double)((uint32_t)t1 - LC(t0)) * M.us_per_tick);
+            LC(sum) += wd;
+            LC(sum2) += wd * wd;
+            LC(tsum) += us;
+            LC(tsum2) += us * us;
+            LC(n)++;
           }
           L.r++;
-          if(L.r < L.r_end) {
-            L.stage = ST_BEGIN;
+          if(L.r < LC(r_end)) {
+            LC(stage) = ST_BEGIN;
           } else {
             htgx_mc_accum a;
-            a.sum = L.sum; a.sum_c = 0.0; a.sum2 = L.sum2; a.sum2_c = 0.0;
-            a.tsum = L.tsum; a.tsum2 = L.tsum2; a.n = L.n; a.nfail = L.nfail;
-            P.partials[L.chunk] = a;
-            L.stage = ST_FETCH;
+            a.sum = LC(sum); a.sum_c = 0.0; a.sum2 = LC(sum2); a.sum2_c = 0.0;
+            a.tsum = LC(tsum); a.tsum2 = LC(tsum2); a.n = LC(n); a.nfail = LC(nfail);
+            P.partials[LC(chunk)] = a;
+            LC(stage) = ST_FETCH;
           }
         }
         WF_TICK(7)
@@ -1036,12 +1060,12 @@ wavefront_kernel(SceneF S, DevMap M, Launch P)
         if(L.nq) {
           L.nq = false;
           if(L.nq_ground) {
-            const GroundF gh = ground_tracef(S, L.pos, L.dir, INFINITY, L.prev_prim, L.prev_edge,
+            const GroundF gh = ground_tracef(S, L.pos, L.dir, INFINITY, LC(prev_prim), L.prev_edge,
                                              false);
             count_ground<COUNT>(L, gh);
             L.gh_hit = gh.hit;
-            L.gh_mtl = gh.mtl;
-            L.gh_prim = gh.prim;
+            LC(gh_mtl) = gh.mtl;
+            LC(gh_prim) = gh.prim;
             L.gh_edge = gh.edge;
             L.qr = gh.t;
           }
