@@ -55,6 +55,10 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--seed", type=int, default=0)
+    p.add_argument("--batch-scale", type=float, default=1.0,
+                   help="--config 3: scale of the realisations per pixel of every map")
+    p.add_argument("--batch-sequential", action="store_true",
+                   help="--config 3: also time the maps as back-to-back single-map launches")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                    help="only run 5 plain device copies of MIB MiB (profiler calibration)")
     return p.parse_args()
@@ -107,10 +111,118 @@ def cpu_baseline(synth, desc, target_s):
                       f"{d.spp} realisations, FP64 C/OpenMP oracle, {dt:.1f} s"}
 
 
+def workload_name(synth, desc, S, steps):
+    from htrdr_kmin_amd import _ffi as F
+    sc = synth.scene
+    c = sc.clouds
+    kind = "image" if desc.sensor == F.SENSOR_CAMERA else "flux map"
+    band = ("SW %g nm" % desc.wlen_range[0] if desc.wlen_range[0] == desc.wlen_range[1]
+            else "%s %g-%g nm, %d bands x %d k-points" % (
+                "LW" if desc.spectral_type == F.SPECTRAL_LW else "SW", desc.wlen_range[0],
+                desc.wlen_range[1], sc.spectral.nbands, sc.spectral.nquads[0]))
+    ground = {F.GROUND_PLANE: "plane ground", F.GROUND_NONE: "no ground",
+              F.GROUND_MESH: "%d-triangle fBm ground" % sc.ground.ntris}[sc.ground.type]
+    return ("htrdr-atmosphere %s %dx%d, %d realisations/px (%d steps x %d), %s, cloud grid "
+            "%dx%dx%d over %.1fx%.1fx%.1f km, %d^3 leaf blocks, %s"
+            % (kind, desc.width, desc.height, S * steps, steps, S, band, c.def_[0], c.def_[1],
+               c.def_[2], (c.upper[0] - c.lower[0]) / 1e3, (c.upper[1] - c.lower[1]) / 1e3,
+               (c.upper[2] - c.lower[2]) / 1e3, sc.leaf_block, ground))
+
+
+def bench_batch(args):
+    """--config 3: the uneven batched workload (64 seeded scenes of uneven
+    grids, image sizes, realisations per pixel and k-points) rendered in one
+    persistent launch (htgx_render_batch), beside the same maps rendered by
+    back-to-back single-map launches."""
+    import numpy as np
+    from htrdr_kmin_amd import _ffi as F
+    from htrdr_kmin_amd import atmosphere as A
+    from htrdr_kmin_amd import hip as H
+    items = A.uneven_batch(seed=args.seed, n=64, spp_scale=args.batch_scale)
+    ctxs = [A.Context(s.scene_ptr) for s, _ in items]
+    bufs = [H.DeviceBuffer(d.width * d.height * A.ACCUM_DTYPE.itemsize) for _, d in items]
+    stream = H.Stream()
+    total = sum(d.width * d.height * d.spp for _, d in items)
+    # counting warm-up per map (4 realisations per pixel): byte rates
+    bytes_total = 0.0
+    for c, (_, d), b in zip(ctxs, items, bufs):
+        w = A.map_preset(0)
+        for f, _ in w._fields_:
+            setattr(w, f, getattr(d, f))
+        w.spp = 4
+        b.zero()
+        c.set_counting(True)
+        c.render_accum(w, b.address, stream.h.value)
+        stream.synchronize()
+        k = c.counters()
+        c.set_counting(False)
+        rate = (8.0 * k["nleaf"] + 8.0 * k["ncoll"] + 32.0 * k["nbvh"] + 36.0 * k["ntri"]) / max(
+            k["nrealisations"], 1)
+        bytes_total += rate * d.width * d.height * d.spp + 64.0 * d.width * d.height
+    print(f"[bench] batch: {len(items)} maps, {total / 1e9:.3f} G realisations", file=sys.stderr, flush=True)
+    for b in bufs:
+        b.zero()
+    e0, e1 = H.Event(), H.Event()
+    H.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    scene_ms = A.render_batch([(c, d, b.address) for c, (_, d), b in zip(ctxs, items, bufs)],
+                              stream.h.value)
+    e1.record(stream)
+    e1.synchronize()
+    batch_s = e0.elapsed_ms(e1) / 1e3
+    wall = time.perf_counter() - t0
+    print(f"[bench] batch: one launch {batch_s:.2f} s", file=sys.stderr, flush=True)
+    seq_s = None
+    if args.batch_sequential:
+        for b in bufs:
+            b.zero()
+        e0.record(stream)
+        for c, (_, d), b in zip(ctxs, items, bufs):
+            c.render_accum(d, b.address, stream.h.value)
+        e1.record(stream)
+        e1.synchronize()
+        seq_s = e0.elapsed_ms(e1) / 1e3
+        print(f"[bench] batch: back-to-back launches {seq_s:.2f} s", file=sys.stderr, flush=True)
+    achieved = bytes_total / batch_s / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(total / batch_s / 1e6, 4),
+        "unit": "Mrealisations/s",
+        "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": round(batch_s * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded scene generator, seed %d)" % args.seed,
+        "config": {
+            "workload": "uneven batch: 64 SW scenes, cloud grids 32^2x32 to 256^2x96, images 64^2 to "
+                        "512^2, 16-1024 realisations/px (x%g), 1-16 k-points, one persistent launch"
+                        % args.batch_scale,
+            "config_index": 3, "realisations": total, "wall_s": round(wall, 3),
+        },
+        "scene_completion_ms": [round(float(x), 1) for x in scene_ms],
+        "back_to_back_launches_s": None if seq_s is None else round(seq_s, 3),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        big = max(range(len(items)), key=lambda i: items[i][1].width * items[i][1].height)
+        out["cpu_baseline"] = cpu_baseline(items[big][0], items[big][1], args.cpu_seconds)
+    for c in ctxs:
+        c.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
+        return
+    if args.config == 3:
+        if args.gpus > 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+            sys.exit("bench.py runs on one GPU only; use --gpus 1")
+        bench_batch(args)
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:
@@ -221,9 +333,7 @@ def main():
         "dtype": "f32" if args.solver == "wavefront" else "f64",
         "data": "synthetic (seeded scene generator, seed %d)" % args.seed,
         "config": {
-            "workload": "htrdr-atmosphere SW image %dx%d, %d realisations/px (%d steps x %d), "
-                        "550 nm, cloud grid 256x256x64 over 25.6x25.6x3.2 km, 2^3 leaf blocks"
-                        % (desc.width, desc.height, S * args.steps, args.steps, S),
+            "workload": workload_name(synth, desc, S, args.steps),
             "solver": "FP32 batched wavefront" if args.solver == "wavefront" else "FP64 plain one-path-per-thread",
             "config_index": args.config,
             "spp_per_step": S,
@@ -300,7 +410,11 @@ def main():
             "z_mean": round(float(z.mean()), 4), "z_std": round(float(z.std()), 4),
             "image_mean_rel_diff": round(float((E_main.mean() - E64.mean()) / E64.mean()), 6),
         }
-    if not args.no_cpu:
+    if synth.scene.ground.type == F.GROUND_MESH:
+        # the oracle casts rays against every triangle: no bounded sample of a
+        # 2-million-triangle terrain fits in the time budget
+        out["cpu_baseline"] = None
+    elif not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(synth, desc, args.cpu_seconds)
     stream.destroy()
     acc.free()

@@ -91,6 +91,10 @@ class MapDesc(C.Structure):
                 ("seed", C.c_uint64), ("solver", C.c_int)]
 
 
+class BatchItem(C.Structure):
+    _fields_ = [("ctx", C.c_void_p), ("desc", C.POINTER(MapDesc)), ("accum", C.c_void_p)]
+
+
 class Segment(C.Structure):
     _fields_ = [("t0", C.c_double), ("t1", C.c_double), ("kmin", C.c_double),
                 ("kmax", C.c_double), ("leaf", C.c_int64), ("layer", C.c_int32),
@@ -135,6 +139,7 @@ SIGNATURES = {
                                             C.c_uint32, c_uint32_p]),
     "htgx_debug_realisations": (C.c_int, [C.c_void_p, C.POINTER(MapDesc), c_uint64_p,
                                           c_uint64_p, C.c_uint32, c_double_p]),
+    "htgx_render_batch": (C.c_int, [C.POINTER(BatchItem), C.c_uint32, C.c_void_p, c_double_p]),
     "htgx_debug_threefry": (C.c_int, [c_uint64_p, c_uint64_p, C.c_uint32, c_uint64_p]),
     "htgx_debug_threefry2x32": (C.c_int, [c_uint32_p, c_uint32_p, C.c_uint32, c_uint32_p]),
     "htgx_debug_sample_dirs": (C.c_int, [C.c_void_p, C.c_int, C.c_double, c_double_p,

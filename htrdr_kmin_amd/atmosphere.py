@@ -176,6 +176,54 @@ class Context:
             pass
 
 
+def uneven_batch(seed=0, n=64, spp_scale=1.0):
+    """The uneven batched workload: n seeded SW scenes and camera images.
+
+    Per scene: cloud grid uniform in [32, 256]^2 x [32, 96] cells (even),
+    100 m cells with the cfg0 LWC law; image N^2 with N uniform in
+    [64, 512] (multiple of 8); realisations per pixel log-uniform in
+    [16, 1024] (multiple of 4, times spp_scale); 1-16 k-points. The camera
+    looks at the grid from its south edge like cfg0's.
+    Returns [(Synth, MapDesc)]."""
+    rng = np.random.default_rng(0x0BA7C4 + seed)
+    out = []
+    for i in range(n):
+        nx, ny = (int(rng.integers(16, 129)) * 2 for _ in range(2))
+        nz = int(rng.integers(16, 49)) * 2
+        npx = int(rng.integers(8, 65)) * 8
+        spp = max(4, int(round(2.0 ** rng.uniform(4, 10) * spp_scale / 4)) * 4)
+        nq = int(rng.integers(1, 17))
+        ext = (nx * 100.0, ny * 100.0, 1000.0 + nz * 100.0)
+        s = Synth(0, seed=seed * 1000 + i, cloud_def=(nx, ny, nz), cloud_upper=ext, nquads=nq)
+        tilt = math.radians(60.0)
+        pos = (ext[0] / 2, -2000.0, 1000.0 + nz * 100.0 + 1800.0)
+        d = map_preset(0, width=npx, height=npx, spp=spp, pos=pos,
+                       tgt=(pos[0], pos[1] + math.cos(tilt), pos[2] - math.sin(tilt)),
+                       solver=F.SOLVER_WAVEFRONT_FP32, seed=i)
+        out.append((s, d))
+    return out
+
+
+def render_batch(items, stream=None):
+    """Several maps in one persistent launch (htgx_render_batch).
+
+    items: [(Context, MapDesc, device accumulator address)]. Returns the
+    completion time of each map after the launch started, in ms."""
+    lib = F.load()
+    n = len(items)
+    arr = (F.BatchItem * n)()
+    keep = []
+    for i, (ctx, desc, accum) in enumerate(items):
+        arr[i].ctx = ctx.handle
+        arr[i].desc = C.pointer(desc)
+        arr[i].accum = C.c_void_p(accum)
+        keep.append(desc)
+    ms = np.zeros(n)
+    F.check("htgx_render_batch", lib.htgx_render_batch(
+        arr, n, C.c_void_p(stream or 0), ms.ctypes.data_as(F.c_double_p)))
+    return ms
+
+
 def threefry(ctr, key):
     """Threefry-4x64-20 blocks computed on the device."""
     lib = F.load()

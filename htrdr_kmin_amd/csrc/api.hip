@@ -628,6 +628,30 @@ htgx_render_accum(htgx_ctx* ctx, const htgx_map_desc* desc, htgx_mc_accum* accum
 }
 
 extern "C" int
+htgx_render_batch(const htgx_batch_item* items, uint32_t n, void* stream, double* scene_ms)
+{
+  if(!items || !n) return HTGX_BAD_ARG;
+  int dev = -1, res;
+  HIP_OK(hipGetDevice(&dev));
+  std::vector<DevMap> Ms(n);
+  std::vector<htgx_ctx*> ctxs(n);
+  std::vector<htgx_mc_accum*> accums(n);
+  for(uint32_t i = 0; i < n; ++i) {
+    const htgx_batch_item& it = items[i];
+    if(!it.ctx || !it.desc || !it.accum || it.ctx->device != dev) return HTGX_BAD_ARG;
+    for(uint32_t k = 0; k < i; ++k) if(items[k].ctx == it.ctx) return HTGX_BAD_ARG;
+    if(it.desc->sensor != items[0].desc->sensor
+    || it.desc->spectral_type != items[0].desc->spectral_type)
+      return HTGX_BAD_ARG;
+    if((res = htgx_map_setup(it.ctx, it.desc, &Ms[i])) != HTGX_OK) return res;
+    ctxs[i] = it.ctx;
+    accums[i] = it.accum;
+  }
+  return htgx_render_wavefront_batch(ctxs.data(), Ms.data(), accums.data(), n,
+                                     (hipStream_t)stream, scene_ms);
+}
+
+extern "C" int
 htgx_set_counting(htgx_ctx* ctx, int enable)
 {
   if(!ctx) return HTGX_BAD_ARG;

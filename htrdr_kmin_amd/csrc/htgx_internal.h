@@ -47,6 +47,11 @@ int htgx_ctx_init(htgx_ctx* ctx, const htgx_scene_desc& scene);
 int htgx_map_setup(htgx_ctx* ctx, const htgx_map_desc* desc, htgx::DevMap* map);
 int htgx_render_wavefront(htgx_ctx* ctx, const htgx::DevMap& map, htgx_mc_accum* accum,
                           hipStream_t stream);
+/* Several maps of distinct contexts (same device, same sensor and spectral
+ * type) in one persistent FP32 launch; synchronous. scene_ms[i] (optional):
+ * completion time of map i after the launch started. */
+int htgx_render_wavefront_batch(htgx_ctx* const* ctxs, const htgx::DevMap* Ms, htgx_mc_accum* const* accums,
+                                uint32_t n, hipStream_t stream, double* scene_ms);
 int htgx_debug_realisations_fp32(htgx_ctx* ctx, const htgx_map_desc* desc,
                                  const uint64_t* pix, const uint64_t* real, uint32_t n,
                                  double* weights);

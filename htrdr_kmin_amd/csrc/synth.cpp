@@ -308,6 +308,10 @@ htgx_synth_create(const htgx_synth_params* p, htgx_synth** out)
         const double lmin = std::log(p->lw_ka_min), lmax = std::log(p->lw_ka_max);
         ka0 = std::exp(lmin + (lmax - lmin) * rng.canonical());
         w[q] = -std::log(1.0 - rng.canonical()); /* Dirichlet(1) */
+      } else if(nq > 1) {
+        /* several shortwave k-points: ka0 log-uniform within a decade */
+        ka0 = p->ka0 * std::pow(10.0, 2.0 * rng.canonical() - 1.0);
+        w[q] = -std::log(1.0 - rng.canonical());
       } else {
         w[q] = 1.0;
       }

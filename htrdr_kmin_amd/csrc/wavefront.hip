@@ -29,7 +29,10 @@
  * the block index and the seed as counter; each block gives two FP32
  * canonicals and every draw site computes its own block.
  */
-#pragma clang fp contract(fast)
+/* Multiply-adds fuse within a source expression only: the arithmetic, hence
+ * every result, does not depend on the kernel the code is inlined into (the
+ * single-map and batched launches give bit-identical maps) */
+#pragma clang fp contract(on)
 
 #include "htgx_internal.h"
 
@@ -112,6 +115,12 @@ struct Launch {
   unsigned int* counters; /* [kXcds] next chunk of each XCD range */
   htgx_mc_accum* partials; /* [ntiles*nsub*64] */
   unsigned long long* stats; /* event counters */
+  /* batched launches only (null otherwise): set once the chunk queues are
+   * empty; chunks completed, out of nvalid; wall clock of the last one */
+  unsigned int* exhausted;
+  unsigned int* done_count;
+  unsigned long long* done_tick;
+  uint32_t nvalid;
 };
 
 enum { LS_SEG = 0, LS_COLL = 1, LS_EVENT = 2, LS_DONE = 3 };
@@ -874,11 +883,9 @@ __device__ __forceinline__ uint64_t wave_sum64(uint64_t v)
   return v;
 }
 
-/* 3 waves per SIMD: the register cap (168) spills a little but hides more
- * latency than 2 waves without spills (measured 14% faster at 1024^2) */
+/* The persistent loop of one wavefront over the chunks of one map */
 template<int SENSOR, int SPECTRAL, bool COUNT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
-wavefront_kernel(SceneF S, DevMap M, Launch P)
+__device__ __forceinline__ void run_scene(const SceneF& S, const DevMap& M, const Launch& P)
 {
   /* canonicals per event: 8, plus 2 for the sun of a SW flux map */
   constexpr int NP = (SENSOR == HTGX_SENSOR_RECTANGLE && SPECTRAL == HTGX_SPECTRAL_SW) ? 5 : 4;
@@ -995,6 +1002,7 @@ wavefront_kernel(SceneF S, DevMap M, Launch P)
           uint32_t c;
           if(!fetch_chunk(P, xcd, &c)) {
             L.lstate = LS_DONE;
+            if(P.exhausted) atomicExch(P.exhausted, 1u);
           } else {
             LC(chunk) = c;
             const uint32_t m = c & 63u, rest = c >> 6;
@@ -1053,6 +1061,8 @@ wavefront_kernel(SceneF S, DevMap M, Launch P)
             a.tsum = LC(tsum); a.tsum2 = LC(tsum2); a.n = LC(n); a.nfail = LC(nfail);
             P.partials[LC(chunk)] = a;
             LC(stage) = ST_FETCH;
+            if(P.done_count && atomicAdd(P.done_count, 1u) + 1u == P.nvalid)
+              *P.done_tick = wall_clock64();
           }
         }
         WF_TICK(7)
@@ -1104,6 +1114,46 @@ wavefront_kernel(SceneF S, DevMap M, Launch P)
       atomicAdd(P.stats + 7, (unsigned long long)g);
     }
   }
+}
+
+/* 3 waves per SIMD: the register cap (168) spills a little but hides more
+ * latency than 2 waves without spills (measured 14% faster at 1024^2) */
+template<int SENSOR, int SPECTRAL, bool COUNT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+wavefront_kernel(SceneF S, DevMap M, Launch P)
+{
+  run_scene<SENSOR, SPECTRAL, COUNT>(S, M, P);
+}
+
+/* One map of a batched launch */
+struct BatchScene {
+  SceneF S;
+  DevMap M;
+  Launch P;
+};
+
+/* Several maps of distinct scenes in one persistent launch. Every wavefront
+ * walks the scenes in table order (largest first): it drains a scene's chunk
+ * queues with the per-scene loop, then moves on, so the tails of the scenes
+ * overlap instead of idling the device between launches. The scene is
+ * uniform over a wavefront, so its constants stay scalar. */
+template<int SENSOR, int SPECTRAL>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+wavefront_batch_kernel(const BatchScene* __restrict__ scenes, uint32_t nscenes)
+{
+  for(uint32_t s = 0; s < nscenes; ++s) {
+    const BatchScene& B = scenes[s];
+    const unsigned int ex = __hip_atomic_load(B.P.exhausted, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+    if(__builtin_amdgcn_readfirstlane(ex)) continue;
+    run_scene<SENSOR, SPECTRAL, false>(B.S, B.M, B.P);
+  }
+}
+
+__global__ void
+tick_kernel(unsigned long long* out)
+{
+  *out = wall_clock64();
 }
 
 /* Fold the chunks of each pixel, in chunk order, into its accumulator */
@@ -1352,37 +1402,47 @@ wavefront_build(htgx_ctx* ctx, htgx_wavefront_state* st)
   return HTGX_OK;
 }
 
-int htgx_render_wavefront(htgx_ctx* ctx, const DevMap& M, htgx_mc_accum* accum, hipStream_t stream)
+/* Launch parameters of one map; sizes the chunk partials of the context */
+static int
+setup_launch(htgx_ctx* ctx, const DevMap& M, Launch* P)
 {
-  int res;
-  if((res = wavefront_prepare(ctx)) != HTGX_OK) return res;
   htgx_wavefront_state* st = ctx->wf;
-  Launch P;
-  std::memset(&P, 0, sizeof(P));
-  P.width = M.width;
-  P.height = M.height;
-  P.ntx = (M.width + 7) / 8;
-  P.ntiles = P.ntx * ((M.height + 7) / 8);
+  std::memset(P, 0, sizeof(*P));
+  P->width = M.width;
+  P->height = M.height;
+  P->ntx = (M.width + 7) / 8;
+  P->ntiles = P->ntx * ((M.height + 7) / 8);
   /* chunk size: the largest divisor of spp not above K */
   uint32_t K = std::min(st->K, M.spp);
   while(M.spp % K) --K;
-  P.K = K;
-  P.nsub = M.spp / K;
-  P.xcd_tiles = (P.ntiles + kXcds - 1) / kXcds;
-  P.spp_offset = M.spp_offset;
-  P.ev_thresh = st->ev_thresh;
-  P.max_steps = 1u << 20;
-  const uint64_t nchunks = (uint64_t)P.ntiles * P.nsub * 64u;
+  P->K = K;
+  P->nsub = M.spp / K;
+  P->xcd_tiles = (P->ntiles + kXcds - 1) / kXcds;
+  P->spp_offset = M.spp_offset;
+  P->ev_thresh = st->ev_thresh;
+  P->max_steps = 1u << 20;
+  const uint64_t nchunks = (uint64_t)P->ntiles * P->nsub * 64u;
   if(nchunks > 0xFFFFFFFFull) return HTGX_BAD_ARG;
+  P->nvalid = (uint32_t)((uint64_t)M.width * M.height * P->nsub);
   const size_t need = nchunks * sizeof(htgx_mc_accum);
   if(need > st->partials_cap) {
     if(st->d_partials) { HIP_OK(hipFree(st->d_partials)); st->d_partials = nullptr; }
     HIP_OK(hipMalloc(&st->d_partials, need));
     st->partials_cap = need;
   }
-  P.counters = st->d_counters;
-  P.partials = (htgx_mc_accum*)st->d_partials;
-  P.stats = ctx->d_counters;
+  P->counters = st->d_counters;
+  P->partials = (htgx_mc_accum*)st->d_partials;
+  P->stats = ctx->d_counters;
+  return HTGX_OK;
+}
+
+int htgx_render_wavefront(htgx_ctx* ctx, const DevMap& M, htgx_mc_accum* accum, hipStream_t stream)
+{
+  int res;
+  if((res = wavefront_prepare(ctx)) != HTGX_OK) return res;
+  htgx_wavefront_state* st = ctx->wf;
+  Launch P;
+  if((res = setup_launch(ctx, M, &P)) != HTGX_OK) return res;
   zero_u32_kernel<<<1, 64, 0, stream>>>(st->d_counters, kXcds);
   HIP_OK(hipGetLastError());
   if(ctx->counting) {
@@ -1397,6 +1457,92 @@ int htgx_render_wavefront(htgx_ctx* ctx, const DevMap& M, htgx_mc_accum* accum, 
     (const htgx_mc_accum*)st->d_partials, M.width, M.height, P.ntx, P.nsub, accum);
   HIP_OK(hipGetLastError());
   ctx->last_nrealisations = (uint64_t)npix * M.spp;
+  return HTGX_OK;
+}
+
+/* Device buffers of one batched call, freed on every return path */
+struct BatchBuffers {
+  BatchScene* table = nullptr;
+  unsigned long long* ticks = nullptr;  /* [n] done ticks, then the start tick */
+  unsigned int* flags = nullptr;        /* [n] exhausted, then [n] done counts */
+  ~BatchBuffers()
+  {
+    if(table) (void)hipFree(table);
+    if(ticks) (void)hipFree(ticks);
+    if(flags) (void)hipFree(flags);
+  }
+};
+
+int htgx_render_wavefront_batch(htgx_ctx* const* ctxs, const DevMap* Ms, htgx_mc_accum* const* accums,
+                                uint32_t n, hipStream_t stream, double* scene_ms)
+{
+  int res;
+  uint32_t grid = 0;
+  std::vector<BatchScene> table(n);
+  std::vector<uint32_t> order(n);
+  for(uint32_t i = 0; i < n; ++i) {
+    if((res = wavefront_prepare(ctxs[i])) != HTGX_OK) return res;
+    grid = std::max(grid, ctxs[i]->wf->grid);
+    order[i] = i;
+  }
+  /* largest maps first: the small ones fill the tail */
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+    return (uint64_t)Ms[a].width * Ms[a].height * Ms[a].spp
+         > (uint64_t)Ms[b].width * Ms[b].height * Ms[b].spp;
+  });
+  BatchBuffers B;
+  HIP_OK(hipMalloc(&B.table, n * sizeof(BatchScene)));
+  HIP_OK(hipMalloc(&B.ticks, (n + 1) * sizeof(unsigned long long)));
+  HIP_OK(hipMalloc(&B.flags, 2 * n * sizeof(unsigned int)));
+  for(uint32_t k = 0; k < n; ++k) {
+    const uint32_t i = order[k];
+    BatchScene& e = table[k];
+    e.S = ctxs[i]->wf->S;
+    e.M = Ms[i];
+    if((res = setup_launch(ctxs[i], Ms[i], &e.P)) != HTGX_OK) return res;
+    e.P.stats = nullptr;
+    e.P.exhausted = B.flags + k;
+    e.P.done_count = B.flags + n + k;
+    e.P.done_tick = B.ticks + k;
+    zero_u32_kernel<<<1, 64, 0, stream>>>(e.P.counters, kXcds);
+    HIP_OK(hipGetLastError());
+  }
+  zero_u32_kernel<<<(2 * n + 255) / 256, 256, 0, stream>>>(B.flags, 2 * n);
+  zero_u64_kernel<<<(n + 1 + 255) / 256, 256, 0, stream>>>(B.ticks, n + 1);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(B.table, table.data(), n * sizeof(BatchScene), hipMemcpyHostToDevice, stream));
+  tick_kernel<<<1, 1, 0, stream>>>(B.ticks + n);
+  HIP_OK(hipGetLastError());
+  const DevMap& M0 = Ms[0];
+  if(M0.sensor == HTGX_SENSOR_CAMERA) {
+    if(M0.spectral_type == HTGX_SPECTRAL_LW)
+      wavefront_batch_kernel<HTGX_SENSOR_CAMERA, HTGX_SPECTRAL_LW><<<grid, 256, 0, stream>>>(B.table, n);
+    else
+      wavefront_batch_kernel<HTGX_SENSOR_CAMERA, HTGX_SPECTRAL_SW><<<grid, 256, 0, stream>>>(B.table, n);
+  } else {
+    if(M0.spectral_type == HTGX_SPECTRAL_LW)
+      wavefront_batch_kernel<HTGX_SENSOR_RECTANGLE, HTGX_SPECTRAL_LW><<<grid, 256, 0, stream>>>(B.table, n);
+    else
+      wavefront_batch_kernel<HTGX_SENSOR_RECTANGLE, HTGX_SPECTRAL_SW><<<grid, 256, 0, stream>>>(B.table, n);
+  }
+  HIP_OK(hipGetLastError());
+  for(uint32_t k = 0; k < n; ++k) {
+    const uint32_t i = order[k];
+    const Launch& P = table[k].P;
+    const uint32_t npix = Ms[i].width * Ms[i].height;
+    wf_merge_kernel<<<(npix + 255) / 256, 256, 0, stream>>>(
+      P.partials, Ms[i].width, Ms[i].height, P.ntx, P.nsub, accums[i]);
+    HIP_OK(hipGetLastError());
+    ctxs[i]->last_nrealisations = (uint64_t)npix * Ms[i].spp;
+  }
+  /* the table, flags and ticks are freed on return: wait for the launch */
+  HIP_OK(hipStreamSynchronize(stream));
+  if(scene_ms) {
+    std::vector<unsigned long long> t(n + 1);
+    HIP_OK(hipMemcpy(t.data(), B.ticks, (n + 1) * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    for(uint32_t k = 0; k < n; ++k)
+      scene_ms[order[k]] = (double)(t[k] - t[n]) * ctxs[order[k]]->us_per_tick * 1e-3;
+  }
   return HTGX_OK;
 }
 

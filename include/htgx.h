@@ -264,6 +264,28 @@ int htgx_render_accum
    struct htgx_mc_accum* accum,
    void* stream);
 
+/* Batched rendering. No counterpart in the reference, which draws one map
+ * per process: several maps of distinct scenes in one persistent FP32
+ * launch, so that maps of uneven sizes share the device and their tails
+ * overlap. Each item renders exactly what htgx_render_accum renders for it
+ * (same random streams, same chunk order: bit-identical accumulators). */
+struct htgx_batch_item {
+  struct htgx_ctx* ctx;              /* distinct contexts on the current device */
+  const struct htgx_map_desc* desc;  /* same sensor and spectral type for all; the
+                                        solver field is ignored (FP32 wavefront) */
+  struct htgx_mc_accum* accum;       /* DEVICE accumulators of the map, as in
+                                        htgx_render_accum */
+};
+
+/* Render n items in one launch on `stream' (a hipStream_t, NULL = default);
+ * returns once they are done. scene_ms (optional, HOST array of n): time from
+ * the start of the launch to the completion of each item, in ms. */
+int htgx_render_batch
+  (const struct htgx_batch_item* items,
+   uint32_t n,
+   void* stream,
+   double* scene_ms);
+
 /* Fold device accumulators into reference pixel structs (host memory). For
  * a longwave camera, the brightness temperature and its error are computed as
  * in draw_pixel_xwave (src/atmosphere/htrdr_atmosphere_draw_map.c:423-435). */

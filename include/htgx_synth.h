@@ -14,7 +14,8 @@
  * ka = ka0*exp(-z/H), ks = ks0*exp(-z/H) at the layer middle,
  * T = T_ground - lapse*z. Longwave bands: per (band, k-point), ka0 is
  * log-uniform in [lw_ka_min, lw_ka_max] and the k-point weights are
- * Dirichlet(1).
+ * Dirichlet(1). Shortwave bands with several k-points: ka0 is log-uniform
+ * in [ka0/10, 10 ka0] per k-point, weights Dirichlet(1).
  *
  * Mesh ground (ground_type = HTGX_GROUND_MESH): a heightfield of hf_res^2
  * vertices over the cloud grid's XY extent, fractional Brownian motion of
