@@ -144,6 +144,9 @@ SIGNATURES = {
     "htgx_debug_threefry2x32": (C.c_int, [c_uint32_p, c_uint32_p, C.c_uint32, c_uint32_p]),
     "htgx_debug_sample_dirs": (C.c_int, [C.c_void_p, C.c_int, C.c_double, c_double_p,
                                          c_double_p, C.c_uint32, c_double_p]),
+    "htgx_debug_mesh_trace": (C.c_int, [C.c_void_p, C.POINTER(SceneDesc), c_double_p,
+                                        C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.c_uint32,
+                                        C.c_int, c_double_p, C.POINTER(C.c_int64), c_uint32_p]),
     "htgx_debug_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "htgx_synth_params_preset": (C.c_int, [C.c_int, C.POINTER(SynthParams)]),
     "htgx_synth_map_preset": (C.c_int, [C.c_int, C.POINTER(MapDesc)]),

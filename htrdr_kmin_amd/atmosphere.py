@@ -248,6 +248,27 @@ def threefry2x32(ctr, key):
     return out
 
 
+def mesh_trace(scene_ptr, rays, prev=None, prev_edge=None, any_hit=False, ctx=None):
+    """Ground-mesh hits of rays [n][8] (org, dir, range) through the BVH: on
+    the host when ctx is None, else on the GPU of `ctx' (which must hold the
+    same scene). Returns (t, prim, nodes visited)."""
+    lib = F.load()
+    rays = np.ascontiguousarray(rays, np.float64).reshape(-1, 8)
+    n = rays.shape[0]
+    prev = np.full(n, -1, np.int64) if prev is None else np.ascontiguousarray(prev, np.int64)
+    prev_edge = (np.zeros(n, np.int32) if prev_edge is None
+                 else np.ascontiguousarray(prev_edge, np.int32))
+    t = np.zeros(n, np.float64)
+    prim = np.zeros(n, np.int64)
+    nv = np.zeros(n, np.uint32)
+    F.check("htgx_debug_mesh_trace", lib.htgx_debug_mesh_trace(
+        ctx.handle if ctx is not None else None, scene_ptr, rays.ctypes.data_as(F.c_double_p),
+        prev.ctypes.data_as(C.POINTER(C.c_int64)), prev_edge.ctypes.data_as(C.POINTER(C.c_int32)),
+        n, int(any_hit), t.ctypes.data_as(F.c_double_p), prim.ctypes.data_as(C.POINTER(C.c_int64)),
+        nv.ctypes.data_as(F.c_uint32_p)))
+    return t, prim, nv
+
+
 def finalize(desc, accum):
     """Reference pixel structs from a host copy of the device accumulators."""
     lib = F.load()

@@ -233,6 +233,31 @@ sample_dirs_kernel(DevScene S, int kind, double param, double dx, double dy, dou
   }
 }
 
+/* One ground-mesh ray cast as the FP64 solver casts it (ground_trace):
+ * ray[8] = org, dir, range; the ray is converted to FP32 */
+HTGX_HD void mesh_trace_ray(const BvhNode* nodes, uint32_t nnodes, const BvhTri* tris,
+                            const double* ray, int32_t prev, bool prev_edge, bool any,
+                            double* t, int32_t* tri, uint32_t* nvisited)
+{
+  const float o[3] = {(float)ray[0], (float)ray[1], (float)ray[2]};
+  const float d[3] = {(float)ray[3], (float)ray[4], (float)ray[5]};
+  const float r1 = ray[7] > (double)FLT_MAX ? INFINITY : (float)ray[7];
+  const MeshHit m = mesh_trace(nodes, nnodes, tris, o, d, (float)ray[6], r1, prev, prev_edge, any);
+  *t = m.tri >= 0 ? (double)m.t : INFINITY;
+  *tri = m.tri;
+  *nvisited = m.nodes;
+}
+
+__global__ void
+mesh_trace_kernel(DevScene S, const double* rays, const int32_t* prev, const int32_t* prev_edge,
+                  uint32_t n, int any, double* t, int32_t* tri, uint32_t* nvisited)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  mesh_trace_ray(S.bvh, S.nbvh, S.tris, rays + 8 * (size_t)i, prev[i], prev_edge[i] != 0,
+                 any != 0, t + i, tri + i, nvisited + i);
+}
+
 /* Plain streaming copy, the reference traffic of the HBM counters */
 __global__ void __launch_bounds__(256)
 copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n)
@@ -897,6 +922,50 @@ htgx_debug_sample_dirs(htgx_ctx* ctx, int kind, double param, const double dir_i
                                                dir_in[2], d_u.p, n, d_o.p);
   HIP_OK(hipGetLastError());
   return d_o.download(out, 3 * (size_t)n);
+}
+
+extern "C" int
+htgx_debug_mesh_trace(htgx_ctx* ctx, const htgx_scene_desc* scene, const double* rays,
+                      const int64_t* prev, const int32_t* prev_edge, uint32_t n, int any,
+                      double* t, int64_t* prim, uint32_t* nvisited)
+{
+  if(!scene || !rays || !prev || !prev_edge || !t || !prim || !nvisited || !n) return HTGX_BAD_ARG;
+  const htgx_ground_desc& g = scene->ground;
+  if(g.type != HTGX_GROUND_MESH) return HTGX_BAD_ARG;
+  if(ctx && ctx->S.ground_type != HTGX_GROUND_MESH) return HTGX_BAD_ARG;
+  BvhHost bvh;
+  int res;
+  if((res = bvh_build(g.verts, g.nverts, g.tris, g.tri_mtl, g.ntris, bvh)) != HTGX_OK) return res;
+  if(ctx && ctx->S.nbvh != bvh.nodes.size()) return HTGX_BAD_ARG; /* not this scene's mesh */
+  /* input triangle index <-> leaf order */
+  std::vector<int32_t> leaf_of(g.ntris);
+  for(uint32_t k = 0; k < (uint32_t)bvh.prim.size(); ++k) leaf_of[bvh.prim[k]] = (int32_t)k;
+  std::vector<int32_t> lprev(n), ledge(n), ltri(n);
+  for(uint32_t i = 0; i < n; ++i) {
+    if(prev[i] >= (int64_t)g.ntris) return HTGX_BAD_ARG;
+    lprev[i] = prev[i] < 0 ? -1 : leaf_of[prev[i]];
+    ledge[i] = prev_edge[i] != 0;
+  }
+  if(!ctx) {
+    for(uint32_t i = 0; i < n; ++i)
+      mesh_trace_ray(bvh.nodes.data(), (uint32_t)bvh.nodes.size(), bvh.tris.data(), rays + 8 * (size_t)i,
+                     lprev[i], ledge[i] != 0, any != 0, t + i, &ltri[i], nvisited + i);
+  } else {
+    DevArray<double> d_rays, d_t;
+    DevArray<int32_t> d_prev, d_edge, d_tri;
+    DevArray<uint32_t> d_nv;
+    if((res = d_rays.upload(rays, 8 * (size_t)n))) return res;
+    if((res = d_prev.upload(lprev.data(), n))) return res;
+    if((res = d_edge.upload(ledge.data(), n))) return res;
+    if((res = d_t.alloc(n)) || (res = d_tri.alloc(n)) || (res = d_nv.alloc(n))) return res;
+    mesh_trace_kernel<<<(n + 255) / 256, 256>>>(ctx->S, d_rays.p, d_prev.p, d_edge.p, n, any,
+                                                d_t.p, d_tri.p, d_nv.p);
+    HIP_OK(hipGetLastError());
+    if((res = d_t.download(t, n)) || (res = d_tri.download(ltri.data(), n))) return res;
+    if((res = d_nv.download(nvisited, n))) return res;
+  }
+  for(uint32_t i = 0; i < n; ++i) prim[i] = ltri[i] < 0 ? -1 : (int64_t)bvh.prim[ltri[i]];
+  return HTGX_OK;
 }
 
 extern "C" int

@@ -34,12 +34,20 @@ struct Box {
   }
 };
 
+/* Tree node of the build: box, children (inner) or triangle range (leaf) */
+struct TreeNode {
+  Box box;
+  int32_t child[2] = {-1, -1};
+  uint32_t tris = 0; /* leaf: first << 3 | count */
+};
+
 struct Builder {
   const float* verts;
   const uint32_t* tris;
   std::vector<Box> tbox;
   std::vector<float> cent; /* [ntris][3] */
   std::vector<uint32_t> order;
+  std::vector<TreeNode> tree;
   BvhHost* out;
 
   void emit_leaf(uint32_t node, uint32_t begin, uint32_t end, const uint32_t* tri_mtl)
@@ -60,25 +68,24 @@ struct Builder {
       out->prim.push_back(t);
       out->mtl.push_back(tri_mtl ? tri_mtl[t] : 0u);
     }
-    out->nodes[node].tris = (first << 3) | (end - begin);
+    tree[node].tris = (first << 3) | (end - begin);
   }
 
-  void build(uint32_t begin, uint32_t end, uint32_t depth, const uint32_t* tri_mtl)
+  uint32_t build(uint32_t begin, uint32_t end, uint32_t depth, const uint32_t* tri_mtl)
   {
-    const uint32_t node = (uint32_t)out->nodes.size();
-    out->nodes.push_back(BvhNode{});
+    const uint32_t node = (uint32_t)tree.size();
+    tree.push_back(TreeNode{});
     out->depth = std::max(out->depth, depth);
     Box bb, cb;
     for(uint32_t i = begin; i < end; ++i) {
       bb.grow(tbox[order[i]]);
       cb.grow(&cent[3 * (size_t)order[i]]);
     }
-    for(int a = 0; a < 3; ++a) { out->nodes[node].lo[a] = bb.lo[a]; out->nodes[node].hi[a] = bb.hi[a]; }
+    tree[node].box = bb;
     const uint32_t n = end - begin;
     if(n <= kLeafMax) {
       emit_leaf(node, begin, end, tri_mtl);
-      out->nodes[node].skip = (uint32_t)out->nodes.size();
-      return;
+      return node;
     }
     /* binned SAH on the widest centroid axis */
     int axis = 0;
@@ -135,9 +142,40 @@ struct Builder {
                          return cx < cy || (cx == cy && x < y);
                        });
     }
-    build(begin, mid, depth + 1, tri_mtl);
-    build(mid, end, depth + 1, tri_mtl);
-    out->nodes[node].skip = (uint32_t)out->nodes.size();
+    /* child 0 holds the lower centroids along `axis' */
+    const uint32_t l = build(begin, mid, depth + 1, tri_mtl);
+    const uint32_t r = build(mid, end, depth + 1, tri_mtl);
+    tree[node].child[0] = (int32_t)l;
+    tree[node].child[1] = (int32_t)r;
+    return node;
+  }
+
+  /* Depth-first layout with skip links, appended to out->nodes (iterative:
+   * the tree can be deep) */
+  void layout()
+  {
+    struct Item { uint32_t node; uint32_t slot; bool done; };
+    std::vector<Item> stack;
+    stack.push_back({0u, 0u, false});
+    while(!stack.empty()) {
+      const Item it = stack.back();
+      stack.pop_back();
+      if(it.done) {
+        out->nodes[it.slot].skip = (uint32_t)out->nodes.size(); /* first node after the subtree */
+        continue;
+      }
+      const TreeNode& T = tree[it.node];
+      const uint32_t slot = (uint32_t)out->nodes.size();
+      out->nodes.push_back(BvhNode{});
+      BvhNode& N = out->nodes[slot];
+      for(int a = 0; a < 3; ++a) { N.lo[a] = T.box.lo[a]; N.hi[a] = T.box.hi[a]; }
+      N.tris = T.tris;
+      stack.push_back({it.node, slot, true});
+      if(!T.tris) {
+        stack.push_back({(uint32_t)T.child[1], 0u, false});
+        stack.push_back({(uint32_t)T.child[0], 0u, false});
+      }
+    }
   }
 };
 
@@ -170,9 +208,11 @@ int bvh_build(const float* verts, uint32_t nverts, const uint32_t* tris, const u
   out.tris.clear();
   out.prim.clear();
   out.mtl.clear();
-  out.nodes.reserve(2 * (size_t)ntris / 2 + 1);
   out.tris.reserve(ntris);
+  b.tree.reserve(ntris);
   b.build(0, ntris, 0, tri_mtl);
+  out.nodes.reserve(b.tree.size());
+  b.layout();
   for(int a = 0; a < 3; ++a) { out.lo[a] = scene.lo[a]; out.hi[a] = scene.hi[a]; }
   return HTGX_OK;
 }

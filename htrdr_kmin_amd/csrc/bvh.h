@@ -2,11 +2,16 @@
  * Ground triangle mesh and its bounding volume hierarchy, built on the host.
  *
  * Replaces the Star-3D / Embree scene behind htrdr_geometry_trace_ray
- * (src/core/htrdr_geometry.c:686-718). The BVH is a binary tree in
+ * (src/core/htrdr_geometry.c:686-718). The BVH is a binary tree laid out in
  * depth-first order with skip links, so that the device walks it without a
  * stack (a stack would cost the wavefront kernel a few dozen registers per
  * lane): a node's first child is the next node, and `skip' is the index of
  * the node that follows its subtree. Leaves hold up to 4 triangles.
+ *
+ * Children are listed lower side first along the split axis. Laying the tree
+ * out once per octant of ray directions (near child first) was measured on a
+ * 2M-triangle terrain: 8 % fewer nodes visited, but 8 times the node bytes
+ * competing for the caches, and no faster.
  *
  * Triangles are stored in leaf order as (v0, e1 = v1 - v0, e2 = v2 - v0) in
  * FP32: the reference traces the ground in FP32 too (the ray is converted at
@@ -36,7 +41,7 @@ struct BvhTri {       /* 36 bytes */
 static_assert(sizeof(BvhTri) == 36, "BvhTri is 36 bytes");
 
 struct BvhHost {
-  std::vector<BvhNode> nodes;
+  std::vector<BvhNode> nodes;   /* depth-first, skip links */
   std::vector<BvhTri> tris;     /* leaf order */
   std::vector<uint32_t> prim;   /* leaf order -> input triangle index */
   std::vector<uint32_t> mtl;    /* leaf order -> material */

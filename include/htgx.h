@@ -379,6 +379,20 @@ int htgx_debug_sample_dirs
    int kind, double param, const double dir_in[3],
    const double* u, uint32_t n, double* out);
 
+/* Closest (or, with `any', first accepted) hits of n rays on the ground mesh
+ * of `scene' through its BVH, with the self-hit filter of
+ * src/core/htrdr_geometry.c:96-150 (replaces htrdr_geometry_trace_ray,
+ * :686-718). rays = [n][8] org, dir, range; prev[i] = input index of the
+ * triangle the ray leaves (-1: none) and prev_edge[i] whether that hit was on
+ * an edge. With ctx NULL the BVH is built and walked on the host by the same
+ * code; else ctx must hold this scene and its device BVH is walked on the
+ * GPU. Outputs: t (INFINITY: no hit), prim (input triangle index or -1) and
+ * the number of BVH nodes visited. HOST arrays. */
+int htgx_debug_mesh_trace
+  (struct htgx_ctx* ctx, const struct htgx_scene_desc* scene,
+   const double* rays, const int64_t* prev, const int32_t* prev_edge,
+   uint32_t n, int any, double* t, int64_t* prim, uint32_t* nvisited);
+
 /* Copy nbytes (multiple of 16, 16-byte aligned DEVICE pointers) with a plain
  * streaming kernel on `stream': the known-size reference used to calibrate
  * the HBM traffic counters of the profiler. Asynchronous. */

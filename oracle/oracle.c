@@ -1249,6 +1249,21 @@ ground_trace(const struct oracle_sky* s, const double o[3], const double d[3],
   return h;
 }
 
+int
+oracle_ground_trace(const struct oracle_sky* s, const double ray[8], int64_t prev_prim,
+                    int prev_edge, double* t, int64_t* prim, int* edge)
+{
+  struct ghit prev = GHIT_NONE;
+  struct ghit h;
+  prev.prim = prev_prim;
+  prev.edge = prev_edge;
+  h = ground_trace(s, ray, ray + 3, ray + 6, prev_prim >= 0 ? &prev : NULL);
+  *t = h.hit ? h.t : INFINITY;
+  *prim = h.hit ? h.prim : -1;
+  *edge = h.edge;
+  return h.hit;
+}
+
 /*******************************************************************************
  * Sun, phase functions, BRDF
  ******************************************************************************/

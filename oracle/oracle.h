@@ -134,6 +134,14 @@ int oracle_sample_dirs
    const double* u, uint32_t n, double* out);
 double oracle_phase_eval(int kind, double param, double cos_theta);
 
+/* Closest ground hit of ray (org, dir, range) leaving triangle prev_prim
+ * (-1: none; prev_edge: that hit was on an edge), by brute force over every
+ * mesh triangle (htrdr_atmosphere_ground_trace_ray, ..._ground.c:170-217).
+ * Returns whether it hits; t = INFINITY and prim = -1 when not. */
+int oracle_ground_trace
+  (const struct oracle_sky* sky, const double ray[8], int64_t prev_prim,
+   int prev_edge, double* t, int64_t* prim, int* edge);
+
 /* Per-pixel estimate (src/core/htrdr_accum.h:48-68) */
 void oracle_accum_estimate
   (const struct htgx_mc_accum* acc, double* E, double* SE);

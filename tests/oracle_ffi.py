@@ -47,6 +47,8 @@ _SIG = {
                                      C.c_uint32, F.c_double_p]),
     "oracle_phase_eval": (C.c_double, [C.c_int, C.c_double, C.c_double]),
     "oracle_accum_estimate": (None, [C.c_void_p, F.c_double_p, F.c_double_p]),
+    "oracle_ground_trace": (C.c_int, [C.c_void_p, F.c_double_p, C.c_int64, C.c_int, F.c_double_p,
+                                      C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
 }
 
 _lib = None

@@ -150,3 +150,26 @@ def test_mesh_counters(hills):
     c = ctx.counters()
     ctx.set_counting(False)
     assert c["nbvh"] > c["nrealisations"] and c["ntri"] > 0
+
+
+def test_device_bvh_walk_matches_host_and_oracle():
+    """The device BVH walk gives the host walk's hits and node counts exactly, and the oracle's brute-force distances bit for bit."""
+    from test_host_bvh import continuation_rays, oracle_trace, random_rays
+    s, v, _, _ = terrain(res=129, relief=600.0, seed=5)
+    ctx = A.Context(s.scene_ptr)
+    sky = O.Sky(s.scene_ptr)
+    rng = np.random.default_rng(5)
+    rays = random_rays(rng, v.min(0), v.max(0), 4000)
+    for any_hit in (False, True):
+        t_h, p_h, n_h = A.mesh_trace(s.scene_ptr, rays, any_hit=any_hit)
+        t_d, p_d, n_d = A.mesh_trace(s.scene_ptr, rays, any_hit=any_hit, ctx=ctx)
+        assert np.array_equal(t_d, t_h) and np.array_equal(p_d, p_h) and np.array_equal(n_d, n_h)
+    none = np.full(rays.shape[0], -1)
+    t_o, p_o, e_o = oracle_trace(sky, rays, none, np.zeros_like(none))
+    assert np.array_equal(A.mesh_trace(s.scene_ptr, rays, ctx=ctx)[0], t_o)
+    r2, p2, e2 = continuation_rays(rng, rays, t_o, p_o, e_o)
+    t2, q2, _ = A.mesh_trace(s.scene_ptr, r2, p2, e2, ctx=ctx)
+    t2_o, q2_o, _ = oracle_trace(sky, r2, p2, e2)
+    assert np.array_equal(t2, t2_o) and (q2 != q2_o).mean() < 1e-3
+    sky.close()
+    ctx.close()
