@@ -55,10 +55,18 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--seed", type=int, default=0)
+    p.add_argument("--autotune-ms", type=float, default=6000.0,
+                   help="device-time budget of the launch-parameter autotuner run before the "
+                        "warm-up (FP32 solver, 8x for --config 3; 0: default parameters)")
     p.add_argument("--batch-scale", type=float, default=1.0,
                    help="--config 3: scale of the realisations per pixel of every map")
     p.add_argument("--batch-sequential", action="store_true",
                    help="--config 3: also time the maps as back-to-back single-map launches")
+    p.add_argument("--batch-steps", type=int, default=1, choices=(1, 2, 4),
+                   help="--config 3: steps per run, each all/half/a quarter of every map's "
+                        "realisations per pixel (one step is fastest: no barrier between steps)")
+    p.add_argument("--batch-no-graph", action="store_true",
+                   help="--config 3: launch the steps directly instead of replaying a HIP graph")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                    help="only run 5 plain device copies of MIB MiB (profiler calibration)")
     return p.parse_args()
@@ -131,9 +139,11 @@ def workload_name(synth, desc, S, steps):
 
 def bench_batch(args):
     """--config 3: the uneven batched workload (64 seeded scenes of uneven
-    grids, image sizes, realisations per pixel and k-points) rendered in one
-    persistent launch (htgx_render_batch), beside the same maps rendered by
-    back-to-back single-map launches."""
+    grids, image sizes, realisations per pixel and k-points). A batch plan
+    renders all of them in --batch-steps steps; each step is one persistent
+    launch over the 64 maps (with its queue resets, clock ticks and merges),
+    recorded once as a HIP graph and replayed. Optionally beside the same maps
+    rendered by back-to-back single-map launches."""
     import numpy as np
     from htrdr_kmin_amd import _ffi as F
     from htrdr_kmin_amd import atmosphere as A
@@ -143,36 +153,68 @@ def bench_batch(args):
     bufs = [H.DeviceBuffer(d.width * d.height * A.ACCUM_DTYPE.itemsize) for _, d in items]
     stream = H.Stream()
     total = sum(d.width * d.height * d.spp for _, d in items)
-    # counting warm-up per map (4 realisations per pixel): byte rates
-    bytes_total = 0.0
-    for c, (_, d), b in zip(ctxs, items, bufs):
+    nsteps = args.batch_steps
+
+    def copy_desc(d, **kw):
         w = A.map_preset(0)
         for f, _ in w._fields_:
             setattr(w, f, getattr(d, f))
-        w.spp = 4
+        for k, v in kw.items():
+            setattr(w, k, v)
+        return w
+
+    # counting warm-up per map (4 realisations per pixel): byte rates
+    bytes_total = 0.0
+    for c, (_, d), b in zip(ctxs, items, bufs):
         b.zero()
         c.set_counting(True)
-        c.render_accum(w, b.address, stream.h.value)
+        c.render_accum(copy_desc(d, spp=4), b.address, stream.h.value)
         stream.synchronize()
         k = c.counters()
         c.set_counting(False)
         rate = (8.0 * k["nleaf"] + 8.0 * k["ncoll"] + 32.0 * k["nbvh"] + 36.0 * k["ntri"]) / max(
             k["nrealisations"], 1)
-        bytes_total += rate * d.width * d.height * d.spp + 64.0 * d.width * d.height
-    print(f"[bench] batch: {len(items)} maps, {total / 1e9:.3f} G realisations", file=sys.stderr, flush=True)
+        bytes_total += rate * d.width * d.height * d.spp + 64.0 * d.width * d.height * nsteps
+    print(f"[bench] batch: {len(items)} maps, {total / 1e9:.3f} G realisations, {nsteps} steps",
+          file=sys.stderr, flush=True)
+    assert all(d.spp % (4 * nsteps) == 0 for _, d in items)
+    step_descs = [copy_desc(d, spp=d.spp // nsteps) for _, d in items]
+    step_items = [(c, d, b.address) for c, d, b in zip(ctxs, step_descs, bufs)]
+    tuned = None
+    if args.autotune_ms > 0:
+        # trials: the whole batch at 1/8 of each map's realisations per pixel
+        budget = 8.0 * args.autotune_ms
+        params, rate, ntrials = A.batch_autotune(step_items, budget, spp_div=8)
+        tuned = {"params": params, "trials": ntrials, "budget_ms": budget,
+                 "trial_rate_mreal_s": round(rate / 1e6, 3)}
+        print(f"[bench] autotune: {params} after {ntrials} trials ({rate / 1e6:.2f} Mreal/s)",
+              file=sys.stderr, flush=True)
+    plan = A.BatchPlan(step_items, use_graph=not args.batch_no_graph)
+    # warm-up: one step of a plan of the smallest map
+    small = min(range(len(items)), key=lambda i: items[i][1].width * items[i][1].height)
+    warm = A.BatchPlan([(ctxs[small], copy_desc(items[small][1], spp=4), bufs[small].address)],
+                       use_graph=not args.batch_no_graph)
+    warm.step(stream.h.value)
+    stream.synchronize()
+    warm.close()
     for b in bufs:
         b.zero()
     e0, e1 = H.Event(), H.Event()
     H.synchronize()
     t0 = time.perf_counter()
     e0.record(stream)
-    scene_ms = A.render_batch([(c, d, b.address) for c, (_, d), b in zip(ctxs, items, bufs)],
-                              stream.h.value)
+    for i in range(nsteps):
+        plan.step(stream.h.value)
     e1.record(stream)
     e1.synchronize()
     batch_s = e0.elapsed_ms(e1) / 1e3
     wall = time.perf_counter() - t0
-    print(f"[bench] batch: one launch {batch_s:.2f} s", file=sys.stderr, flush=True)
+    scene_ms = plan.scene_ms()
+    plan.close()
+    n_done = sum(int(b.download(np.zeros(d.width * d.height, A.ACCUM_DTYPE))["n"].sum())
+                 for (_, d), b in zip(items, bufs))
+    assert n_done == total, (n_done, total)
+    print(f"[bench] batch: {nsteps} graph steps {batch_s:.2f} s", file=sys.stderr, flush=True)
     seq_s = None
     if args.batch_sequential:
         for b in bufs:
@@ -189,16 +231,19 @@ def bench_batch(args):
         "metric": METRIC,
         "value": round(total / batch_s / 1e6, 4),
         "unit": "Mrealisations/s",
-        "n_gpus": 1, "steps": 1, "warmup": 1,
-        "ms_per_step": round(batch_s * 1e3, 3),
+        "n_gpus": 1, "steps": nsteps, "warmup": 1,
+        "ms_per_step": round(batch_s * 1e3 / nsteps, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded scene generator, seed %d)" % args.seed,
         "config": {
             "workload": "uneven batch: 64 SW scenes, cloud grids 32^2x32 to 256^2x96, images 64^2 to "
-                        "512^2, 16-1024 realisations/px (x%g), 1-16 k-points, one persistent launch"
-                        % args.batch_scale,
+                        "512^2, 16-1024 realisations/px (x%g), 1-16 k-points, %d step(s) of one "
+                        "persistent launch each%s" % (args.batch_scale, nsteps,
+                                                     "" if args.batch_no_graph else
+                                                     ", replayed from a HIP graph"),
             "config_index": 3, "realisations": total, "wall_s": round(wall, 3),
+            "autotune": tuned,
         },
         "scene_completion_ms": [round(float(x), 1) for x in scene_ms],
         "back_to_back_launches_s": None if seq_s is None else round(seq_s, 3),
@@ -300,6 +345,18 @@ def main():
         return spp_req * npix * total_s / nreal, spp_req, float(np.mean(rel_var)), E, SE
 
     main_solver = F.SOLVER_WAVEFRONT_FP32 if args.solver == "wavefront" else F.SOLVER_PLAIN_FP64
+    tuned = None
+    if main_solver == F.SOLVER_WAVEFRONT_FP32 and args.autotune_ms > 0:
+        dt = A.map_preset(args.config)
+        for f, _ in dt._fields_:
+            setattr(dt, f, getattr(desc, f))
+        dt.solver = main_solver
+        dt.spp = S  # the launches of the run
+        params, rate, ntrials = ctx.autotune(dt, args.autotune_ms)
+        tuned = {"params": params, "trials": ntrials, "budget_ms": args.autotune_ms,
+                 "trial_rate_mreal_s": round(rate / 1e6, 3)}
+        print(f"[bench] autotune: {params} after {ntrials} trials ({rate / 1e6:.2f} Mreal/s)",
+              file=sys.stderr, flush=True)
     total_s, wall, h, counters = run(main_solver, args.steps, acc, True)
     real = npix * S * args.steps
     value = real / total_s / 1e6
@@ -338,6 +395,7 @@ def main():
             "config_index": args.config,
             "spp_per_step": S,
             "wall_s": round(wall, 3),
+            "autotune": tuned,
         },
         "time_to_0p5pct_s": round(t05_main, 3),
         "spp_required_0p5pct_q99": round(spp_req, 1),

@@ -91,6 +91,11 @@ class MapDesc(C.Structure):
                 ("seed", C.c_uint64), ("solver", C.c_int)]
 
 
+class WfParams(C.Structure):
+    _fields_ = [("ev_thresh", C.c_uint32), ("chunk", C.c_uint32), ("grid", C.c_uint32),
+                ("grid_max", C.c_uint32)]
+
+
 class BatchItem(C.Structure):
     _fields_ = [("ctx", C.c_void_p), ("desc", C.POINTER(MapDesc)), ("accum", C.c_void_p)]
 
@@ -140,6 +145,18 @@ SIGNATURES = {
     "htgx_debug_realisations": (C.c_int, [C.c_void_p, C.POINTER(MapDesc), c_uint64_p,
                                           c_uint64_p, C.c_uint32, c_double_p]),
     "htgx_render_batch": (C.c_int, [C.POINTER(BatchItem), C.c_uint32, C.c_void_p, c_double_p]),
+    "htgx_get_wavefront_params": (C.c_int, [C.c_void_p, C.POINTER(WfParams)]),
+    "htgx_set_wavefront_params": (C.c_int, [C.c_void_p, C.POINTER(WfParams)]),
+    "htgx_autotune": (C.c_int, [C.c_void_p, C.POINTER(MapDesc), C.c_double, C.POINTER(WfParams),
+                                c_double_p, c_uint32_p]),
+    "htgx_batch_autotune": (C.c_int, [C.POINTER(BatchItem), C.c_uint32, C.c_uint32, C.c_double,
+                                      C.POINTER(WfParams), c_double_p, c_uint32_p]),
+    "htgx_batch_plan_create": (C.c_int, [C.POINTER(BatchItem), C.c_uint32, C.c_int,
+                                         C.POINTER(C.c_void_p)]),
+    "htgx_batch_plan_step": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "htgx_batch_plan_rewind": (C.c_int, [C.c_void_p]),
+    "htgx_batch_plan_scene_ms": (C.c_int, [C.c_void_p, c_double_p]),
+    "htgx_batch_plan_destroy": (C.c_int, [C.c_void_p]),
     "htgx_debug_threefry": (C.c_int, [c_uint64_p, c_uint64_p, C.c_uint32, c_uint64_p]),
     "htgx_debug_threefry2x32": (C.c_int, [c_uint32_p, c_uint32_p, C.c_uint32, c_uint32_p]),
     "htgx_debug_sample_dirs": (C.c_int, [C.c_void_p, C.c_int, C.c_double, c_double_p,

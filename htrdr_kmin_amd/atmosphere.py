@@ -119,6 +119,23 @@ class Context:
         F.check("htgx_render_accum", self._lib.htgx_render_accum(
             self._h, C.byref(desc), C.c_void_p(accum_ptr), C.c_void_p(stream or 0)))
 
+    def wavefront_params(self):
+        p = F.WfParams()
+        F.check("htgx_get_wavefront_params", self._lib.htgx_get_wavefront_params(self._h, C.byref(p)))
+        return {k: getattr(p, k) for k, _ in F.WfParams._fields_}
+
+    def set_wavefront_params(self, ev_thresh=0, chunk=0, grid=0):
+        p = F.WfParams(ev_thresh, chunk, grid, 0)
+        F.check("htgx_set_wavefront_params", self._lib.htgx_set_wavefront_params(self._h, C.byref(p)))
+
+    def autotune(self, desc, budget_ms=2000.0):
+        """Tune the FP32 solver's launch parameters for `desc' (htgx_autotune);
+        returns (params, realisations/s of the best trial, #trials)."""
+        p, rate, n = F.WfParams(), C.c_double(), C.c_uint32()
+        F.check("htgx_autotune", self._lib.htgx_autotune(self._h, C.byref(desc), budget_ms, C.byref(p),
+                                                         C.byref(rate), C.byref(n)))
+        return {k: getattr(p, k) for k, _ in F.WfParams._fields_}, rate.value, n.value
+
     def set_counting(self, on):
         F.check("htgx_set_counting", self._lib.htgx_set_counting(self._h, 1 if on else 0))
 
@@ -182,7 +199,8 @@ def uneven_batch(seed=0, n=64, spp_scale=1.0):
     Per scene: cloud grid uniform in [32, 256]^2 x [32, 96] cells (even),
     100 m cells with the cfg0 LWC law; image N^2 with N uniform in
     [64, 512] (multiple of 8); realisations per pixel log-uniform in
-    [16, 1024] (multiple of 4, times spp_scale); 1-16 k-points. The camera
+    [16, 1024] (times spp_scale, a multiple of 16: 4 steps of a multiple of
+    4 each); 1-16 k-points. The camera
     looks at the grid from its south edge like cfg0's.
     Returns [(Synth, MapDesc)]."""
     rng = np.random.default_rng(0x0BA7C4 + seed)
@@ -191,7 +209,7 @@ def uneven_batch(seed=0, n=64, spp_scale=1.0):
         nx, ny = (int(rng.integers(16, 129)) * 2 for _ in range(2))
         nz = int(rng.integers(16, 49)) * 2
         npx = int(rng.integers(8, 65)) * 8
-        spp = max(4, int(round(2.0 ** rng.uniform(4, 10) * spp_scale / 4)) * 4)
+        spp = max(16, int(round(2.0 ** rng.uniform(4, 10) * spp_scale / 16)) * 16)
         nq = int(rng.integers(1, 17))
         ext = (nx * 100.0, ny * 100.0, 1000.0 + nz * 100.0)
         s = Synth(0, seed=seed * 1000 + i, cloud_def=(nx, ny, nz), cloud_upper=ext, nquads=nq)
@@ -211,17 +229,68 @@ def render_batch(items, stream=None):
     completion time of each map after the launch started, in ms."""
     lib = F.load()
     n = len(items)
-    arr = (F.BatchItem * n)()
-    keep = []
-    for i, (ctx, desc, accum) in enumerate(items):
-        arr[i].ctx = ctx.handle
-        arr[i].desc = C.pointer(desc)
-        arr[i].accum = C.c_void_p(accum)
-        keep.append(desc)
+    arr = _batch_items(items)
     ms = np.zeros(n)
     F.check("htgx_render_batch", lib.htgx_render_batch(
         arr, n, C.c_void_p(stream or 0), ms.ctypes.data_as(F.c_double_p)))
     return ms
+
+
+def _batch_items(items):
+    arr = (F.BatchItem * len(items))()
+    for i, (ctx, desc, accum) in enumerate(items):
+        arr[i].ctx = ctx.handle
+        arr[i].desc = C.pointer(desc)
+        arr[i].accum = C.c_void_p(accum)
+    return arr
+
+
+def batch_autotune(items, budget_ms=10000.0, spp_div=8):
+    """Tune the launch parameters of a batch (htgx_batch_autotune) and set
+    them on every context; returns (params, realisations/s of the best trial,
+    #trials). items: [(Context, MapDesc, device accumulator address)]."""
+    lib = F.load()
+    p, rate, n = F.WfParams(), C.c_double(), C.c_uint32()
+    F.check("htgx_batch_autotune", lib.htgx_batch_autotune(
+        _batch_items(items), len(items), spp_div, budget_ms, C.byref(p), C.byref(rate), C.byref(n)))
+    return {k: getattr(p, k) for k, _ in F.WfParams._fields_}, rate.value, n.value
+
+
+class BatchPlan:
+    """A batch rendered in repeated steps (htgx_batch_plan_*), each step
+    replaying one HIP graph when use_graph.
+
+    items: [(Context, MapDesc, device accumulator address)]."""
+
+    def __init__(self, items, use_graph=True):
+        self._lib = F.load()
+        self.n = len(items)
+        self._h = C.c_void_p()
+        F.check("htgx_batch_plan_create", self._lib.htgx_batch_plan_create(
+            _batch_items(items), self.n, int(use_graph), C.byref(self._h)))
+
+    def step(self, stream=None):
+        F.check("htgx_batch_plan_step", self._lib.htgx_batch_plan_step(self._h, C.c_void_p(stream or 0)))
+
+    def rewind(self):
+        F.check("htgx_batch_plan_rewind", self._lib.htgx_batch_plan_rewind(self._h))
+
+    def scene_ms(self):
+        ms = np.zeros(self.n)
+        F.check("htgx_batch_plan_scene_ms", self._lib.htgx_batch_plan_scene_ms(
+            self._h, ms.ctypes.data_as(F.c_double_p)))
+        return ms
+
+    def close(self):
+        if self._h:
+            self._lib.htgx_batch_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def threefry(ctr, key):

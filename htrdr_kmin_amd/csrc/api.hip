@@ -653,14 +653,41 @@ htgx_render_accum(htgx_ctx* ctx, const htgx_map_desc* desc, htgx_mc_accum* accum
 }
 
 extern "C" int
-htgx_render_batch(const htgx_batch_item* items, uint32_t n, void* stream, double* scene_ms)
+htgx_get_wavefront_params(htgx_ctx* ctx, htgx_wf_params* params)
+{
+  if(!ctx || !params) return HTGX_BAD_ARG;
+  return htgx_wavefront_params(ctx, params, nullptr);
+}
+
+extern "C" int
+htgx_set_wavefront_params(htgx_ctx* ctx, const htgx_wf_params* params)
+{
+  if(!ctx || !params) return HTGX_BAD_ARG;
+  return htgx_wavefront_params(ctx, nullptr, params);
+}
+
+extern "C" int
+htgx_autotune(htgx_ctx* ctx, const htgx_map_desc* desc, double budget_ms, htgx_wf_params* best,
+              double* rate, uint32_t* ntrials)
+{
+  if(!ctx || !desc || !(budget_ms >= 0)) return HTGX_BAD_ARG;
+  DevMap M;
+  int res;
+  if((res = htgx_map_setup(ctx, desc, &M)) != HTGX_OK) return res;
+  return htgx_wavefront_autotune(ctx, M, budget_ms, best, rate, ntrials);
+}
+
+/* Checks of a batch and its device maps */
+static int
+batch_setup(const htgx_batch_item* items, uint32_t n, std::vector<DevMap>& Ms,
+            std::vector<htgx_ctx*>& ctxs, std::vector<htgx_mc_accum*>& accums)
 {
   if(!items || !n) return HTGX_BAD_ARG;
   int dev = -1, res;
   HIP_OK(hipGetDevice(&dev));
-  std::vector<DevMap> Ms(n);
-  std::vector<htgx_ctx*> ctxs(n);
-  std::vector<htgx_mc_accum*> accums(n);
+  Ms.resize(n);
+  ctxs.resize(n);
+  accums.resize(n);
   for(uint32_t i = 0; i < n; ++i) {
     const htgx_batch_item& it = items[i];
     if(!it.ctx || !it.desc || !it.accum || it.ctx->device != dev) return HTGX_BAD_ARG;
@@ -672,8 +699,77 @@ htgx_render_batch(const htgx_batch_item* items, uint32_t n, void* stream, double
     ctxs[i] = it.ctx;
     accums[i] = it.accum;
   }
+  return HTGX_OK;
+}
+
+extern "C" int
+htgx_render_batch(const htgx_batch_item* items, uint32_t n, void* stream, double* scene_ms)
+{
+  std::vector<DevMap> Ms;
+  std::vector<htgx_ctx*> ctxs;
+  std::vector<htgx_mc_accum*> accums;
+  int res;
+  if((res = batch_setup(items, n, Ms, ctxs, accums)) != HTGX_OK) return res;
   return htgx_render_wavefront_batch(ctxs.data(), Ms.data(), accums.data(), n,
                                      (hipStream_t)stream, scene_ms);
+}
+
+extern "C" int
+htgx_batch_plan_create(const htgx_batch_item* items, uint32_t n, int use_graph,
+                       htgx_batch_plan** plan)
+{
+  if(!plan) return HTGX_BAD_ARG;
+  *plan = nullptr;
+  std::vector<DevMap> Ms;
+  std::vector<htgx_ctx*> ctxs;
+  std::vector<htgx_mc_accum*> accums;
+  int res;
+  if((res = batch_setup(items, n, Ms, ctxs, accums)) != HTGX_OK) return res;
+  return wf_plan_build(ctxs.data(), Ms.data(), accums.data(), n, true, use_graph != 0, plan);
+}
+
+extern "C" int
+htgx_batch_autotune(const htgx_batch_item* items, uint32_t n, uint32_t spp_div, double budget_ms,
+                    htgx_wf_params* best, double* rate, uint32_t* ntrials)
+{
+  if(!(budget_ms >= 0) || !spp_div) return HTGX_BAD_ARG;
+  std::vector<DevMap> Ms;
+  std::vector<htgx_ctx*> ctxs;
+  std::vector<htgx_mc_accum*> accums;
+  int res;
+  if((res = batch_setup(items, n, Ms, ctxs, accums)) != HTGX_OK) return res;
+  return htgx_wavefront_batch_autotune(ctxs.data(), Ms.data(), n, spp_div, budget_ms, best, rate,
+                                       ntrials);
+}
+
+extern "C" int
+htgx_batch_plan_step(htgx_batch_plan* plan, void* stream)
+{
+  if(!plan) return HTGX_BAD_ARG;
+  return wf_plan_enqueue(plan, (hipStream_t)stream);
+}
+
+extern "C" int
+htgx_batch_plan_rewind(htgx_batch_plan* plan)
+{
+  if(!plan) return HTGX_BAD_ARG;
+  return wf_plan_rewind(plan);
+}
+
+extern "C" int
+htgx_batch_plan_scene_ms(htgx_batch_plan* plan, double* scene_ms)
+{
+  if(!plan || !scene_ms) return HTGX_BAD_ARG;
+  return wf_plan_times(plan, scene_ms);
+}
+
+extern "C" int
+htgx_batch_plan_destroy(htgx_batch_plan* plan)
+{
+  if(!plan) return HTGX_BAD_ARG;
+  (void)hipDeviceSynchronize();
+  wf_plan_free(plan);
+  return HTGX_OK;
 }
 
 extern "C" int

@@ -52,6 +52,24 @@ int htgx_render_wavefront(htgx_ctx* ctx, const htgx::DevMap& map, htgx_mc_accum*
  * completion time of map i after the launch started. */
 int htgx_render_wavefront_batch(htgx_ctx* const* ctxs, const htgx::DevMap* Ms, htgx_mc_accum* const* accums,
                                 uint32_t n, hipStream_t stream, double* scene_ms);
+/* Launch parameters of the FP32 solver of a context: read (get) and/or
+ * replace (set; zero fields take the defaults) */
+int htgx_wavefront_params(htgx_ctx* ctx, htgx_wf_params* get, const htgx_wf_params* set);
+int htgx_wavefront_autotune(htgx_ctx* ctx, const htgx::DevMap& M, double budget_ms,
+                            htgx_wf_params* best, double* best_rate, uint32_t* ntrials);
+int htgx_wavefront_batch_autotune(htgx_ctx* const* ctxs, const htgx::DevMap* Ms, uint32_t n,
+                                  uint32_t spp_div, double budget_ms, htgx_wf_params* best,
+                                  double* best_rate, uint32_t* ntrials);
+/* Batch plans (htgx_batch_plan_* of htgx.h). own_partials: the plan holds
+ * its own chunk partials and wavelength tables, so that the contexts may draw
+ * other maps between its steps. use_graph: one step is recorded as a HIP
+ * graph at build time and replayed by wf_plan_enqueue. */
+int wf_plan_build(htgx_ctx* const* ctxs, const htgx::DevMap* Ms, htgx_mc_accum* const* accums,
+                  uint32_t n, bool own_partials, bool use_graph, htgx_batch_plan** out);
+int wf_plan_enqueue(htgx_batch_plan* p, hipStream_t stream);
+int wf_plan_rewind(htgx_batch_plan* p);
+int wf_plan_times(htgx_batch_plan* p, double* scene_ms);
+void wf_plan_free(htgx_batch_plan* p);
 int htgx_debug_realisations_fp32(htgx_ctx* ctx, const htgx_map_desc* desc,
                                  const uint64_t* pix, const uint64_t* real, uint32_t n,
                                  double* weights);

@@ -1130,6 +1130,7 @@ struct BatchScene {
   SceneF S;
   DevMap M;
   Launch P;
+  htgx_mc_accum* accum; /* the map's pixel accumulators */
 };
 
 /* Several maps of distinct scenes in one persistent launch. Every wavefront
@@ -1150,19 +1151,34 @@ wavefront_batch_kernel(const BatchScene* __restrict__ scenes, uint32_t nscenes)
   }
 }
 
+/* Start of a batched step: ticks[0] = now, and ticks[1] = now on the first
+ * step (*step == 0) */
 __global__ void
-tick_kernel(unsigned long long* out)
+batch_tick_kernel(unsigned long long* ticks, const unsigned int* step)
 {
-  *out = wall_clock64();
+  const unsigned long long now = wall_clock64();
+  ticks[0] = now;
+  if(*step == 0u) ticks[1] = now;
 }
 
-/* Fold the chunks of each pixel, in chunk order, into its accumulator */
-__global__ void __launch_bounds__(256)
-wf_merge_kernel(const htgx_mc_accum* partials, uint32_t width, uint32_t height, uint32_t ntx,
-                uint32_t nsub, htgx_mc_accum* accum)
+/* End of a batched step: the next step draws the next spp realisations of
+ * every pixel */
+__global__ void
+batch_advance_kernel(BatchScene* scenes, uint32_t n, unsigned int* step)
 {
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if(p >= width * height) return;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < n) {
+    scenes[i].P.spp_offset += scenes[i].M.spp;
+    scenes[i].M.spp_offset += scenes[i].M.spp;
+  }
+  if(i == 0) *step += 1u;
+}
+
+/* Fold the chunks of pixel p, in chunk order, into its accumulator */
+__device__ __forceinline__ void
+merge_pixel(const htgx_mc_accum* partials, uint32_t width, uint32_t ntx, uint32_t nsub,
+            htgx_mc_accum* accum, uint32_t p)
+{
   const uint32_t x = p % width, y = p / width;
   const uint32_t t = (y / 8u) * ntx + (x / 8u);
   const uint32_t ix = x & 7u, iy = y & 7u;
@@ -1181,6 +1197,24 @@ wf_merge_kernel(const htgx_mc_accum* partials, uint32_t width, uint32_t height, 
     a.nfail += q.nfail;
   }
   accum[p] = a;
+}
+
+__global__ void __launch_bounds__(256)
+wf_merge_kernel(const htgx_mc_accum* partials, uint32_t width, uint32_t height, uint32_t ntx,
+                uint32_t nsub, htgx_mc_accum* accum)
+{
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if(p < width * height) merge_pixel(partials, width, ntx, nsub, accum, p);
+}
+
+/* The merges of every map of a batch: blockIdx.y is the table slot */
+__global__ void __launch_bounds__(256)
+batch_merge_kernel(const BatchScene* scenes)
+{
+  const BatchScene& B = scenes[blockIdx.y];
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if(p < B.M.width * B.M.height)
+    merge_pixel(B.P.partials, B.M.width, B.P.ntx, B.P.nsub, B.accum, p);
 }
 
 __global__ void
@@ -1258,8 +1292,24 @@ struct htgx_wavefront_state {
   void* d_partials = nullptr;
   size_t partials_cap = 0;
   uint32_t grid = 0;
+  uint32_t grid_max = 0; /* resident blocks of the device */
   uint32_t K = 4;
   uint32_t ev_thresh = 24;
+};
+
+/* Device array freed with its owner */
+template<class T>
+struct DevArrayLite {
+  T* p = nullptr;
+  DevArrayLite() = default;
+  DevArrayLite(const DevArrayLite&) = delete;
+  DevArrayLite(DevArrayLite&& o) noexcept : p(o.p) { o.p = nullptr; }
+  ~DevArrayLite() { if(p) (void)hipFree(p); }
+  int alloc(size_t n)
+  {
+    HIP_OK(hipMalloc((void**)&p, std::max<size_t>(n, 1) * sizeof(T)));
+    return HTGX_OK;
+  }
 };
 
 template<class T>
@@ -1395,7 +1445,7 @@ wavefront_build(htgx_ctx* ctx, htgx_wavefront_state* st)
   HIP_OK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
   HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu,
     wavefront_kernel<HTGX_SENSOR_CAMERA, HTGX_SPECTRAL_SW, false>, 256, 0));
-  st->grid = (uint32_t)std::max(1, ncu * std::max(1, per_cu));
+  st->grid = st->grid_max = (uint32_t)std::max(1, ncu * std::max(1, per_cu));
   if(const char* e = std::getenv("HTGX_WF_GRID")) st->grid = (uint32_t)std::max(1, atoi(e));
   if(const char* e = std::getenv("HTGX_WF_EVTHRESH")) st->ev_thresh = (uint32_t)std::max(1, atoi(e));
   if(const char* e = std::getenv("HTGX_WF_CHUNK")) st->K = (uint32_t)std::min(16, std::max(1, atoi(e)));
@@ -1460,89 +1510,380 @@ int htgx_render_wavefront(htgx_ctx* ctx, const DevMap& M, htgx_mc_accum* accum, 
   return HTGX_OK;
 }
 
-/* Device buffers of one batched call, freed on every return path */
-struct BatchBuffers {
-  BatchScene* table = nullptr;
-  unsigned long long* ticks = nullptr;  /* [n] done ticks, then the start tick */
-  unsigned int* flags = nullptr;        /* [n] exhausted, then [n] done counts */
-  ~BatchBuffers()
-  {
-    if(table) (void)hipFree(table);
-    if(ticks) (void)hipFree(ticks);
-    if(flags) (void)hipFree(flags);
+int htgx_wavefront_params(htgx_ctx* ctx, htgx_wf_params* get, const htgx_wf_params* set)
+{
+  int res;
+  if((res = wavefront_prepare(ctx)) != HTGX_OK) return res;
+  htgx_wavefront_state* st = ctx->wf;
+  if(set) {
+    if(set->ev_thresh > 64 || set->chunk > 16 || set->grid > st->grid_max) return HTGX_BAD_ARG;
+    st->ev_thresh = set->ev_thresh ? set->ev_thresh : 24;
+    st->K = set->chunk ? set->chunk : 4;
+    st->grid = set->grid ? set->grid : st->grid_max;
   }
+  if(get) {
+    get->ev_thresh = st->ev_thresh;
+    get->chunk = st->K;
+    get->grid = st->grid;
+    get->grid_max = st->grid_max;
+  }
+  return HTGX_OK;
+}
+
+/* A batch of maps prepared for repeated steps (htgx_batch_plan_*): its own
+ * device table, chunk queues, flags and clock ticks, and optionally its own
+ * chunk partials and a HIP graph of one step. */
+struct htgx_batch_plan {
+  int device = 0;
+  uint32_t n = 0;
+  int sensor = 0, spectral = 0;
+  uint32_t grid = 0;
+  uint32_t maxpix = 0;
+  std::vector<uint32_t> order;      /* table slot -> item */
+  std::vector<BatchScene> table_h;  /* the table at step 0 */
+  std::vector<double> us_per_tick;  /* per table slot */
+  std::vector<void*> bufs;          /* owned device memory */
+  BatchScene* table = nullptr;
+  unsigned long long* ticks = nullptr; /* [n] done ticks, step start, first start */
+  unsigned int* flags = nullptr;       /* [n] exhausted, [n] done counts, step counter */
+  unsigned int* queues = nullptr;      /* [n][kXcds] chunk counters */
+  hipStream_t cap = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
 };
+
+void wf_plan_free(htgx_batch_plan* p)
+{
+  if(!p) return;
+  if(p->exec) (void)hipGraphExecDestroy(p->exec);
+  if(p->graph) (void)hipGraphDestroy(p->graph);
+  if(p->cap) (void)hipStreamDestroy(p->cap);
+  for(void* b : p->bufs) (void)hipFree(b);
+  delete p;
+}
+
+template<class T>
+static int plan_alloc(htgx_batch_plan* p, size_t n, T** out)
+{
+  void* d = nullptr;
+  HIP_OK(hipMalloc(&d, std::max<size_t>(n, 1) * sizeof(T)));
+  p->bufs.push_back(d);
+  *out = (T*)d;
+  return HTGX_OK;
+}
+
+/* Launches of one step on `stream': no allocation, no host copy, no memset
+ * (so that a HIP graph can record them) */
+static int plan_launch_step(htgx_batch_plan* p, hipStream_t stream)
+{
+  const uint32_t n = p->n;
+  const uint32_t nq = n * kXcds;
+  zero_u32_kernel<<<(nq + 255) / 256, 256, 0, stream>>>(p->queues, nq);
+  zero_u32_kernel<<<(2 * n + 255) / 256, 256, 0, stream>>>(p->flags, 2 * n);
+  batch_tick_kernel<<<1, 1, 0, stream>>>(p->ticks + n, p->flags + 2 * n);
+  HIP_OK(hipGetLastError());
+  if(p->sensor == HTGX_SENSOR_CAMERA) {
+    if(p->spectral == HTGX_SPECTRAL_LW)
+      wavefront_batch_kernel<HTGX_SENSOR_CAMERA, HTGX_SPECTRAL_LW><<<p->grid, 256, 0, stream>>>(p->table, n);
+    else
+      wavefront_batch_kernel<HTGX_SENSOR_CAMERA, HTGX_SPECTRAL_SW><<<p->grid, 256, 0, stream>>>(p->table, n);
+  } else {
+    if(p->spectral == HTGX_SPECTRAL_LW)
+      wavefront_batch_kernel<HTGX_SENSOR_RECTANGLE, HTGX_SPECTRAL_LW><<<p->grid, 256, 0, stream>>>(p->table, n);
+    else
+      wavefront_batch_kernel<HTGX_SENSOR_RECTANGLE, HTGX_SPECTRAL_SW><<<p->grid, 256, 0, stream>>>(p->table, n);
+  }
+  HIP_OK(hipGetLastError());
+  batch_merge_kernel<<<dim3((p->maxpix + 255) / 256, n), 256, 0, stream>>>(p->table);
+  batch_advance_kernel<<<(n + 255) / 256, 256, 0, stream>>>(p->table, n, p->flags + 2 * n);
+  HIP_OK(hipGetLastError());
+  return HTGX_OK;
+}
+
+int wf_plan_build(htgx_ctx* const* ctxs, const DevMap* Ms, htgx_mc_accum* const* accums,
+                  uint32_t n, bool own_partials, bool use_graph, htgx_batch_plan** out)
+{
+  int res;
+  *out = nullptr;
+  htgx_batch_plan* p = new htgx_batch_plan;
+  struct Guard {
+    htgx_batch_plan*& p;
+    ~Guard() { wf_plan_free(p); }
+  } guard{p};
+  HIP_OK(hipGetDevice(&p->device));
+  p->n = n;
+  p->sensor = Ms[0].sensor;
+  p->spectral = Ms[0].spectral_type;
+  p->order.resize(n);
+  for(uint32_t i = 0; i < n; ++i) {
+    if((res = wavefront_prepare(ctxs[i])) != HTGX_OK) return res;
+    p->grid = std::max(p->grid, ctxs[i]->wf->grid);
+    p->order[i] = i;
+  }
+  /* largest maps first: the small ones fill the tail */
+  std::stable_sort(p->order.begin(), p->order.end(), [&](uint32_t a, uint32_t b) {
+    return (uint64_t)Ms[a].width * Ms[a].height * Ms[a].spp
+         > (uint64_t)Ms[b].width * Ms[b].height * Ms[b].spp;
+  });
+  if((res = plan_alloc(p, n, &p->table))) return res;
+  if((res = plan_alloc(p, n + 2, &p->ticks))) return res;
+  if((res = plan_alloc(p, 2 * n + 1, &p->flags))) return res;
+  if((res = plan_alloc(p, (size_t)n * kXcds, &p->queues))) return res;
+  p->table_h.resize(n);
+  p->us_per_tick.resize(n);
+  for(uint32_t k = 0; k < n; ++k) {
+    const uint32_t i = p->order[k];
+    BatchScene& e = p->table_h[k];
+    e.S = ctxs[i]->wf->S;
+    e.M = Ms[i];
+    e.accum = accums[i];
+    if((res = setup_launch(ctxs[i], Ms[i], &e.P)) != HTGX_OK) return res;
+    e.P.stats = nullptr;
+    e.P.counters = p->queues + (size_t)k * kXcds;
+    e.P.exhausted = p->flags + k;
+    e.P.done_count = p->flags + n + k;
+    e.P.done_tick = p->ticks + k;
+    if(own_partials) {
+      /* the context's partials and wavelength tables may be reallocated by
+       * its next map: the plan keeps copies of its own */
+      const size_t np = (size_t)e.P.ntiles * e.P.nsub * 64u;
+      if((res = plan_alloc(p, np, &e.P.partials))) return res;
+      if(e.M.nwbands) {
+        double *cdf = nullptr, *pdf = nullptr;
+        if((res = plan_alloc(p, e.M.nwbands, &cdf)) || (res = plan_alloc(p, e.M.nwbands, &pdf)))
+          return res;
+        HIP_OK(hipMemcpy(cdf, e.M.wcdf, e.M.nwbands * sizeof(double), hipMemcpyDeviceToDevice));
+        HIP_OK(hipMemcpy(pdf, e.M.wpdf, e.M.nwbands * sizeof(double), hipMemcpyDeviceToDevice));
+        e.M.wcdf = cdf;
+        e.M.wpdf = pdf;
+      }
+    }
+    p->maxpix = std::max(p->maxpix, Ms[i].width * Ms[i].height);
+    p->us_per_tick[k] = ctxs[i]->us_per_tick;
+  }
+  HIP_OK(hipMemcpy(p->table, p->table_h.data(), n * sizeof(BatchScene), hipMemcpyHostToDevice));
+  HIP_OK(hipMemset(p->flags, 0, (2 * n + 1) * sizeof(unsigned int)));
+  HIP_OK(hipMemset(p->ticks, 0, (n + 2) * sizeof(unsigned long long)));
+  if(use_graph) {
+    HIP_OK(hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking));
+    HIP_OK(hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal));
+    res = plan_launch_step(p, p->cap);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(p->cap, &g);
+    p->graph = g;
+    if(res) return res;
+    if(e != hipSuccess || !g) return HTGX_UNKNOWN_ERR;
+    HIP_OK(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+  }
+  *out = p;
+  p = nullptr; /* handed to the caller */
+  return HTGX_OK;
+}
+
+int wf_plan_enqueue(htgx_batch_plan* p, hipStream_t stream)
+{
+  int dev = -1;
+  HIP_OK(hipGetDevice(&dev));
+  if(dev != p->device) return HTGX_BAD_ARG;
+  if(p->exec) {
+    HIP_OK(hipGraphLaunch(p->exec, stream));
+    return HTGX_OK;
+  }
+  return plan_launch_step(p, stream);
+}
+
+int wf_plan_rewind(htgx_batch_plan* p)
+{
+  HIP_OK(hipDeviceSynchronize());
+  HIP_OK(hipMemcpy(p->table, p->table_h.data(), p->n * sizeof(BatchScene), hipMemcpyHostToDevice));
+  HIP_OK(hipMemset(p->flags + 2 * p->n, 0, sizeof(unsigned int)));
+  return HTGX_OK;
+}
+
+int wf_plan_times(htgx_batch_plan* p, double* scene_ms)
+{
+  const uint32_t n = p->n;
+  std::vector<unsigned long long> t(n + 2);
+  HIP_OK(hipDeviceSynchronize());
+  HIP_OK(hipMemcpy(t.data(), p->ticks, (n + 2) * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  for(uint32_t k = 0; k < n; ++k)
+    scene_ms[p->order[k]] = (double)(t[k] - t[n + 1]) * p->us_per_tick[k] * 1e-3;
+  return HTGX_OK;
+}
 
 int htgx_render_wavefront_batch(htgx_ctx* const* ctxs, const DevMap* Ms, htgx_mc_accum* const* accums,
                                 uint32_t n, hipStream_t stream, double* scene_ms)
 {
+  htgx_batch_plan* p = nullptr;
   int res;
-  uint32_t grid = 0;
-  std::vector<BatchScene> table(n);
-  std::vector<uint32_t> order(n);
+  if((res = wf_plan_build(ctxs, Ms, accums, n, false, false, &p)) != HTGX_OK) return res;
+  struct Guard {
+    htgx_batch_plan* p;
+    ~Guard() { wf_plan_free(p); }
+  } guard{p};
+  if((res = plan_launch_step(p, stream)) != HTGX_OK) return res;
+  for(uint32_t i = 0; i < n; ++i)
+    ctxs[i]->last_nrealisations = (uint64_t)Ms[i].width * Ms[i].height * Ms[i].spp;
+  /* the plan's buffers are freed on return: wait for the launch */
+  HIP_OK(hipStreamSynchronize(stream));
+  if(scene_ms) return wf_plan_times(p, scene_ms);
+  return HTGX_OK;
+}
+
+/*******************************************************************************
+ * Autotuner of the launch parameters
+ ******************************************************************************/
+/* A private stream and two events for timed trials */
+struct TrialClock {
+  hipStream_t stream = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  ~TrialClock()
+  {
+    if(stream) (void)hipStreamSynchronize(stream);
+    if(e0) (void)hipEventDestroy(e0);
+    if(e1) (void)hipEventDestroy(e1);
+    if(stream) (void)hipStreamDestroy(stream);
+  }
+  int init()
+  {
+    HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    return HTGX_OK;
+  }
+  /* device time of the launches enqueued by f() on `stream' */
+  template<class F>
+  int time(F&& f, double* ms)
+  {
+    HIP_OK(hipEventRecord(e0, stream));
+    const int res = f();
+    if(res != HTGX_OK) return res;
+    HIP_OK(hipEventRecord(e1, stream));
+    HIP_OK(hipEventSynchronize(e1));
+    float t = 0;
+    HIP_OK(hipEventElapsedTime(&t, e0, e1));
+    *ms = t;
+    return HTGX_OK;
+  }
+};
+
+/* Coordinate descent over the event threshold, then the chunk size, then the
+ * grid. trial(p, &ms) runs one timed launch with parameters p; a candidate
+ * replaces the current parameters only when more than 1% faster (trial
+ * timings repeat to a fraction of that). The first launch is an untimed
+ * warm-up; the search stops before a trial that would exceed the budget,
+ * estimated from the last one. */
+template<class Trial>
+static int tune_search(Trial&& trial, double budget_ms, uint32_t grid_max, htgx_wf_params* cur,
+                       double* cur_ms, uint32_t* ntrials)
+{
+  static const uint32_t thresholds[] = {8, 16, 24, 32, 40, 48};
+  static const uint32_t chunks[] = {2, 4, 8};
+  const uint32_t grids[] = {grid_max, std::max(1u, grid_max * 3 / 4), std::max(1u, grid_max / 2)};
+  double ms = 0;
+  int res;
+  if((res = trial(*cur, &ms)) != HTGX_OK) return res;
+  if((res = trial(*cur, &ms)) != HTGX_OK) return res;
+  double spent = ms, last = ms;
+  uint32_t n = 1;
+  *cur_ms = ms;
+  for(int dim = 0; dim < 3; ++dim) {
+    const uint32_t* vals = dim == 0 ? thresholds : dim == 1 ? chunks : grids;
+    const int nv = dim == 0 ? 6 : 3;
+    const htgx_wf_params base = *cur;
+    for(int i = 0; i < nv; ++i) {
+      htgx_wf_params p = base;
+      if(dim == 0) p.ev_thresh = vals[i];
+      else if(dim == 1) p.chunk = vals[i];
+      else p.grid = vals[i];
+      if(p.ev_thresh == base.ev_thresh && p.chunk == base.chunk && p.grid == base.grid) continue;
+      if(spent + last > budget_ms) { dim = 3; break; }
+      if((res = trial(p, &ms)) != HTGX_OK) return res;
+      spent += ms;
+      last = ms;
+      ++n;
+      if(ms < 0.99 * *cur_ms) { *cur = p; *cur_ms = ms; }
+    }
+  }
+  *ntrials = n;
+  return HTGX_OK;
+}
+
+static void set_params(htgx_wavefront_state* st, const htgx_wf_params& p)
+{
+  st->ev_thresh = p.ev_thresh;
+  st->K = p.chunk;
+  st->grid = p.grid;
+}
+
+int htgx_wavefront_autotune(htgx_ctx* ctx, const DevMap& M, double budget_ms,
+                            htgx_wf_params* best, double* best_rate, uint32_t* ntrials)
+{
+  int res;
+  if((res = wavefront_prepare(ctx)) != HTGX_OK) return res;
+  htgx_wavefront_state* st = ctx->wf;
+  const size_t npix = (size_t)M.width * M.height;
+  TrialClock clk;
+  DevArrayLite<htgx_mc_accum> acc;
+  if((res = clk.init()) || (res = acc.alloc(npix))) return res;
+  const int counting = ctx->counting;
+  ctx->counting = 0;
+  htgx_wf_params cur = {st->ev_thresh, st->K, st->grid, st->grid_max};
+  double ms = 0;
+  uint32_t n = 0;
+  res = tune_search([&](const htgx_wf_params& p, double* t) {
+    set_params(st, p);
+    return clk.time([&] { return htgx_render_wavefront(ctx, M, acc.p, clk.stream); }, t);
+  }, budget_ms, st->grid_max, &cur, &ms, &n);
+  ctx->counting = counting;
+  set_params(st, cur); /* kept for the maps that follow */
+  if(res != HTGX_OK) return res;
+  if(best) *best = cur;
+  if(best_rate) *best_rate = (double)npix * M.spp / (ms * 1e-3);
+  if(ntrials) *ntrials = n;
+  return HTGX_OK;
+}
+
+int htgx_wavefront_batch_autotune(htgx_ctx* const* ctxs, const DevMap* Ms, uint32_t n,
+                                  uint32_t spp_div, double budget_ms, htgx_wf_params* best,
+                                  double* best_rate, uint32_t* ntrials)
+{
+  int res;
+  uint32_t grid_max = 0;
+  std::vector<DevMap> T(Ms, Ms + n);
+  std::vector<DevArrayLite<htgx_mc_accum>> acc(n);
+  std::vector<htgx_mc_accum*> accp(n);
+  double nreal = 0;
   for(uint32_t i = 0; i < n; ++i) {
     if((res = wavefront_prepare(ctxs[i])) != HTGX_OK) return res;
-    grid = std::max(grid, ctxs[i]->wf->grid);
-    order[i] = i;
+    grid_max = std::max(grid_max, ctxs[i]->wf->grid_max);
+    /* trial maps: 1/spp_div of the realisations per pixel, at least 4 */
+    T[i].spp = std::max(4u, T[i].spp / spp_div / 4u * 4u);
+    T[i].spp_offset = 0;
+    if((res = acc[i].alloc((size_t)T[i].width * T[i].height))) return res;
+    accp[i] = acc[i].p;
+    nreal += (double)T[i].width * T[i].height * T[i].spp;
   }
-  /* largest maps first: the small ones fill the tail */
-  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
-    return (uint64_t)Ms[a].width * Ms[a].height * Ms[a].spp
-         > (uint64_t)Ms[b].width * Ms[b].height * Ms[b].spp;
-  });
-  BatchBuffers B;
-  HIP_OK(hipMalloc(&B.table, n * sizeof(BatchScene)));
-  HIP_OK(hipMalloc(&B.ticks, (n + 1) * sizeof(unsigned long long)));
-  HIP_OK(hipMalloc(&B.flags, 2 * n * sizeof(unsigned int)));
-  for(uint32_t k = 0; k < n; ++k) {
-    const uint32_t i = order[k];
-    BatchScene& e = table[k];
-    e.S = ctxs[i]->wf->S;
-    e.M = Ms[i];
-    if((res = setup_launch(ctxs[i], Ms[i], &e.P)) != HTGX_OK) return res;
-    e.P.stats = nullptr;
-    e.P.exhausted = B.flags + k;
-    e.P.done_count = B.flags + n + k;
-    e.P.done_tick = B.ticks + k;
-    zero_u32_kernel<<<1, 64, 0, stream>>>(e.P.counters, kXcds);
-    HIP_OK(hipGetLastError());
-  }
-  zero_u32_kernel<<<(2 * n + 255) / 256, 256, 0, stream>>>(B.flags, 2 * n);
-  zero_u64_kernel<<<(n + 1 + 255) / 256, 256, 0, stream>>>(B.ticks, n + 1);
-  HIP_OK(hipGetLastError());
-  HIP_OK(hipMemcpyAsync(B.table, table.data(), n * sizeof(BatchScene), hipMemcpyHostToDevice, stream));
-  tick_kernel<<<1, 1, 0, stream>>>(B.ticks + n);
-  HIP_OK(hipGetLastError());
-  const DevMap& M0 = Ms[0];
-  if(M0.sensor == HTGX_SENSOR_CAMERA) {
-    if(M0.spectral_type == HTGX_SPECTRAL_LW)
-      wavefront_batch_kernel<HTGX_SENSOR_CAMERA, HTGX_SPECTRAL_LW><<<grid, 256, 0, stream>>>(B.table, n);
-    else
-      wavefront_batch_kernel<HTGX_SENSOR_CAMERA, HTGX_SPECTRAL_SW><<<grid, 256, 0, stream>>>(B.table, n);
-  } else {
-    if(M0.spectral_type == HTGX_SPECTRAL_LW)
-      wavefront_batch_kernel<HTGX_SENSOR_RECTANGLE, HTGX_SPECTRAL_LW><<<grid, 256, 0, stream>>>(B.table, n);
-    else
-      wavefront_batch_kernel<HTGX_SENSOR_RECTANGLE, HTGX_SPECTRAL_SW><<<grid, 256, 0, stream>>>(B.table, n);
-  }
-  HIP_OK(hipGetLastError());
-  for(uint32_t k = 0; k < n; ++k) {
-    const uint32_t i = order[k];
-    const Launch& P = table[k].P;
-    const uint32_t npix = Ms[i].width * Ms[i].height;
-    wf_merge_kernel<<<(npix + 255) / 256, 256, 0, stream>>>(
-      P.partials, Ms[i].width, Ms[i].height, P.ntx, P.nsub, accums[i]);
-    HIP_OK(hipGetLastError());
-    ctxs[i]->last_nrealisations = (uint64_t)npix * Ms[i].spp;
-  }
-  /* the table, flags and ticks are freed on return: wait for the launch */
-  HIP_OK(hipStreamSynchronize(stream));
-  if(scene_ms) {
-    std::vector<unsigned long long> t(n + 1);
-    HIP_OK(hipMemcpy(t.data(), B.ticks, (n + 1) * sizeof(unsigned long long), hipMemcpyDeviceToHost));
-    for(uint32_t k = 0; k < n; ++k)
-      scene_ms[order[k]] = (double)(t[k] - t[n]) * ctxs[order[k]]->us_per_tick * 1e-3;
-  }
+  TrialClock clk;
+  if((res = clk.init())) return res;
+  htgx_wavefront_state* st0 = ctxs[0]->wf;
+  htgx_wf_params cur = {st0->ev_thresh, st0->K, st0->grid, grid_max};
+  double ms = 0;
+  uint32_t nt = 0;
+  res = tune_search([&](const htgx_wf_params& p, double* t) {
+    for(uint32_t i = 0; i < n; ++i) set_params(ctxs[i]->wf, p);
+    htgx_batch_plan* plan = nullptr;
+    int r = wf_plan_build(ctxs, T.data(), accp.data(), n, false, false, &plan);
+    if(r != HTGX_OK) return r;
+    r = clk.time([&] { return wf_plan_enqueue(plan, clk.stream); }, t);
+    wf_plan_free(plan);
+    return r;
+  }, budget_ms, grid_max, &cur, &ms, &nt);
+  for(uint32_t i = 0; i < n; ++i) set_params(ctxs[i]->wf, cur);
+  if(res != HTGX_OK) return res;
+  if(best) *best = cur;
+  if(best_rate) *best_rate = nreal / (ms * 1e-3);
+  if(ntrials) *ntrials = nt;
   return HTGX_OK;
 }
 

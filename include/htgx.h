@@ -264,6 +264,39 @@ int htgx_render_accum
    struct htgx_mc_accum* accum,
    void* stream);
 
+/* Launch parameters of the FP32 wavefront solver of a context. No
+ * counterpart in the reference, whose tile dispatch has no such knobs. They
+ * change the schedule only: the realisations, and so the estimates, are the
+ * same for any values; the chunk size changes how partial sums are grouped,
+ * hence the last bits of the accumulators. */
+struct htgx_wf_params {
+  uint32_t ev_thresh; /* lanes of a 64-lane wavefront that must wait for the
+                         event step before it runs (1-64; 0 = default 24) */
+  uint32_t chunk;     /* realisations per work-queue chunk (1-16; 0 = default 4) */
+  uint32_t grid;      /* blocks of 256 lanes of the persistent launch
+                         (1-grid_max; 0 = grid_max) */
+  uint32_t grid_max;  /* read only: blocks resident on the device at once */
+};
+
+int htgx_get_wavefront_params(struct htgx_ctx* ctx, struct htgx_wf_params* params);
+int htgx_set_wavefront_params(struct htgx_ctx* ctx, const struct htgx_wf_params* params);
+
+/* Autotuner of the parameters above for a map: trial launches of the map as
+ * described (desc->spp realisations per pixel: pass the launch size that
+ * will be used), timed on the device, by coordinate descent (event
+ * threshold, then chunk size, then grid) within about budget_ms of device
+ * time; a candidate must be more than 1% faster to be taken. The best
+ * parameters are kept by the context for the maps that follow; best, rate
+ * (realisations/s of the best trial) and ntrials are optional outputs.
+ * Synchronous. */
+int htgx_autotune
+  (struct htgx_ctx* ctx,
+   const struct htgx_map_desc* desc,
+   double budget_ms,
+   struct htgx_wf_params* best,
+   double* rate,
+   uint32_t* ntrials);
+
 /* Batched rendering. No counterpart in the reference, which draws one map
  * per process: several maps of distinct scenes in one persistent FP32
  * launch, so that maps of uneven sizes share the device and their tails
@@ -285,6 +318,56 @@ int htgx_render_batch
    uint32_t n,
    void* stream,
    double* scene_ms);
+
+/* A batch prepared once and rendered in repeated steps (no counterpart in the
+ * reference). Step s renders, for each item, the realisations
+ * [desc->spp_offset + s*spp, desc->spp_offset + (s+1)*spp) of every pixel
+ * and accumulates them into the item's accumulators: bit-identical to
+ * htgx_render_accum with that spp_offset. A step is six launches on
+ * `stream', with no allocation, host copy or synchronisation; with use_graph
+ * they are recorded once, at creation, as a HIP graph that each step
+ * replays. The plan keeps its own copy of the descriptors, chunk queues and
+ * partial sums: the contexts may draw other maps between steps, but must
+ * outlive the plan; steps of one plan must not overlap (same stream, or
+ * synchronised). */
+struct htgx_batch_plan;
+
+int htgx_batch_plan_create
+  (const struct htgx_batch_item* items,
+   uint32_t n,
+   int use_graph,
+   struct htgx_batch_plan** plan);
+
+/* The autotuner for a batch: the same search, each trial one batched launch
+ * of all items at 1/spp_div of their realisations per pixel (at least 4).
+ * Small trials (large spp_div) time mostly the tails of the maps, not their
+ * steady state: 8 represents a launch of whole maps. The best parameters are
+ * set on every item's context; call it before htgx_batch_plan_create, whose
+ * plan keeps the parameters it was built with. rate counts the trial's
+ * realisations. Synchronous. */
+int htgx_batch_autotune
+  (const struct htgx_batch_item* items,
+   uint32_t n,
+   uint32_t spp_div,
+   double budget_ms,
+   struct htgx_wf_params* best,
+   double* rate,
+   uint32_t* ntrials);
+
+/* Enqueue the next step on `stream' (asynchronous), on the plan's device */
+int htgx_batch_plan_step(struct htgx_batch_plan* plan, void* stream);
+
+/* Back to step 0 (same realisations again; completion times restart).
+ * Synchronises the device. */
+int htgx_batch_plan_rewind(struct htgx_batch_plan* plan);
+
+/* Completion time of each item in its last step, in ms after the start of
+ * the first step since creation or rewind (HOST array of n, in item order).
+ * Synchronises the device. */
+int htgx_batch_plan_scene_ms(struct htgx_batch_plan* plan, double* scene_ms);
+
+/* Waits for the device, then frees the plan */
+int htgx_batch_plan_destroy(struct htgx_batch_plan* plan);
 
 /* Fold device accumulators into reference pixel structs (host memory). For
  * a longwave camera, the brightness temperature and its error are computed as

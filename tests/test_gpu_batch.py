@@ -1,7 +1,8 @@
-"""Batched rendering (htgx_render_batch): several maps of distinct scenes in
-one persistent launch give bit-identical accumulators to separate
-htgx_render_accum calls (same random streams, same chunk order), except the
-wall-clock time sums."""
+"""Batched rendering (htgx_render_batch, htgx_batch_plan_*): several maps of
+distinct scenes in one persistent launch give bit-identical accumulators to
+separate htgx_render_accum calls (same random streams, same chunk order),
+except the wall-clock time sums; so does every step of a batch plan, replayed
+from a HIP graph or launched directly."""
 import ctypes as C
 
 import numpy as np
@@ -59,3 +60,68 @@ def test_batch_bad_arguments(batch):
     arr[1].ctx, arr[1].desc = ctxs[1].handle, C.pointer(d_flux)
     assert lib.htgx_render_batch(arr, 2, None, None) == F.BAD_ARG
     assert lib.htgx_render_batch(arr, 0, None, None) == F.BAD_ARG
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_plan_steps_equal_sliced_launches(batch, use_graph):
+    """Each step of a plan (a replayed HIP graph, or direct launches) equals
+    htgx_render_accum of the next spp slice, after every step, with other
+    launches of the same contexts between the steps."""
+    items, ctxs = batch
+    npx = [d.width * d.height for _, d in items]
+    plan_bufs = [H.DeviceBuffer(n * A.ACCUM_DTYPE.itemsize) for n in npx]
+    ref_bufs = [H.DeviceBuffer(n * A.ACCUM_DTYPE.itemsize) for n in npx]
+    for b in plan_bufs + ref_bufs:
+        b.zero()
+    plan = A.BatchPlan([(c, d, b.address) for c, (_, d), b in zip(ctxs, items, plan_bufs)],
+                       use_graph=use_graph)
+    try:
+        for step in range(3):
+            plan.step()
+            H.synchronize()
+            for c, (_, d), b in zip(ctxs, items, ref_bufs):
+                d.spp_offset = step * d.spp
+                c.render_accum(d, b.address)
+                d.spp_offset = 0
+            H.synchronize()
+            for (_, d), n, pb, rb in zip(items, npx, plan_bufs, ref_bufs):
+                got = pb.download(np.zeros(n, A.ACCUM_DTYPE))
+                ref = rb.download(np.zeros(n, A.ACCUM_DTYPE))
+                assert np.all(ref["n"] == (step + 1) * d.spp)
+                for k in EXACT:
+                    assert np.array_equal(got[k], ref[k]), (step, k)
+        ms = plan.scene_ms()
+        assert np.all(ms > 0)
+        # rewind: step 0 again, from zeroed accumulators
+        plan.rewind()
+        for b in plan_bufs:
+            b.zero()
+        plan.step()
+        ms0 = plan.scene_ms()
+        assert np.all(ms0 > 0)
+        for c, (_, d), n, pb in zip(ctxs, items, npx, plan_bufs):
+            rb = H.DeviceBuffer(n * A.ACCUM_DTYPE.itemsize)
+            rb.zero()
+            c.render_accum(d, rb.address)
+            H.synchronize()
+            got = pb.download(np.zeros(n, A.ACCUM_DTYPE))
+            ref = rb.download(np.zeros(n, A.ACCUM_DTYPE))
+            for k in EXACT:
+                assert np.array_equal(got[k], ref[k]), k
+    finally:
+        plan.close()
+
+
+def test_plan_bad_arguments(batch):
+    items, ctxs = batch
+    lib = F.load()
+    buf = H.DeviceBuffer(512 * 512 * A.ACCUM_DTYPE.itemsize)
+    (_, d0) = items[0]
+    arr = (F.BatchItem * 2)()
+    for i in range(2):
+        arr[i].ctx, arr[i].desc, arr[i].accum = ctxs[0].handle, C.pointer(d0), C.c_void_p(buf.address)
+    h = C.c_void_p()
+    assert lib.htgx_batch_plan_create(arr, 2, 1, C.byref(h)) == F.BAD_ARG and not h.value
+    assert lib.htgx_batch_plan_create(arr, 0, 1, C.byref(h)) == F.BAD_ARG
+    assert lib.htgx_batch_plan_step(None, None) == F.BAD_ARG
+    assert lib.htgx_batch_plan_destroy(None) == F.BAD_ARG
