@@ -45,6 +45,10 @@ class PixelFlux(C.Structure):  # struct atmosphere_pixel_flux
     _fields_ = [("flux", Accum), ("time", Accum)]
 
 
+class PixelImage(C.Structure):  # struct atmosphere_pixel_image (CIE XYZ)
+    _fields_ = [("X", Estimate), ("Y", Estimate), ("Z", Estimate), ("time", Accum)]
+
+
 class McAccum(C.Structure):
     _fields_ = [("sum", C.c_double), ("sum_c", C.c_double), ("sum2", C.c_double),
                 ("sum2_c", C.c_double), ("tsum", C.c_double), ("tsum2", C.c_double),

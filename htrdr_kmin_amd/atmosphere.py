@@ -23,11 +23,28 @@ FLUX_DTYPE = np.dtype([("flux_sum", "<f8"), ("flux_sum2", "<f8"),
                        ("flux_n", "<u8"), ("flux_nfail", "<u8"),
                        ("time_sum", "<f8"), ("time_sum2", "<f8"),
                        ("time_n", "<u8"), ("time_nfail", "<u8")])
+IMAGE_DTYPE = np.dtype([("X_E", "<f8"), ("X_SE", "<f8"), ("Y_E", "<f8"), ("Y_SE", "<f8"),
+                        ("Z_E", "<f8"), ("Z_SE", "<f8"),
+                        ("time_sum", "<f8"), ("time_sum2", "<f8"),
+                        ("time_n", "<u8"), ("time_nfail", "<u8")])
 ACCUM_DTYPE = np.dtype([("sum", "<f8"), ("sum_c", "<f8"), ("sum2", "<f8"),
                         ("sum2_c", "<f8"), ("tsum", "<f8"), ("tsum2", "<f8"),
                         ("n", "<u8"), ("nfail", "<u8")])
 assert XWAVE_DTYPE.itemsize == C.sizeof(F.PixelXwave) == 64
 assert FLUX_DTYPE.itemsize == C.sizeof(F.PixelFlux) == 64
+assert IMAGE_DTYPE.itemsize == C.sizeof(F.PixelImage) == 80
+
+
+def pixel_dtype(desc):
+    """Reference pixel struct of a map (atmosphere_get_pixel_format)."""
+    if desc.sensor == F.SENSOR_RECTANGLE:
+        return FLUX_DTYPE
+    return IMAGE_DTYPE if desc.spectral_type == F.SPECTRAL_SW_CIE_XYZ else XWAVE_DTYPE
+
+
+def accum_planes(desc):
+    """Accumulator planes of a map: 3 for a CIE XYZ image (X, Y, Z), else 1."""
+    return 3 if desc.spectral_type == F.SPECTRAL_SW_CIE_XYZ else 1
 assert ACCUM_DTYPE.itemsize == C.sizeof(F.McAccum) == 64
 
 
@@ -109,7 +126,7 @@ class Context:
 
     def draw_map(self, desc):
         """Pixels of the map as a structured array of shape (height, width)."""
-        dt = FLUX_DTYPE if desc.sensor == F.SENSOR_RECTANGLE else XWAVE_DTYPE
+        dt = pixel_dtype(desc)
         out = np.zeros((desc.height, desc.width), dt)
         F.check("htgx_draw_map", self._lib.htgx_draw_map(
             self._h, C.byref(desc), out.ctypes.data_as(C.c_void_p), desc.width * dt.itemsize))
@@ -342,7 +359,7 @@ def finalize(desc, accum):
     """Reference pixel structs from a host copy of the device accumulators."""
     lib = F.load()
     accum = np.ascontiguousarray(accum, ACCUM_DTYPE)
-    dt = FLUX_DTYPE if desc.sensor == F.SENSOR_RECTANGLE else XWAVE_DTYPE
+    dt = pixel_dtype(desc)
     out = np.zeros((desc.height, desc.width), dt)
     F.check("htgx_finalize_pixels", lib.htgx_finalize_pixels(
         C.byref(desc), accum.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p),
@@ -392,12 +409,17 @@ def dump_buffer(pixels, desc, stream):
             acc[2] += n
 
     flux = desc.sensor == F.SENSOR_RECTANGLE
+    cie = not flux and desc.spectral_type == F.SPECTRAL_SW_CIE_XYZ
     for y in range(h):
         for x in range(w):
             p = pixels[y, x]
             if flux:
                 dump_accum(p["flux_sum"], p["flux_sum2"], int(p["flux_n"]), facc)
                 stream.write("0 0 0 0 ")
+            elif cie:
+                # dump_pixel_image, ..._draw_map.c:516-527
+                for c in "XYZ":
+                    stream.write("%s %s " % (_fmt_g(float(p[c + "_E"])), _fmt_g(float(p[c + "_SE"]))))
             else:
                 # ..._draw_map.c:536-540: "%g %g %f %f 0 0 "; the temperature
                 # columns are written 0 0 for shortwave maps (tile memory is

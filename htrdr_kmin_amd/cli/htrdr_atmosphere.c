@@ -16,7 +16,7 @@
  *   -g plane | synth             an infinite plane (the quad repeated with -R)
  *                                or the configuration's fBm heightfield
  * Options without a counterpart on the GPU path are rejected with an error
- * (-d, -P, -r, -s cie_xyz) or accepted and ignored (-t, -V, -O, -n).
+ * (-d, -P, -r) or accepted and ignored (-t, -V, -O, -n).
  * The solver is the FP32 wavefront one; HTGX_SOLVER=plain selects the FP64
  * plain one.
  */
@@ -159,7 +159,13 @@ image_param(const char* key, const char* val, struct args* a)
 static int
 spectral_param(const char* key, const char* val, struct args* a)
 {
-  if(!strcmp(key, "cie_xyz")) { a->spectral = HTGX_SPECTRAL_SW_CIE_XYZ; return 0; }
+  if(!strcmp(key, "cie_xyz")) {
+    /* the CIE 1931 colour space, src/core/htrdr_args.c:451-455 */
+    a->spectral = HTGX_SPECTRAL_SW_CIE_XYZ;
+    a->wlen[0] = 380;
+    a->wlen[1] = 780;
+    return 0;
+  }
   if(!val) return -1;
   if(!strcmp(key, "sw") || !strcmp(key, "lw")) {
     a->spectral = key[0] == 's' ? HTGX_SPECTRAL_SW : HTGX_SPECTRAL_LW;
@@ -183,6 +189,7 @@ parse_args(int argc, char** argv, struct args* a)
   a->rtgt[2] = 1; a->rup[1] = 1; a->rsz[0] = a->rsz[1] = 1;
   a->def[0] = 320; a->def[1] = 240; a->spp = 1;
   a->spectral = HTGX_SPECTRAL_SW_CIE_XYZ;
+  a->wlen[0] = 380; a->wlen[1] = 780;
   a->tref = -1;
   a->sun_el = 90;
   a->thickness = 1;
@@ -277,6 +284,13 @@ dump(FILE* out, const struct args* a, const void* pixels, unsigned w, unsigned h
           fprintf(out, "0 0 ");
         }
         fprintf(out, "0 0 0 0 ");
+        t = &p->time;
+      } else if(a->spectral == HTGX_SPECTRAL_SW_CIE_XYZ) {
+        /* dump_pixel_image, ..._draw_map.c:516-527 */
+        const struct htgx_pixel_image* p = (const struct htgx_pixel_image*)pixels + (size_t)y * w + x;
+        fprintf(out, "%g %g ", p->X.E, p->X.SE);
+        fprintf(out, "%g %g ", p->Y.E, p->Y.SE);
+        fprintf(out, "%g %g ", p->Z.E, p->Z.SE);
         t = &p->time;
       } else {
         const struct htgx_pixel_xwave* p = (const struct htgx_pixel_xwave*)pixels + (size_t)y * w + x;
@@ -389,7 +403,8 @@ main(int argc, char** argv)
     memcpy(d.tgt, a.tgt, sizeof(d.tgt));
     memcpy(d.up, a.up, sizeof(d.up));
     d.fov_y = a.fov;
-    elmt = sizeof(struct htgx_pixel_xwave);
+    elmt = a.spectral == HTGX_SPECTRAL_SW_CIE_XYZ
+      ? sizeof(struct htgx_pixel_image) : sizeof(struct htgx_pixel_xwave);
   }
 
   if(a.output) {
@@ -418,7 +433,8 @@ main(int argc, char** argv)
   if(a.verbose) {
     const double s = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
     fprintf(stderr, "Rendering time: %.3f s (%.3f Mrealisations/s)\n", s,
-            (double)d.width * d.height * d.spp / s * 1e-6);
+            (double)d.width * d.height * d.spp * (a.spectral == HTGX_SPECTRAL_SW_CIE_XYZ ? 3 : 1)
+              / s * 1e-6);
   }
   dump(out, &a, pixels, d.width, d.height, &tacc, &facc);
   {

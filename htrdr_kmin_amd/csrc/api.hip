@@ -457,6 +457,7 @@ htgx_ctx_destroy(htgx_ctx* ctx)
   if(ctx->d_partials) (void)hipFree(ctx->d_partials);
   if(ctx->d_wcdf) (void)hipFree(ctx->d_wcdf);
   if(ctx->d_wpdf) (void)hipFree(ctx->d_wpdf);
+  if(ctx->d_cie_cdf) (void)hipFree(ctx->d_cie_cdf);
   if(ctx->d_counters) (void)hipFree(ctx->d_counters);
   htgx_wavefront_release(ctx);
   delete ctx;
@@ -500,12 +501,11 @@ htgx_map_setup(htgx_ctx* ctx, const htgx_map_desc* d, DevMap* M)
   if(!d->width || !d->height || !d->spp) return HTGX_BAD_ARG;
   if(d->width > 16384 || d->height > 16384) return HTGX_BAD_ARG; /* htrdr_args.c:61 */
   if(d->sensor != HTGX_SENSOR_CAMERA && d->sensor != HTGX_SENSOR_RECTANGLE) return HTGX_BAD_ARG;
-  if(d->spectral_type == HTGX_SPECTRAL_SW_CIE_XYZ) {
-    /* CIE XYZ flux maps are invalid (src/atmosphere/htrdr_atmosphere.c:189-194);
-     * CIE XYZ images (draw_pixel_image) are not built */
-    return d->sensor == HTGX_SENSOR_RECTANGLE ? HTGX_BAD_ARG : HTGX_BAD_OP;
-  }
-  if(d->spectral_type != HTGX_SPECTRAL_SW && d->spectral_type != HTGX_SPECTRAL_LW)
+  /* CIE XYZ flux maps are invalid (src/atmosphere/htrdr_atmosphere.c:189-194) */
+  if(d->spectral_type == HTGX_SPECTRAL_SW_CIE_XYZ && d->sensor != HTGX_SENSOR_CAMERA)
+    return HTGX_BAD_ARG;
+  if(d->spectral_type != HTGX_SPECTRAL_SW && d->spectral_type != HTGX_SPECTRAL_LW
+  && d->spectral_type != HTGX_SPECTRAL_SW_CIE_XYZ)
     return HTGX_BAD_ARG;
   if(d->spectral_type == HTGX_SPECTRAL_LW && !ctx->has_temperature) return HTGX_BAD_ARG;
   if(!(d->wlen_range[1] >= d->wlen_range[0])) return HTGX_BAD_ARG;
@@ -556,6 +556,56 @@ htgx_map_setup(htgx_ctx* ctx, const htgx_map_desc* d, DevMap* M)
   M->range[0] = range[0];
   M->range[1] = range[1];
   M->Tref = Tref;
+  if(d->spectral_type == HTGX_SPECTRAL_SW_CIE_XYZ) {
+    /* htrdr_ran_wlen_cie_xyz_create (src/core/htrdr_ran_wlen_cie_xyz.c:
+     * 288-329): 1-nm bands within [380, 780] nm */
+    if(range[0] < 380.0 || range[1] > 780.0 || !nwb) return HTGX_BAD_ARG;
+    M->spectral_type = HTGX_SPECTRAL_SW;
+    M->wmode = 3;
+    M->band_len = (range[1] - range[0]) / (double)nwb;
+    M->nwbands = (uint32_t)nwb;
+    const bool cached = ctx->d_cie_cdf && ctx->cie_nbands == nwb
+      && ctx->cie_range[0] == range[0] && ctx->cie_range[1] == range[1];
+    if(!cached) {
+      /* setup_cie_xyz, :183-270: per band the trapezoidal integral of each
+       * fit over 2 sub-intervals per nm (trapezoidal_integration, :55-78) */
+      std::vector<double> cdf(3 * nwb);
+      for(int c = 0; c < 3; ++c) {
+        double* f = cdf.data() + (size_t)c * nwb;
+        double sum = 0;
+        for(size_t i = 0; i < nwb; ++i) {
+          const double lo = range[0] + (double)i * M->band_len;
+          const double hi = std::min(lo + M->band_len, range[1]);
+          const size_t n = (size_t)(hi - lo) + 1;
+          const double dl = (hi - lo) / (double)n;
+          double integral = 0;
+          for(size_t k = 0; k < n; ++k) {
+            const double f1 = cie_fit(c, lo + dl * (double)k);
+            const double f2 = cie_fit(c, lo + dl * (double)(k + 1));
+            integral += (f1 + f2) * dl * 0.5;
+          }
+          f[i] = integral;
+          sum += integral;
+        }
+        ctx->cie_rcp[c] = 1.0 / sum;
+        for(size_t i = 0; i < nwb; ++i) {
+          f[i] /= sum;
+          if(i) f[i] += f[i - 1];
+        }
+        f[nwb - 1] = 1.0;
+      }
+      if(ctx->d_cie_cdf) { (void)hipFree(ctx->d_cie_cdf); ctx->d_cie_cdf = nullptr; }
+      ctx->cie_nbands = 0;
+      HIP_OK(hipMalloc(&ctx->d_cie_cdf, 3 * nwb * sizeof(double)));
+      HIP_OK(hipMemcpy(ctx->d_cie_cdf, cdf.data(), 3 * nwb * sizeof(double), hipMemcpyHostToDevice));
+      ctx->cie_nbands = nwb;
+      ctx->cie_range[0] = range[0];
+      ctx->cie_range[1] = range[1];
+    }
+    *M = htgx_cie_channel_map(ctx, *M, 0);
+    M->seed = d->seed; /* channel maps derive their seeds from this one */
+    return HTGX_OK;
+  }
   if(nwb) {
     M->wmode = 1;
     const bool cached = ctx->d_wcdf && ctx->w_nbands == nwb && ctx->w_range[0] == range[0]
@@ -596,6 +646,17 @@ htgx_map_setup(htgx_ctx* ctx, const htgx_map_desc* d, DevMap* M)
     M->wmode = 2;
   }
   return HTGX_OK;
+}
+
+DevMap htgx_cie_channel_map(const htgx_ctx* ctx, const DevMap& M, int c)
+{
+  DevMap C = M;
+  C.cie_channel = c;
+  C.cie_rcp = ctx->cie_rcp[c];
+  C.wcdf = ctx->d_cie_cdf + (size_t)c * ctx->cie_nbands;
+  C.wpdf = nullptr;
+  C.seed = M.seed ^ ((uint64_t)(c + 1) << 56);
+  return C;
 }
 
 /*******************************************************************************
@@ -647,9 +708,22 @@ htgx_render_accum(htgx_ctx* ctx, const htgx_map_desc* desc, htgx_mc_accum* accum
   int res;
   if(!ctx || !desc || !accum) return HTGX_BAD_ARG;
   if((res = htgx_map_setup(ctx, desc, &M)) != HTGX_OK) return res;
-  if(desc->solver == HTGX_SOLVER_WAVEFRONT_FP32)
-    return htgx_render_wavefront(ctx, M, accum, (hipStream_t)stream);
-  return render_plain(ctx, M, accum, (hipStream_t)stream);
+  /* a CIE XYZ image is 3 maps, one per channel, into 3 accumulator planes
+   * (draw_pixel_image, src/atmosphere/htrdr_atmosphere_draw_map.c:134-199) */
+  const int nch = desc->spectral_type == HTGX_SPECTRAL_SW_CIE_XYZ ? 3 : 1;
+  const size_t npix = (size_t)desc->width * desc->height;
+  uint64_t nreal = 0;
+  for(int c = 0; c < nch; ++c) {
+    const DevMap Mc = nch == 3 ? htgx_cie_channel_map(ctx, M, c) : M;
+    htgx_mc_accum* a = accum + (size_t)c * npix;
+    res = desc->solver == HTGX_SOLVER_WAVEFRONT_FP32
+      ? htgx_render_wavefront(ctx, Mc, a, (hipStream_t)stream)
+      : render_plain(ctx, Mc, a, (hipStream_t)stream);
+    if(res != HTGX_OK) return res;
+    nreal += ctx->last_nrealisations;
+  }
+  ctx->last_nrealisations = nreal;
+  return HTGX_OK;
 }
 
 extern "C" int
@@ -695,6 +769,8 @@ batch_setup(const htgx_batch_item* items, uint32_t n, std::vector<DevMap>& Ms,
     if(it.desc->sensor != items[0].desc->sensor
     || it.desc->spectral_type != items[0].desc->spectral_type)
       return HTGX_BAD_ARG;
+    /* a CIE XYZ image is 3 maps: batch its channels as separate items */
+    if(it.desc->spectral_type == HTGX_SPECTRAL_SW_CIE_XYZ) return HTGX_BAD_OP;
     if((res = htgx_map_setup(it.ctx, it.desc, &Ms[i])) != HTGX_OK) return res;
     ctxs[i] = it.ctx;
     accums[i] = it.accum;
@@ -839,13 +915,43 @@ radiance_temperature(double lmin, double lmax, double radiance)
   return T;
 }
 
+/* Size of the reference pixel struct of a map (atmosphere_get_pixel_format,
+ * src/atmosphere/htrdr_atmosphere.c:482-512) */
+static size_t pixel_size(const htgx_map_desc* d)
+{
+  if(d->sensor == HTGX_SENSOR_RECTANGLE) return sizeof(htgx_pixel_flux);
+  if(d->spectral_type == HTGX_SPECTRAL_SW_CIE_XYZ) return sizeof(htgx_pixel_image);
+  return sizeof(htgx_pixel_xwave);
+}
+
 extern "C" int
 htgx_finalize_pixels(const htgx_map_desc* d, const htgx_mc_accum* acc, void* pixels, size_t pitch)
 {
   if(!d || !acc || !pixels || !d->width || !d->height) return HTGX_BAD_ARG;
-  const size_t elmt = d->sensor == HTGX_SENSOR_RECTANGLE
-    ? sizeof(htgx_pixel_flux) : sizeof(htgx_pixel_xwave);
+  const size_t elmt = pixel_size(d);
   if(pitch < elmt * d->width) return HTGX_BAD_ARG;
+  if(d->spectral_type == HTGX_SPECTRAL_SW_CIE_XYZ) {
+    /* draw_pixel_image, src/atmosphere/htrdr_atmosphere_draw_map.c:201-206:
+     * one estimate per channel, the time over all realisations */
+    const size_t npix = (size_t)d->width * d->height;
+    for(uint32_t y = 0; y < d->height; ++y) {
+      char* row = (char*)pixels + (size_t)y * pitch;
+      for(uint32_t x = 0; x < d->width; ++x) {
+        htgx_pixel_image* p = (htgx_pixel_image*)(row + x * elmt);
+        htgx_estimate* est[3] = {&p->X, &p->Y, &p->Z};
+        p->time.sum_weights = p->time.sum_weights_sqr = 0;
+        p->time.nweights = p->time.nfailures = 0;
+        for(int c = 0; c < 3; ++c) {
+          const htgx_mc_accum& a = acc[(size_t)c * npix + (size_t)y * d->width + x];
+          estimate(a.sum + a.sum_c, a.sum2 + a.sum2_c, a.n, &est[c]->E, &est[c]->SE);
+          p->time.sum_weights += a.tsum;
+          p->time.sum_weights_sqr += a.tsum2;
+          p->time.nweights += (size_t)a.n;
+        }
+      }
+    }
+    return HTGX_OK;
+  }
   for(uint32_t y = 0; y < d->height; ++y) {
     char* row = (char*)pixels + (size_t)y * pitch;
     for(uint32_t x = 0; x < d->width; ++x) {
@@ -888,10 +994,10 @@ htgx_draw_map(htgx_ctx* ctx, const htgx_map_desc* desc, void* pixels, size_t pit
 {
   if(!ctx || !desc || !pixels) return HTGX_BAD_ARG;
   if(!desc->width || !desc->height || !desc->spp) return HTGX_BAD_ARG;
-  const size_t elmt = desc->sensor == HTGX_SENSOR_RECTANGLE
-    ? sizeof(htgx_pixel_flux) : sizeof(htgx_pixel_xwave);
+  const size_t elmt = pixel_size(desc);
   if(pitch < elmt * desc->width) return HTGX_BAD_ARG;
-  const size_t npix = (size_t)desc->width * desc->height;
+  const size_t npix = (size_t)desc->width * desc->height
+    * (desc->spectral_type == HTGX_SPECTRAL_SW_CIE_XYZ ? 3 : 1);
   htgx_mc_accum* d_acc = nullptr;
   HIP_OK(hipMalloc(&d_acc, npix * sizeof(htgx_mc_accum)));
   int res = HTGX_OK;
@@ -967,14 +1073,34 @@ htgx_debug_realisations(htgx_ctx* ctx, const htgx_map_desc* desc, const uint64_t
   DevMap M;
   int res;
   if((res = htgx_map_setup(ctx, desc, &M)) != HTGX_OK) return res;
-  DevArray<uint64_t> d_p, d_r;
-  DevArray<double> d_w;
-  if((res = d_p.upload(pix, n))) return res;
-  if((res = d_r.upload(real, n))) return res;
-  if((res = d_w.alloc(n))) return res;
-  realisations_kernel<<<(n + 63) / 64, 64>>>(ctx->S, M, d_p.p, d_r.p, n, d_w.p);
-  HIP_OK(hipGetLastError());
-  return d_w.download(weights, n);
+  const bool cie = desc->spectral_type == HTGX_SPECTRAL_SW_CIE_XYZ;
+  for(uint32_t i = 0; i < n; ++i)
+    if((real[i] >> 32) > (cie ? 2u : 0u)) return HTGX_BAD_ARG;
+  /* one launch per channel over the realisations of that channel */
+  for(int c = 0; c < (cie ? 3 : 1); ++c) {
+    std::vector<uint64_t> p, r;
+    std::vector<uint32_t> idx;
+    for(uint32_t i = 0; i < n; ++i)
+      if((int)(real[i] >> 32) == c) {
+        idx.push_back(i);
+        p.push_back(pix[i]);
+        r.push_back(real[i] & 0xFFFFFFFFull);
+      }
+    if(idx.empty()) continue;
+    const uint32_t m = (uint32_t)idx.size();
+    const DevMap Mc = cie ? htgx_cie_channel_map(ctx, M, c) : M;
+    DevArray<uint64_t> d_p, d_r;
+    DevArray<double> d_w;
+    if((res = d_p.upload(p.data(), m))) return res;
+    if((res = d_r.upload(r.data(), m))) return res;
+    if((res = d_w.alloc(m))) return res;
+    realisations_kernel<<<(m + 63) / 64, 64>>>(ctx->S, Mc, d_p.p, d_r.p, m, d_w.p);
+    HIP_OK(hipGetLastError());
+    std::vector<double> w(m);
+    if((res = d_w.download(w.data(), m))) return res;
+    for(uint32_t k = 0; k < m; ++k) weights[idx[k]] = w[k];
+  }
+  return HTGX_OK;
 }
 
 extern "C" int

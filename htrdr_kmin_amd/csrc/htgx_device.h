@@ -88,13 +88,16 @@ struct DevMap {
   double tan_half, aspect;
   double axis_x[3], axis_y[3];
   /* wavelength sampler (src/core/htrdr_ran_wlen_planck.c) */
-  int wmode; /* 0 monochromatic, 1 discrete CDF, 2 continuous bisection */
+  int wmode; /* 0 monochromatic, 1 discrete CDF, 2 continuous bisection,
+                3 CIE 1931 channel (piecewise-linear CDF inversion) */
   double range[2];
   double band_len;
   uint32_t nwbands;
   const double* wcdf;
   const double* wpdf;
   double Tref;
+  int cie_channel;  /* wmode 3: 0 X, 1 Y, 2 Z */
+  double cie_rcp;   /* wmode 3: 1 / integral of the channel's fit, nm^-1 */
   double us_per_tick; /* wall clock period in microseconds */
 };
 
@@ -201,9 +204,63 @@ HTGX_HD double planck_range(double lmin, double lmax, double T)
   return blackbody_fraction(lmin, lmax, T) * 5.6696e-8 * T4 / (kPi * (lmax - lmin));
 }
 
-/* htrdr_ran_wlen_planck_sample, src/core/htrdr_ran_wlen_planck.c:359-376 */
+/* The CIE 1931 colour matching functions as the sums of piecewise Gaussians
+ * of C. Wyman, P.-P. Sloan and P. Shirley, "Simple Analytic Approximations
+ * to the CIE XYZ Color Matching Functions", JCGT 2(2), 2013 -- the fits
+ * src/core/htrdr_ran_wlen_cie_xyz.c:80-107 integrates. A lobe has its peak
+ * wavelength and one inverse width on each side of it. */
+HTGX_HD double cie_lobe(double l, double peak, double inv_w_lo, double inv_w_hi)
+{
+  const double t = (l - peak) * (l < peak ? inv_w_lo : inv_w_hi);
+  return exp(-0.5 * t * t);
+}
+
+HTGX_HD double cie_fit(int channel, double l)
+{
+  if(channel == 0)
+    return 0.362 * cie_lobe(l, 442.0, 0.0624, 0.0374) + 1.056 * cie_lobe(l, 599.8, 0.0264, 0.0323)
+         - 0.065 * cie_lobe(l, 501.1, 0.0490, 0.0382);
+  if(channel == 1)
+    return 0.821 * cie_lobe(l, 568.8, 0.0213, 0.0247) + 0.286 * cie_lobe(l, 530.9, 0.0613, 0.0322);
+  return 1.217 * cie_lobe(l, 437.0, 0.0845, 0.0278) + 0.681 * cie_lobe(l, 459.0, 0.0385, 0.0725);
+}
+
+/* Wavelength within band [lmin, lmin + len[ whose density is linear between
+ * the channel's values at the band ends: root of the quadratic CDF equal to
+ * r1 (sample_cie_xyz, src/core/htrdr_ran_wlen_cie_xyz.c:109-181; the band
+ * middle when no root falls in the band) */
+HTGX_HD double cie_band_sample(int channel, double lmin, double len, double r1)
+{
+  const double lmax = lmin + len;
+  const double fmin = cie_fit(channel, lmin), fmax = cie_fit(channel, lmax);
+  const double a = 0.5 * (fmax - fmin) / len;
+  const double b = (lmax * fmin - lmin * fmax) / len;
+  const double c = -lmin * fmin + lmin * lmin * a;
+  const double d = 0.5 * (fmax + fmin) * len;
+  double delta = b * b - 4.0 * a * (c - d * r1);
+  if(delta < 0 && fabs(delta) <= 1e-6) delta = 0;
+  const double sq = sqrt(delta);
+  const double l1 = (-b - sq) / (2.0 * a), l2 = (-b + sq) / (2.0 * a);
+  if(lmin <= l1 && l1 < lmax) return l1;
+  if(lmin <= l2 && l2 < lmax) return l2;
+  return (lmin + lmax) * 0.5;
+}
+
+/* htrdr_ran_wlen_planck_sample, src/core/htrdr_ran_wlen_planck.c:359-376,
+ * and htrdr_ran_wlen_cie_xyz_sample_X/Y/Z, htrdr_ran_wlen_cie_xyz.c:346-397.
+ * The pdf is in nm^-1; for a CIE channel it is the constant 1 / integral of
+ * the channel, as the reference returns it. */
 HTGX_D double sample_wavelength(const DevMap& M, double r0, double r1, double* pdf)
 {
+  if(M.wmode == 3) {
+    uint32_t lo = 0, hi = M.nwbands - 1;
+    while(lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if(M.wcdf[mid] > r0) hi = mid; else lo = mid + 1;
+    }
+    *pdf = M.cie_rcp;
+    return cie_band_sample(M.cie_channel, M.range[0] + (double)lo * M.band_len, M.band_len, r1);
+  }
   if(M.wmode == 1) {
     uint32_t lo = 0, hi = M.nwbands - 1;
     while(lo < hi) {

@@ -35,6 +35,11 @@ struct htgx_ctx {
   size_t w_nbands = 0;
   double w_range[2] = {0, 0};
   double w_Tref = 0;
+  /* CIE XYZ channel CDFs of the last image ([3][cie_nbands], cached by range) */
+  double* d_cie_cdf = nullptr;
+  size_t cie_nbands = 0;
+  double cie_range[2] = {0, 0};
+  double cie_rcp[3] = {0, 0, 0};
   /* event counters */
   int counting = 0;
   unsigned long long* d_counters = nullptr; /* HTGX_NSTATS slots */
@@ -45,6 +50,9 @@ struct htgx_ctx {
 
 int htgx_ctx_init(htgx_ctx* ctx, const htgx_scene_desc& scene);
 int htgx_map_setup(htgx_ctx* ctx, const htgx_map_desc* desc, htgx::DevMap* map);
+/* The map of channel c (0 X, 1 Y, 2 Z) of a CIE XYZ image set up by
+ * htgx_map_setup: the channel's wavelength CDF and its own random streams */
+htgx::DevMap htgx_cie_channel_map(const htgx_ctx* ctx, const htgx::DevMap& M, int c);
 int htgx_render_wavefront(htgx_ctx* ctx, const htgx::DevMap& map, htgx_mc_accum* accum,
                           hipStream_t stream);
 /* Several maps of distinct contexts (same device, same sensor and spectral

@@ -54,8 +54,8 @@ enum htgx_res {
 enum htgx_spectral_type {
   HTGX_SPECTRAL_LW = 0,
   HTGX_SPECTRAL_SW = 1,
-  HTGX_SPECTRAL_SW_CIE_XYZ = 2 /* Not built: images return HTGX_BAD_OP, flux maps
-                                  HTGX_BAD_ARG as in the reference */
+  HTGX_SPECTRAL_SW_CIE_XYZ = 2 /* Shortwave CIE 1931 XYZ image (camera only: a flux
+                                  map returns HTGX_BAD_ARG, as in the reference) */
 };
 
 /* Radiance components, enum atmosphere_radiance_cpnt_flag
@@ -119,6 +119,15 @@ struct htgx_pixel_xwave {
 struct htgx_pixel_flux {
   struct htgx_accum flux; /* W/m^2 */
   struct htgx_accum time; /* microseconds per realisation */
+};
+
+/* struct atmosphere_pixel_image, src/atmosphere/htrdr_atmosphere_c.h:70-75:
+ * the pixel of a CIE XYZ image */
+struct htgx_pixel_image {
+  struct htgx_estimate X; /* W/m^2/sr */
+  struct htgx_estimate Y; /* W/m^2/sr */
+  struct htgx_estimate Z; /* W/m^2/sr */
+  struct htgx_accum time; /* microseconds per realisation, all 3 channels */
 };
 
 /* Device-side per-pixel Monte Carlo accumulator. The weight sums are FP64
@@ -244,8 +253,10 @@ int htgx_ctx_get_svo_info
  * atmosphere_draw_map (src/atmosphere/htrdr_atmosphere_draw_map.c:593-616):
  * renders desc->spp realisations per pixel and writes one pixel struct per
  * pixel, row-major, into the caller's HOST memory `pixels' whose rows are
- * `pitch' bytes apart. The pixel struct is htgx_pixel_xwave for a camera and
- * htgx_pixel_flux for a rectangle. Returns HTGX_BAD_ARG on the conditions
+ * `pitch' bytes apart. The pixel struct is htgx_pixel_image for a CIE XYZ
+ * camera (spp realisations per channel, draw_pixel_image, ..._draw_map.c:
+ * 110-207), htgx_pixel_xwave for another camera and htgx_pixel_flux for a
+ * rectangle. Returns HTGX_BAD_ARG on the conditions
  * of check_draw_map_args (src/core/htrdr_draw_map.c:66-79): null spp,
  * null definition, pitch smaller than a row. */
 int htgx_draw_map
@@ -256,8 +267,10 @@ int htgx_draw_map
 
 /* Device-resident variant for callers that keep the accumulators in HBM and
  * render in several calls (realisation slices [spp_offset, spp_offset+spp)).
- * `accum' is a DEVICE pointer to width*height struct htgx_mc_accum, row-major;
- * `stream' is a hipStream_t (NULL = default stream). Asynchronous. */
+ * `accum' is a DEVICE pointer to width*height struct htgx_mc_accum, row-major
+ * (for a CIE XYZ image: 3 such planes, X then Y then Z; channel c draws from
+ * streams keyed by seed ^ (c+1) << 56); `stream' is a hipStream_t (NULL =
+ * default stream). Asynchronous. */
 int htgx_render_accum
   (struct htgx_ctx* ctx,
    const struct htgx_map_desc* desc,
@@ -304,8 +317,9 @@ int htgx_autotune
  * (same random streams, same chunk order: bit-identical accumulators). */
 struct htgx_batch_item {
   struct htgx_ctx* ctx;              /* distinct contexts on the current device */
-  const struct htgx_map_desc* desc;  /* same sensor and spectral type for all; the
-                                        solver field is ignored (FP32 wavefront) */
+  const struct htgx_map_desc* desc;  /* same sensor and spectral type for all, not
+                                        CIE XYZ (HTGX_BAD_OP); the solver field
+                                        is ignored (FP32 wavefront) */
   struct htgx_mc_accum* accum;       /* DEVICE accumulators of the map, as in
                                         htgx_render_accum */
 };
@@ -371,7 +385,8 @@ int htgx_batch_plan_destroy(struct htgx_batch_plan* plan);
 
 /* Fold device accumulators into reference pixel structs (host memory). For
  * a longwave camera, the brightness temperature and its error are computed as
- * in draw_pixel_xwave (src/atmosphere/htrdr_atmosphere_draw_map.c:423-435). */
+ * in draw_pixel_xwave (src/atmosphere/htrdr_atmosphere_draw_map.c:423-435);
+ * a CIE XYZ image takes its 3 accumulator planes. */
 int htgx_finalize_pixels
   (const struct htgx_map_desc* desc,
    const struct htgx_mc_accum* accum_host, /* HOST copy of the accumulators */
@@ -433,7 +448,8 @@ int htgx_debug_trace_segments
 
 /* Weight of individual realisations (pixel index y*W+x, realisation index),
  * computed by the solver desc->solver. For the plain FP64 solver these use the
- * same random streams as htgx_draw_map. HOST arrays. */
+ * same random streams as htgx_draw_map. For a CIE XYZ image, bits 32-33 of
+ * the realisation index select the channel (0 X, 1 Y, 2 Z). HOST arrays. */
 int htgx_debug_realisations
   (struct htgx_ctx* ctx,
    const struct htgx_map_desc* desc,

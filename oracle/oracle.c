@@ -267,6 +267,8 @@ struct oracle_wlen_planck {
   size_t nbands;
   double* pdf;
   double* cdf;
+  int cie;        /* -1: Planck; 0, 1, 2: CIE 1931 X, Y, Z channel */
+  double cie_rcp; /* 1 / integral of the CIE channel, nm^-1 */
 };
 
 /* src/atmosphere/htrdr_atmosphere.c:53-70 */
@@ -289,6 +291,7 @@ oracle_wlen_planck_create
   p->range[1] = range[1];
   p->Tref = Tref;
   p->nbands = nbands;
+  p->cie = -1;
   if(nbands) {
     p->band_len = (range[1] - range[0]) / (double)nbands;
     p->pdf = malloc(nbands * sizeof(double));
@@ -317,6 +320,121 @@ oracle_wlen_planck_destroy(struct oracle_wlen_planck* p)
   free(p->pdf);
   free(p->cdf);
   free(p);
+}
+
+/*******************************************************************************
+ * CIE 1931 XYZ wavelength sampler: src/core/htrdr_ran_wlen_cie_xyz.c
+ ******************************************************************************/
+/* The colour matching functions as fitted by Wyman, Sloan and Shirley
+ * (JCGT 2(2), 2013), :80-107 */
+static double
+cie_fit(int channel, double l)
+{
+  double a, b, c;
+  switch(channel) {
+    case 0:
+      a = (l - 442.0) * (l < 442.0 ? 0.0624 : 0.0374);
+      b = (l - 599.8) * (l < 599.8 ? 0.0264 : 0.0323);
+      c = (l - 501.1) * (l < 501.1 ? 0.0490 : 0.0382);
+      return 0.362*exp(-0.5*a*a) + 1.056*exp(-0.5*b*b) - 0.065*exp(-0.5*c*c);
+    case 1:
+      a = (l - 568.8) * (l < 568.8 ? 0.0213 : 0.0247);
+      b = (l - 530.9) * (l < 530.9 ? 0.0613 : 0.0322);
+      return 0.821*exp(-0.5*a*a) + 0.286*exp(-0.5*b*b);
+    default:
+      a = (l - 437.0) * (l < 437.0 ? 0.0845 : 0.0278);
+      b = (l - 459.0) * (l < 459.0 ? 0.0385 : 0.0725);
+      return 1.217*exp(-0.5*a*a) + 0.681*exp(-0.5*b*b);
+  }
+}
+
+double
+oracle_cie_fit(int channel, double lambda_nm)
+{
+  return cie_fit(channel, lambda_nm);
+}
+
+/* trapezoidal_integration, :55-78: (hi - lo) + 1 sub-intervals */
+static double
+cie_integral(int channel, double lo, double hi)
+{
+  const size_t n = (size_t)(hi - lo) + 1;
+  const double dl = (hi - lo) / (double)n;
+  double sum = 0;
+  size_t i;
+  for(i = 0; i < n; ++i) {
+    const double f1 = cie_fit(channel, lo + dl*(double)i);
+    const double f2 = cie_fit(channel, lo + dl*(double)(i+1));
+    sum += (f1 + f2)*dl*0.5;
+  }
+  return sum;
+}
+
+/* htrdr_ran_wlen_cie_xyz_create + setup_cie_xyz, :183-329, for one channel */
+int
+oracle_wlen_cie_create
+  (const double range[2], size_t nbands, int channel, struct oracle_wlen_planck** out)
+{
+  struct oracle_wlen_planck* p;
+  double sum = 0;
+  size_t i;
+  if(!nbands || channel < 0 || channel > 2 || range[0] < 380 || range[1] > 780
+  || !(range[0] < range[1]))
+    return HTGX_BAD_ARG;
+  p = calloc(1, sizeof(*p));
+  if(!p) return HTGX_MEM_ERR;
+  p->range[0] = range[0];
+  p->range[1] = range[1];
+  p->nbands = nbands;
+  p->cie = channel;
+  p->band_len = (range[1] - range[0]) / (double)nbands;
+  p->pdf = malloc(nbands * sizeof(double));
+  p->cdf = malloc(nbands * sizeof(double));
+  if(!p->pdf || !p->cdf) { oracle_wlen_planck_destroy(p); return HTGX_MEM_ERR; }
+  for(i = 0; i < nbands; ++i) {
+    const double lo = range[0] + (double)i * p->band_len;
+    const double hi = fmin(lo + p->band_len, range[1]);
+    p->pdf[i] = cie_integral(channel, lo, hi);
+    sum += p->pdf[i];
+  }
+  p->cie_rcp = 1.0 / sum;
+  for(i = 0; i < nbands; ++i) {
+    p->pdf[i] /= sum;
+    p->cdf[i] = i ? p->pdf[i] + p->cdf[i-1] : p->pdf[i];
+  }
+  p->cdf[nbands-1] = 1.0;
+  *out = p;
+  return HTGX_OK;
+}
+
+/* sample_cie_xyz, :109-181: the band whose cumulative first exceeds r0, then
+ * the root in the band of the quadratic CDF of the linear density through
+ * the fit's values at the band ends; the band middle if no root is in it */
+static double
+cie_sample(const struct oracle_wlen_planck* p, double r0, double r1)
+{
+  size_t lo = 0, hi = p->nbands - 1;
+  double lmin, lmax, fmin, fmax, a, b, c, d, delta, sq, l1, l2;
+  while(lo < hi) {
+    const size_t mid = (lo + hi) / 2;
+    if(p->cdf[mid] > r0) hi = mid; else lo = mid + 1;
+  }
+  lmin = p->range[0] + p->band_len * (double)lo;
+  lmax = lmin + p->band_len;
+  fmin = cie_fit(p->cie, lmin);
+  fmax = cie_fit(p->cie, lmax);
+  a = 0.5 * (fmax - fmin) / p->band_len;
+  b = (lmax * fmin - lmin * fmax) / p->band_len;
+  c = -lmin * fmin + lmin*lmin * a;
+  d = 0.5 * (fmax + fmin) * p->band_len;
+  delta = b*b - 4*a*(c - d*r1);
+  if(delta < 0 && fabs(delta) <= 1.e-6) delta = 0;
+  sq = sqrt(delta);
+  l1 = (-b - sq) / (2*a);
+  l2 = (-b + sq) / (2*a);
+  if(lmin <= l1 && l1 < lmax) return l1;
+  if(lmin <= l2 && l2 < lmax) return l2;
+  return (lmin + lmax) * 0.5;
 }
 
 /* Bisection on the black-body fraction, src/core/htrdr_ran_wlen_planck.c:136-216 */
@@ -348,6 +466,10 @@ double
 oracle_wlen_planck_sample
   (const struct oracle_wlen_planck* p, double r0, double r1, double* pdf)
 {
+  if(p->cie >= 0) {
+    if(pdf) *pdf = p->cie_rcp;
+    return cie_sample(p, r0, r1);
+  }
   if(p->nbands) {
     /* first band whose cumulative is strictly greater than r0 */
     size_t lo = 0, hi = p->nbands - 1;
@@ -1675,6 +1797,26 @@ realisation
 
 static int
 make_planck(const struct oracle_sky* s, const struct htgx_map_desc* d,
+            struct oracle_wlen_planck** p);
+
+/* The desc and sampler of channel c of a CIE XYZ image: the channel's CDF
+ * and its own random streams, keyed by seed ^ (c+1) << 56 */
+static int
+cie_channel(const struct oracle_sky* s, const struct htgx_map_desc* d, int c,
+            struct htgx_map_desc* dc, struct oracle_wlen_planck** p)
+{
+  const uint32_t nb = s->scene.spectral.nbands;
+  double range[2];
+  range[0] = fmax(d->wlen_range[0], s->scene.spectral.band_lo[0]);
+  range[1] = fmin(d->wlen_range[1], s->scene.spectral.band_hi[nb - 1]);
+  *dc = *d;
+  dc->seed = d->seed ^ ((uint64_t)(c + 1) << 56);
+  dc->spectral_type = HTGX_SPECTRAL_SW;
+  return oracle_wlen_cie_create(range, oracle_spectral_bands_count(range), c, p);
+}
+
+static int
+make_planck(const struct oracle_sky* s, const struct htgx_map_desc* d,
             struct oracle_wlen_planck** p)
 {
   /* src/atmosphere/htrdr_atmosphere.c:384-415: the range is clamped to the
@@ -1699,11 +1841,19 @@ oracle_realisation
 {
   struct sensor sn;
   struct oracle_wlen_planck* planck = NULL;
+  struct htgx_map_desc dc = *d;
   int rejected;
   double w;
   if(sensor_setup(d, &sn) != HTGX_OK) return NAN;
-  if(make_planck(s, d, &planck) != HTGX_OK) return NAN;
-  w = realisation(s, d, &sn, planck, pixel, ireal, &rejected);
+  if(d->spectral_type == HTGX_SPECTRAL_SW_CIE_XYZ) {
+    /* bits 32-33 of the realisation index select the channel */
+    if((ireal >> 32) > 2) return NAN;
+    if(cie_channel(s, d, (int)(ireal >> 32), &dc, &planck) != HTGX_OK) return NAN;
+    ireal &= 0xFFFFFFFFull;
+  } else if(make_planck(s, d, &planck) != HTGX_OK) {
+    return NAN;
+  }
+  w = realisation(s, &dc, &sn, planck, pixel, ireal, &rejected);
   oracle_wlen_planck_destroy(planck);
   return rejected ? NAN : w;
 }
@@ -1718,6 +1868,10 @@ nsum(double* sum, double* c, double x)
   *sum = t;
 }
 
+static int draw_map_channel
+  (const struct oracle_sky* s, const struct htgx_map_desc* d, const struct sensor* sn,
+   const struct oracle_wlen_planck* planck, struct htgx_mc_accum* accum, int nthreads);
+
 int
 oracle_draw_map
   (const struct oracle_sky* s, const struct htgx_map_desc* d,
@@ -1725,11 +1879,35 @@ oracle_draw_map
 {
   struct sensor sn;
   struct oracle_wlen_planck* planck = NULL;
-  int64_t npix, ip;
-  int res;
+  int res, c;
   if(!s || !d || !accum || !d->width || !d->height || !d->spp) return HTGX_BAD_ARG;
   if((res = sensor_setup(d, &sn)) != HTGX_OK) return res;
-  if((res = make_planck(s, d, &planck)) != HTGX_OK) return res;
+  if(d->spectral_type != HTGX_SPECTRAL_SW_CIE_XYZ) {
+    if((res = make_planck(s, d, &planck)) != HTGX_OK) return res;
+    res = draw_map_channel(s, d, &sn, planck, accum, nthreads);
+    oracle_wlen_planck_destroy(planck);
+    return res;
+  }
+  /* draw_pixel_image, ..._draw_map.c:134-199: spp realisations per channel,
+   * into 3 accumulator planes */
+  for(c = 0; c < 3; ++c) {
+    struct htgx_map_desc dc;
+    if((res = cie_channel(s, d, c, &dc, &planck)) != HTGX_OK) return res;
+    res = draw_map_channel(s, &dc, &sn, planck,
+                           accum + (size_t)c * d->width * d->height, nthreads);
+    oracle_wlen_planck_destroy(planck);
+    if(res != HTGX_OK) return res;
+  }
+  return HTGX_OK;
+}
+
+static int
+draw_map_channel
+  (const struct oracle_sky* s, const struct htgx_map_desc* d, const struct sensor* snp,
+   const struct oracle_wlen_planck* planck, struct htgx_mc_accum* accum, int nthreads)
+{
+  const struct sensor sn = *snp;
+  int64_t npix, ip;
   npix = (int64_t)d->width * d->height;
 #ifdef _OPENMP
   if(nthreads > 0) omp_set_num_threads(nthreads);
@@ -1757,22 +1935,16 @@ oracle_draw_map
     accum[ip] = a;
   }
   (void)nthreads;
-  oracle_wlen_planck_destroy(planck);
   return HTGX_OK;
 }
 
-int
-oracle_draw_pixels
-  (const struct oracle_sky* s, const struct htgx_map_desc* d,
-   const uint64_t* pixels, uint64_t npixels, struct htgx_mc_accum* accum, int nthreads)
+static int
+draw_pixels_channel
+  (const struct oracle_sky* s, const struct htgx_map_desc* d, const struct sensor* sn,
+   const struct oracle_wlen_planck* planck, const uint64_t* pixels, uint64_t npixels,
+   struct htgx_mc_accum* accum, int nthreads)
 {
-  struct sensor sn;
-  struct oracle_wlen_planck* planck = NULL;
   int64_t i;
-  int res;
-  if(!s || !d || !pixels || !accum || !d->width || !d->height || !d->spp) return HTGX_BAD_ARG;
-  if((res = sensor_setup(d, &sn)) != HTGX_OK) return res;
-  if((res = make_planck(s, d, &planck)) != HTGX_OK) return res;
 #ifdef _OPENMP
   if(nthreads > 0) omp_set_num_threads(nthreads);
   #pragma omp parallel for schedule(dynamic, 4)
@@ -1783,7 +1955,7 @@ oracle_draw_pixels
     memset(&a, 0, sizeof(a));
     for(r = 0; r < d->spp; ++r) {
       int rejected;
-      const double w = realisation(s, d, &sn, planck, pixels[i],
+      const double w = realisation(s, d, sn, planck, pixels[i],
         (uint64_t)(d->spp_offset + r), &rejected);
       if(rejected) { a.nfail++; continue; }
       nsum(&a.sum, &a.sum_c, w);
@@ -1793,7 +1965,33 @@ oracle_draw_pixels
     accum[i] = a;
   }
   (void)nthreads;
-  oracle_wlen_planck_destroy(planck);
+  return HTGX_OK;
+}
+
+int
+oracle_draw_pixels
+  (const struct oracle_sky* s, const struct htgx_map_desc* d,
+   const uint64_t* pixels, uint64_t npixels, struct htgx_mc_accum* accum, int nthreads)
+{
+  struct sensor sn;
+  struct oracle_wlen_planck* planck = NULL;
+  int res, c;
+  if(!s || !d || !pixels || !accum || !d->width || !d->height || !d->spp) return HTGX_BAD_ARG;
+  if((res = sensor_setup(d, &sn)) != HTGX_OK) return res;
+  if(d->spectral_type != HTGX_SPECTRAL_SW_CIE_XYZ) {
+    if((res = make_planck(s, d, &planck)) != HTGX_OK) return res;
+    res = draw_pixels_channel(s, d, &sn, planck, pixels, npixels, accum, nthreads);
+    oracle_wlen_planck_destroy(planck);
+    return res;
+  }
+  for(c = 0; c < 3; ++c) {
+    struct htgx_map_desc dc;
+    if((res = cie_channel(s, d, c, &dc, &planck)) != HTGX_OK) return res;
+    res = draw_pixels_channel(s, &dc, &sn, planck, pixels, npixels,
+                              accum + (size_t)c * npixels, nthreads);
+    oracle_wlen_planck_destroy(planck);
+    if(res != HTGX_OK) return res;
+  }
   return HTGX_OK;
 }
 

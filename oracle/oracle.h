@@ -94,13 +94,16 @@ double oracle_radiance
    const double pos[3], const double dir[3], double wlen,
    uint32_t iband, uint32_t iquad);
 
-/* The whole map into per-pixel accumulators (row-major), nthreads OpenMP
- * threads (0 = all) */
+/* The whole map into per-pixel accumulators (row-major; for a CIE XYZ
+ * image 3 planes, X, Y, Z), nthreads OpenMP threads (0 = all). CIE channel c
+ * draws from the streams keyed by seed ^ (c+1) << 56; oracle_realisation
+ * takes the channel from bits 32-33 of the realisation index. */
 int oracle_draw_map
   (const struct oracle_sky* sky, const struct htgx_map_desc* desc,
    struct htgx_mc_accum* accum, int nthreads);
 
-/* A subset of the pixels of a map (pixel ids y*W+x) into accum[npixels] */
+/* A subset of the pixels of a map (pixel ids y*W+x) into accum[npixels]
+ * (for a CIE XYZ image accum[3][npixels]) */
 int oracle_draw_pixels
   (const struct oracle_sky* sky, const struct htgx_map_desc* desc,
    const uint64_t* pixels, uint64_t npixels, struct htgx_mc_accum* accum,
@@ -141,6 +144,14 @@ double oracle_phase_eval(int kind, double param, double cos_theta);
 int oracle_ground_trace
   (const struct oracle_sky* sky, const double ray[8], int64_t prev_prim,
    int prev_edge, double* t, int64_t* prim, int* edge);
+
+/* CIE 1931 XYZ sampler of one channel (0 X, 1 Y, 2 Z), range within
+ * [380, 780] nm in nbands bands; sampled with oracle_wlen_planck_sample (pdf
+ * = 1 / integral of the channel, nm^-1) and freed with
+ * oracle_wlen_planck_destroy (src/core/htrdr_ran_wlen_cie_xyz.c) */
+int oracle_wlen_cie_create
+  (const double range[2], size_t nbands, int channel, struct oracle_wlen_planck** out);
+double oracle_cie_fit(int channel, double lambda_nm);
 
 /* Per-pixel estimate (src/core/htrdr_accum.h:48-68) */
 void oracle_accum_estimate

@@ -47,6 +47,8 @@ _SIG = {
                                      C.c_uint32, F.c_double_p]),
     "oracle_phase_eval": (C.c_double, [C.c_int, C.c_double, C.c_double]),
     "oracle_accum_estimate": (None, [C.c_void_p, F.c_double_p, F.c_double_p]),
+    "oracle_wlen_cie_create": (C.c_int, [F.c_double_p, C.c_size_t, C.c_int, C.POINTER(C.c_void_p)]),
+    "oracle_cie_fit": (C.c_double, [C.c_int, C.c_double]),
     "oracle_ground_trace": (C.c_int, [C.c_void_p, F.c_double_p, C.c_int64, C.c_int, F.c_double_p,
                                       C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
 }
@@ -115,22 +117,27 @@ class Sky:
         return lib().oracle_realisation(self._h, C.byref(desc), pixel, ireal)
 
     def draw_map(self, desc, nthreads=0):
-        from htrdr_kmin_amd.atmosphere import ACCUM_DTYPE
-        out = np.zeros(desc.width * desc.height, ACCUM_DTYPE)
+        """Accumulators (height, width), or (3, height, width) for CIE XYZ"""
+        from htrdr_kmin_amd.atmosphere import ACCUM_DTYPE, accum_planes
+        nc = accum_planes(desc)
+        out = np.zeros(nc * desc.width * desc.height, ACCUM_DTYPE)
         r = lib().oracle_draw_map(self._h, C.byref(desc), out.ctypes.data_as(C.c_void_p), nthreads)
         if r != 0:
             raise RuntimeError(f"oracle_draw_map: {r}")
-        return out.reshape(desc.height, desc.width)
+        shape = (desc.height, desc.width) if nc == 1 else (nc, desc.height, desc.width)
+        return out.reshape(shape)
 
     def draw_pixels(self, desc, pixels, nthreads=0):
-        from htrdr_kmin_amd.atmosphere import ACCUM_DTYPE
+        """Accumulators (npixels,), or (3, npixels) for CIE XYZ"""
+        from htrdr_kmin_amd.atmosphere import ACCUM_DTYPE, accum_planes
         pixels = np.ascontiguousarray(pixels, np.uint64)
-        out = np.zeros(pixels.size, ACCUM_DTYPE)
+        nc = accum_planes(desc)
+        out = np.zeros(nc * pixels.size, ACCUM_DTYPE)
         r = lib().oracle_draw_pixels(self._h, C.byref(desc), pixels.ctypes.data_as(F.c_uint64_p),
                                      pixels.size, out.ctypes.data_as(C.c_void_p), nthreads)
         if r != 0:
             raise RuntimeError(f"oracle_draw_pixels: {r}")
-        return out
+        return out if nc == 1 else out.reshape(nc, pixels.size)
 
     def close(self):
         if self._h:

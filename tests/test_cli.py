@@ -70,15 +70,21 @@ def parse_image(text):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("kind", ["image", "flux"])
+@pytest.mark.parametrize("kind", ["image", "flux", "cie"])
 def test_cli_matches_library(tmp_path, kind):
     """Same scene and map through the CLI and through Context.draw_map +
-    dump_buffer: identical columns 1-6 (the time columns are wall clocks)."""
+    dump_buffer: identical columns 1-6 (the time columns are wall clocks).
+    Without -s, a camera renders a CIE XYZ image over [380, 780] nm (the
+    reference's default, src/core/htrdr_args.h.in:121-125), clamped to the
+    sky's bands."""
     out = tmp_path / "out.txt"
     common = ["-a", "synth:0,3", "-c", "synth", "-g", "plane", "-R", "-D", "0,60", "-f",
               "-o", str(out), "-i", "def=24x16:spp=8"]
+    cam = ["-C", "pos=3200,-2000,5000:tgt=3200,-1999.5,4999.133974596:up=0,0,1:fov=60"]
     if kind == "image":
-        extra = ["-s", "sw=550,550", "-C", "pos=3200,-2000,5000:tgt=3200,-1999.5,4999.133974596:up=0,0,1:fov=60"]
+        extra = ["-s", "sw=550,550"] + cam
+    elif kind == "cie":
+        extra = cam
     else:
         extra = ["-s", "sw=550,550", "-p", "pos=3200,3200,2000:tgt=3200,3200,2001:up=0,1,0:sz=6400,6400"]
     r = run(*(common + extra))
@@ -89,7 +95,10 @@ def test_cli_matches_library(tmp_path, kind):
 
     s = A.Synth(0, seed=3, sun_azimuth=0.0, sun_elevation=60.0)
     d = A.map_preset(0, width=24, height=16, spp=8, solver=F.SOLVER_WAVEFRONT_FP32)
-    if kind == "image":
+    if kind == "cie":
+        d.spectral_type = F.SPECTRAL_SW_CIE_XYZ
+        d.wlen_range[:] = (380.0, 780.0)
+    if kind in ("image", "cie"):
         d.pos[:] = (3200, -2000, 5000)
         d.tgt[:] = (3200, -1999.5, 4999.133974596)
         d.fov_y = 60.0

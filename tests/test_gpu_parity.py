@@ -148,10 +148,14 @@ def test_bad_args(ctx0):
     desc = A.map_preset(0)
     r = lib.htgx_draw_map(ctx0.handle, C.byref(desc), buf.ctypes.data_as(C.c_void_p), 10)
     assert r == F.BAD_ARG
-    # CIE XYZ: images are not built, flux maps are invalid (as in the reference)
-    desc = A.map_preset(0, spectral_type=F.SPECTRAL_SW_CIE_XYZ)
-    r = lib.htgx_draw_map(ctx0.handle, C.byref(desc), buf.ctypes.data_as(C.c_void_p), 64 * 64 * 64)
-    assert r == F.BAD_OP
+    # CIE XYZ: an empty spectral range, and a pitch below 64 pixel_image
+    # structs of 80 B; flux maps are invalid (as in the reference)
+    desc = A.map_preset(0, spectral_type=F.SPECTRAL_SW_CIE_XYZ, wlen_range=(550.0, 550.0))
+    r = lib.htgx_draw_map(ctx0.handle, C.byref(desc), buf.ctypes.data_as(C.c_void_p), 64 * 80)
+    assert r == F.BAD_ARG
+    desc = A.map_preset(0, spectral_type=F.SPECTRAL_SW_CIE_XYZ, wlen_range=(500.0, 600.0))
+    r = lib.htgx_draw_map(ctx0.handle, C.byref(desc), buf.ctypes.data_as(C.c_void_p), 64 * 64)
+    assert r == F.BAD_ARG
     desc = A.map_preset(0, spectral_type=F.SPECTRAL_SW_CIE_XYZ, sensor=F.SENSOR_RECTANGLE,
                         rect_size=(100.0, 100.0))
     r = lib.htgx_draw_map(ctx0.handle, C.byref(desc), buf.ctypes.data_as(C.c_void_p), 64 * 64 * 64)
