@@ -16,6 +16,7 @@ OK, BAD_ARG, MEM_ERR, IO_ERR, UNKNOWN_ERR, BAD_OP = range(6)
 RES_NAMES = {0: "OK", 1: "BAD_ARG", 2: "MEM_ERR", 3: "IO_ERR", 4: "UNKNOWN_ERR", 5: "BAD_OP"}
 
 SPECTRAL_LW, SPECTRAL_SW, SPECTRAL_SW_CIE_XYZ = 0, 1, 2
+PROJECTION_PERSPECTIVE, PROJECTION_ORTHOGRAPHIC = 0, 1
 RADIANCE_DIRECT, RADIANCE_DIFFUSE, RADIANCE_ALL = 1, 2, 3
 Ka, Ks, Kext = 0, 1, 2
 SENSOR_CAMERA, SENSOR_RECTANGLE = 0, 1
@@ -92,7 +93,9 @@ class MapDesc(C.Structure):
                 ("tgt", C.c_double * 3), ("up", C.c_double * 3), ("fov_y", C.c_double),
                 ("rect_size", C.c_double * 2), ("spectral_type", C.c_int),
                 ("wlen_range", C.c_double * 2), ("ref_temperature", C.c_double),
-                ("seed", C.c_uint64), ("solver", C.c_int)]
+                ("seed", C.c_uint64), ("solver", C.c_int), ("projection", C.c_int),
+                ("lens_radius", C.c_double), ("focal_dst", C.c_double),
+                ("ortho_height", C.c_double)]
 
 
 class WfParams(C.Structure):

@@ -16,7 +16,7 @@
  *   -g plane | synth             an infinite plane (the quad repeated with -R)
  *                                or the configuration's fBm heightfield
  * Options without a counterpart on the GPU path are rejected with an error
- * (-d, -P, -r) or accepted and ignored (-t, -V, -O, -n).
+ * (-d, -r, -C focal-length) or accepted and ignored (-t, -V, -O, -n).
  * The solver is the FP32 wavefront one; HTGX_SOLVER=plain selects the FP64
  * plain one.
  */
@@ -43,6 +43,9 @@ struct args {
   int force, verbose, repeat_ground;
   int output_type; /* 0 image (-C), 1 flux map (-p) */
   double pos[3], tgt[3], up[3], fov; /* camera */
+  int ortho;                         /* -P: orthographic camera */
+  double lens_radius, focal_dst;     /* -C thin lens */
+  double ortho_height;               /* -P image plane height */
   double rpos[3], rtgt[3], rup[3], rsz[2]; /* flux map rectangle */
   unsigned def[2], spp;
   int spectral; /* enum htgx_spectral_type */
@@ -115,12 +118,32 @@ camera_param(const char* key, const char* val, struct args* a)
     return a->fov > 0 && a->fov < 180 ? 0 : -1;
   }
   if(!strcmp(key, "lens-radius")) {
-    double r;
-    if(parse_doubles(val, &r, 1)) return -1;
-    if(r != 0) fprintf(stderr, "thin-lens cameras are not built: lens-radius must be 0\n");
-    return r == 0 ? 0 : -1;
+    if(parse_doubles(val, &a->lens_radius, 1)) return -1;
+    return a->lens_radius >= 0 ? 0 : -1;
   }
-  if(!strcmp(key, "focal-dst") || !strcmp(key, "focal-length")) return 0; /* pinhole */
+  if(!strcmp(key, "focal-dst")) {
+    if(parse_doubles(val, &a->focal_dst, 1)) return -1;
+    return a->focal_dst > 0 ? 0 : -1;
+  }
+  if(!strcmp(key, "focal-length")) {
+    fprintf(stderr, "focal-length is not supported on this path: give the field of view (fov)\n");
+    return -1;
+  }
+  return -1;
+}
+
+/* -P: htrdr_args_camera_orthographic_parse (src/core/htrdr_args.c:303-345) */
+static int
+ortho_param(const char* key, const char* val, struct args* a)
+{
+  if(!val) return -1;
+  if(!strcmp(key, "pos")) return parse_doubles(val, a->pos, 3);
+  if(!strcmp(key, "tgt")) return parse_doubles(val, a->tgt, 3);
+  if(!strcmp(key, "up")) return parse_doubles(val, a->up, 3);
+  if(!strcmp(key, "height")) {
+    if(parse_doubles(val, &a->ortho_height, 1)) return -1;
+    return a->ortho_height > 0 ? 0 : -1;
+  }
   return -1;
 }
 
@@ -186,6 +209,7 @@ parse_args(int argc, char** argv, struct args* a)
   memset(a, 0, sizeof(*a));
   /* defaults of the reference (config.mk:36-53) */
   a->tgt[1] = 1; a->up[2] = 1; a->fov = 70;
+  a->focal_dst = 1; a->ortho_height = 1;
   a->rtgt[2] = 1; a->rup[1] = 1; a->rsz[0] = a->rsz[1] = 1;
   a->def[0] = 320; a->def[1] = 240; a->spp = 1;
   a->spectral = HTGX_SPECTRAL_SW_CIE_XYZ;
@@ -197,7 +221,7 @@ parse_args(int argc, char** argv, struct args* a)
     int bad = 0;
     switch(opt) {
       case 'a': a->gas = optarg; break;
-      case 'C': a->output_type = 0; bad = parse_list(optarg, camera_param, a); break;
+      case 'C': a->output_type = 0; a->ortho = 0; bad = parse_list(optarg, camera_param, a); break;
       case 'c': a->clouds = optarg; break;
       case 'D': {
         double d[2];
@@ -216,9 +240,7 @@ parse_args(int argc, char** argv, struct args* a)
       case 'n': break; /* the sky material is implicit */
       case 'O': break; /* no octree cache: the octree is rebuilt in a fraction of a second */
       case 'o': a->output = optarg; break;
-      case 'P':
-        fprintf(stderr, "%s: orthographic cameras (-P) are not built\n", argv[0]);
-        return HTGX_BAD_OP;
+      case 'P': a->output_type = 0; a->ortho = 1; bad = parse_list(optarg, ortho_param, a); break;
       case 'p': a->output_type = 1; bad = parse_list(optarg, rect_param, a); break;
       case 'R': a->repeat_ground = 1; break;
       case 'r':
@@ -403,6 +425,10 @@ main(int argc, char** argv)
     memcpy(d.tgt, a.tgt, sizeof(d.tgt));
     memcpy(d.up, a.up, sizeof(d.up));
     d.fov_y = a.fov;
+    d.projection = a.ortho ? HTGX_PROJECTION_ORTHOGRAPHIC : HTGX_PROJECTION_PERSPECTIVE;
+    d.lens_radius = a.lens_radius;
+    d.focal_dst = a.focal_dst;
+    d.ortho_height = a.ortho_height;
     elmt = a.spectral == HTGX_SPECTRAL_SW_CIE_XYZ
       ? sizeof(struct htgx_pixel_image) : sizeof(struct htgx_pixel_xwave);
   }

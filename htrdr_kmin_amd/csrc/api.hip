@@ -528,11 +528,25 @@ htgx_map_setup(htgx_ctx* ctx, const htgx_map_desc* d, DevMap* M)
   cross(t, M->z, d->up);
   if(!(normalize(M->x, t) > 0)) return HTGX_BAD_ARG;
   if(d->sensor == HTGX_SENSOR_CAMERA) {
-    /* pinhole camera; fov in ]0, 180[ (htrdr_args.c:77-83) */
-    if(!(d->fov_y > 0 && d->fov_y < 180)) return HTGX_BAD_ARG;
     cross(M->y, M->x, M->z);
-    M->tan_half = std::tan(d->fov_y * kPi / 180.0 * 0.5);
     M->aspect = (double)d->width / (double)d->height;
+    if(d->projection == HTGX_PROJECTION_ORTHOGRAPHIC) {
+      /* image plane height > 0 (htrdr_args_camera_orthographic_check) */
+      if(!(d->ortho_height > 0)) return HTGX_BAD_ARG;
+      M->ortho = 1;
+      M->ortho_half_h = 0.5 * d->ortho_height;
+    } else if(d->projection == HTGX_PROJECTION_PERSPECTIVE) {
+      /* fov in ]0, 180[, lens radius >= 0, focal distance > 0
+       * (htrdr_args_camera_perspective_check, src/core/htrdr_args.c) */
+      if(!(d->fov_y > 0 && d->fov_y < 180)) return HTGX_BAD_ARG;
+      if(!(d->lens_radius >= 0)) return HTGX_BAD_ARG;
+      if(d->lens_radius > 0 && !(d->focal_dst > 0)) return HTGX_BAD_ARG;
+      M->tan_half = std::tan(d->fov_y * kPi / 180.0 * 0.5);
+      M->lens_radius = d->lens_radius;
+      M->focal_dst = d->focal_dst;
+    } else {
+      return HTGX_BAD_ARG;
+    }
   } else {
     /* src/core/htrdr_rectangle.c:93-131 */
     if(!(d->rect_size[0] > 0 && d->rect_size[1] > 0)) return HTGX_BAD_ARG;

@@ -86,6 +86,10 @@ struct DevMap {
   int spectral_type;
   double pos[3], x[3], y[3], z[3];
   double tan_half, aspect;
+  int ortho;               /* camera: orthographic projection */
+  double ortho_half_h;     /* orthographic: half height of the image plane */
+  double lens_radius;      /* perspective: thin-lens radius (0: pinhole) */
+  double focal_dst;        /* thin lens: distance of the plane in focus */
   double axis_x[3], axis_y[3];
   /* wavelength sampler (src/core/htrdr_ran_wlen_planck.c) */
   int wmode; /* 0 monochromatic, 1 discrete CDF, 2 continuous bisection,
@@ -957,6 +961,46 @@ HTGX_D void stage_consume(const DevScene& S, Path& P, Stream& rng, const TrackOu
   }
 }
 
+/* Camera ray through the film point (fx, fy) in [0,1]^2 (fy = 0 at the top
+ * row) with the lens sample (l0, l1), as scam_generate_ray does for the
+ * cameras of src/atmosphere/htrdr_atmosphere.c:112-165:
+ *  - orthographic: from the image plane of height 2 ortho_half_h centred on
+ *    the position, along the view axis;
+ *  - pinhole: from the position through the film point;
+ *  - thin lens (the model of Pharr, Jakob and Humphreys, PBRT 3rd ed.,
+ *    6.2.3): from a point of the lens disk, r = R sqrt(l0), phi = 2 pi l1,
+ *    towards where the pinhole ray meets the plane in focus at focal_dst
+ *    along the view axis. */
+HTGX_D void camera_ray(const DevMap& M, double fx, double fy, double l0, double l1,
+                       double org[3], double dir[3])
+{
+  const double sx = 2.0 * fx - 1.0, sy = 1.0 - 2.0 * fy;
+  if(M.ortho) {
+#pragma unroll
+    for(int a = 0; a < 3; ++a) {
+      org[a] = M.pos[a] + M.x[a] * (sx * M.ortho_half_h * M.aspect) + M.y[a] * (sy * M.ortho_half_h);
+      dir[a] = M.z[a];
+    }
+    return;
+  }
+#pragma unroll
+  for(int a = 0; a < 3; ++a) {
+    dir[a] = M.z[a] + M.x[a] * (sx * M.tan_half * M.aspect) + M.y[a] * (sy * M.tan_half);
+    org[a] = M.pos[a];
+  }
+  if(M.lens_radius > 0) {
+    const double r = M.lens_radius * sqrt(l0), phi = 2.0 * kPi * l1;
+    const double lx = r * cos(phi), ly = r * sin(phi);
+#pragma unroll
+    for(int a = 0; a < 3; ++a) {
+      const double focus = M.pos[a] + dir[a] * M.focal_dst;
+      org[a] = M.pos[a] + M.x[a] * lx + M.y[a] * ly;
+      dir[a] = focus - org[a];
+    }
+  }
+  normalize3(dir);
+}
+
 /* Start a realisation: sensor ray, wavelength, band, k-point, first walk */
 HTGX_D void real_begin(const DevScene& S, const DevMap& M, Path& P, Stream& rng,
                        uint64_t pixel, uint64_t ireal)
@@ -970,14 +1014,8 @@ HTGX_D void real_begin(const DevScene& S, const DevMap& M, Path& P, Stream& rng,
   if(M.sensor == HTGX_SENSOR_CAMERA) {
     const double fx = ((double)px + rng.canonical()) * psx;
     const double fy = ((double)py + rng.canonical()) * psy;
-    (void)rng.canonical(); (void)rng.canonical(); /* lens sample */
-#pragma unroll
-    for(int a = 0; a < 3; ++a) {
-      P.dir0[a] = M.z[a] + M.x[a] * ((2.0 * fx - 1.0) * M.tan_half * M.aspect)
-                         + M.y[a] * ((1.0 - 2.0 * fy) * M.tan_half);
-      P.org[a] = M.pos[a];
-    }
-    normalize3(P.dir0);
+    const double l0 = rng.canonical(), l1 = rng.canonical(); /* lens sample */
+    camera_ray(M, fx, fy, l0, l1, P.org, P.dir0);
   } else {
     /* sample_rectangle_ray, ..._draw_map.c:52-107 */
     const double sx = ((double)px + rng.canonical()) * psx;

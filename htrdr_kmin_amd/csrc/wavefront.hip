@@ -759,19 +759,40 @@ __device__ __forceinline__ void real_begin(const SceneF& S, const DevMap& M, Lan
   const uint32_t px = L.pixel % M.width, py = L.pixel / M.width;
   float org[3], dir[3];
   if(SENSOR == HTGX_SENSOR_CAMERA) {
-    /* film sample; a pinhole camera needs no lens sample */
+    /* film sample u[0..1], lens sample u[2..3] (the camera_ray models of
+     * htgx_device.h) */
     const float fx = ((float)px + u[0]) / (float)M.width;
     const float fy = ((float)py + u[1]) / (float)M.height;
-    const float sx = (2.f * fx - 1.f) * (float)(M.tan_half * M.aspect);
-    const float sy = (1.f - 2.f * fy) * (float)M.tan_half;
+    if(M.ortho) {
+      const float sx = (2.f * fx - 1.f) * (float)(M.ortho_half_h * M.aspect);
+      const float sy = (1.f - 2.f * fy) * (float)M.ortho_half_h;
 #pragma unroll
-    for(int a = 0; a < 3; ++a) {
-      dir[a] = (float)M.z[a] + (float)M.x[a] * sx + (float)M.y[a] * sy;
-      org[a] = (float)M.pos[a];
+      for(int a = 0; a < 3; ++a) {
+        org[a] = (float)M.pos[a] + (float)M.x[a] * sx + (float)M.y[a] * sy;
+        dir[a] = (float)M.z[a];
+      }
+    } else {
+      const float sx = (2.f * fx - 1.f) * (float)(M.tan_half * M.aspect);
+      const float sy = (1.f - 2.f * fy) * (float)M.tan_half;
+#pragma unroll
+      for(int a = 0; a < 3; ++a) {
+        dir[a] = (float)M.z[a] + (float)M.x[a] * sx + (float)M.y[a] * sy;
+        org[a] = (float)M.pos[a];
+      }
+      if(M.lens_radius > 0) {
+        const float r = (float)M.lens_radius * sqrtf(u[2]), phi = 2.f * kPif * u[3];
+        const float lx = r * cosf(phi), ly = r * sinf(phi);
+#pragma unroll
+        for(int a = 0; a < 3; ++a) {
+          const float focus = org[a] + dir[a] * (float)M.focal_dst;
+          org[a] += (float)M.x[a] * lx + (float)M.y[a] * ly;
+          dir[a] = focus - org[a];
+        }
+      }
+      const float inv = rsqrtf(dot3f(dir, dir));
+#pragma unroll
+      for(int a = 0; a < 3; ++a) dir[a] *= inv;
     }
-    const float inv = rsqrtf(dot3f(dir, dir));
-#pragma unroll
-    for(int a = 0; a < 3; ++a) dir[a] *= inv;
   } else {
     const float sx = ((float)px + u[0]) / (float)M.width;
     const float sy = ((float)py + u[1]) / (float)M.height;

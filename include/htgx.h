@@ -79,6 +79,14 @@ enum htgx_sensor_type {
   HTGX_SENSOR_RECTANGLE = 1  /* flux-density map, option -p */
 };
 
+/* Camera projections (star-camera, set up at src/atmosphere/
+ * htrdr_atmosphere.c:112-165): -C is perspective (a pinhole when the lens
+ * radius is 0, else a thin lens), -P orthographic */
+enum htgx_projection {
+  HTGX_PROJECTION_PERSPECTIVE = 0,
+  HTGX_PROJECTION_ORTHOGRAPHIC = 1
+};
+
 enum htgx_solver {
   HTGX_SOLVER_PLAIN_FP64 = 0,    /* one path per thread, FP64 everywhere */
   HTGX_SOLVER_WAVEFRONT_FP32 = 1 /* FP32 transport, FP64 Tmin + accumulators */
@@ -227,6 +235,11 @@ struct htgx_map_desc {
   double ref_temperature; /* K, option -s Tref= */
   uint64_t seed;          /* key of the Threefry streams */
   int solver;             /* enum htgx_solver */
+  /* camera projection (zero-initialised: a pinhole perspective camera) */
+  int projection;         /* enum htgx_projection */
+  double lens_radius;     /* perspective: lens radius, m; 0 = pinhole (-C lens-radius) */
+  double focal_dst;       /* perspective with a lens: distance in focus, m (-C focal-dst) */
+  double ortho_height;    /* orthographic: height of the image plane, m (-P height) */
 };
 
 /*******************************************************************************

@@ -1671,6 +1671,9 @@ struct sensor {
   double pos[3];
   double x[3], y[3], z[3]; /* camera: right, up, forward; rectangle: frame */
   double tan_half, aspect;
+  int ortho;               /* orthographic camera (-P) */
+  double half_h;           /* its image plane half height */
+  double lens_radius, focal_dst; /* thin-lens perspective camera */
   double axis_x[3], axis_y[3];
 };
 
@@ -1689,6 +1692,16 @@ sensor_setup(const struct htgx_map_desc* d, struct sensor* c)
     cross3(c->y, c->x, c->z);
     c->tan_half = tan(d->fov_y * PI / 180.0 * 0.5);
     c->aspect = (double)d->width / (double)d->height;
+    if(d->projection != HTGX_PROJECTION_PERSPECTIVE && d->projection != HTGX_PROJECTION_ORTHOGRAPHIC)
+      return HTGX_BAD_ARG;
+    c->ortho = d->projection == HTGX_PROJECTION_ORTHOGRAPHIC;
+    c->half_h = 0.5 * d->ortho_height;
+    c->lens_radius = d->lens_radius;
+    c->focal_dst = d->focal_dst;
+    if(c->ortho ? !(d->ortho_height > 0)
+                : !(d->fov_y > 0 && d->fov_y < 180) || !(d->lens_radius >= 0)
+                  || (d->lens_radius > 0 && !(d->focal_dst > 0)))
+      return HTGX_BAD_ARG;
   } else {
     /* Rectangle frame, src/core/htrdr_rectangle.c:101-131 */
     cross3(t, c->z, c->x);
@@ -1746,14 +1759,34 @@ realisation
   if(d->sensor == HTGX_SENSOR_CAMERA) {
     const double fx = ((double)px + canonical(&rng)) * psx;   /* :364-369 */
     const double fy = ((double)py + canonical(&rng)) * psy;
+    const double l0 = canonical(&rng), l1 = canonical(&rng); /* lens sample */
     double v[3];
-    (void)canonical(&rng); (void)canonical(&rng);           /* lens sample */
-    for(a = 0; a < 3; ++a) {
-      v[a] = sn->z[a] + sn->x[a] * ((2.0*fx - 1.0) * sn->tan_half * sn->aspect)
-                      + sn->y[a] * ((1.0 - 2.0*fy) * sn->tan_half);
-      org[a] = sn->pos[a];
+    if(sn->ortho) {
+      /* orthographic: from the image plane, along the view axis */
+      for(a = 0; a < 3; ++a) {
+        org[a] = sn->pos[a] + sn->x[a] * ((2.0*fx - 1.0) * sn->half_h * sn->aspect)
+                            + sn->y[a] * ((1.0 - 2.0*fy) * sn->half_h);
+        dir[a] = sn->z[a];
+      }
+    } else {
+      for(a = 0; a < 3; ++a) {
+        v[a] = sn->z[a] + sn->x[a] * ((2.0*fx - 1.0) * sn->tan_half * sn->aspect)
+                        + sn->y[a] * ((1.0 - 2.0*fy) * sn->tan_half);
+        org[a] = sn->pos[a];
+      }
+      if(sn->lens_radius > 0) {
+        /* thin lens: a point of the lens disk towards the point of the
+         * pinhole ray on the plane in focus */
+        const double r = sn->lens_radius * sqrt(l0), phi = 2.0 * PI * l1;
+        const double lx = r * cos(phi), ly = r * sin(phi);
+        for(a = 0; a < 3; ++a) {
+          const double focus = sn->pos[a] + v[a] * sn->focal_dst;
+          org[a] = sn->pos[a] + sn->x[a] * lx + sn->y[a] * ly;
+          v[a] = focus - org[a];
+        }
+      }
+      normalize3(dir, v);
     }
-    normalize3(dir, v);
   } else {
     const double sx = ((double)px + canonical(&rng)) * psx;   /* :69-73 */
     const double sy = ((double)py + canonical(&rng)) * psy;

@@ -40,7 +40,9 @@ def test_help():
     (("-a", "synth:0", "-s", "sw=600,500"), "invalid option argument"),
     (("-a", "synth:0", "-T", "-1"), "invalid option argument"),
     (("-a", "synth:0", "-d"), "not built"),
-    (("-a", "synth:0", "-P", "pos=0,0,0"), "not built"),
+    (("-a", "synth:0", "-P", "height=0"), "invalid option argument"),
+    (("-a", "synth:0", "-C", "focal-length=0.05"), "focal-length is not supported"),
+    (("-a", "synth:0", "-C", "lens-radius=-1"), "invalid option argument"),
     (("-a", "synth:0", "-r"), "not built"),
     (("-a", "synth:9", "-s", "sw=550,550"), "unknown synthetic configuration"),
 ])
@@ -70,7 +72,7 @@ def parse_image(text):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("kind", ["image", "flux", "cie"])
+@pytest.mark.parametrize("kind", ["image", "flux", "cie", "ortho", "lens"])
 def test_cli_matches_library(tmp_path, kind):
     """Same scene and map through the CLI and through Context.draw_map +
     dump_buffer: identical columns 1-6 (the time columns are wall clocks).
@@ -85,6 +87,12 @@ def test_cli_matches_library(tmp_path, kind):
         extra = ["-s", "sw=550,550"] + cam
     elif kind == "cie":
         extra = cam
+    elif kind == "ortho":
+        extra = ["-s", "sw=550,550", "-P",
+                 "pos=3200,-2000,5000:tgt=3200,-1999.5,4999.133974596:up=0,0,1:height=3000"]
+    elif kind == "lens":
+        extra = ["-s", "sw=550,550", "-C", "pos=3200,-2000,5000:tgt=3200,-1999.5,4999.133974596:"
+                 "up=0,0,1:fov=60:lens-radius=50:focal-dst=4000"]
     else:
         extra = ["-s", "sw=550,550", "-p", "pos=3200,3200,2000:tgt=3200,3200,2001:up=0,1,0:sz=6400,6400"]
     r = run(*(common + extra))
@@ -98,7 +106,12 @@ def test_cli_matches_library(tmp_path, kind):
     if kind == "cie":
         d.spectral_type = F.SPECTRAL_SW_CIE_XYZ
         d.wlen_range[:] = (380.0, 780.0)
-    if kind in ("image", "cie"):
+    if kind == "ortho":
+        d.projection = F.PROJECTION_ORTHOGRAPHIC
+        d.ortho_height = 3000.0
+    if kind == "lens":
+        d.lens_radius, d.focal_dst = 50.0, 4000.0
+    if kind in ("image", "cie", "ortho", "lens"):
         d.pos[:] = (3200, -2000, 5000)
         d.tgt[:] = (3200, -1999.5, 4999.133974596)
         d.fov_y = 60.0
