@@ -16,7 +16,7 @@
  *   -g plane | synth             an infinite plane (the quad repeated with -R)
  *                                or the configuration's fBm heightfield
  * Options without a counterpart on the GPU path are rejected with an error
- * (-d, -r, -C focal-length) or accepted and ignored (-t, -V, -O, -n).
+ * (-d, -C focal-length) or accepted and ignored (-t, -V, -O, -n).
  * The solver is the FP32 wavefront one; HTGX_SOLVER=plain selects the FP64
  * plain one.
  */
@@ -40,7 +40,7 @@ struct args {
   const char* ground;
   const char* mtl;
   const char* output;
-  int force, verbose, repeat_ground;
+  int force, verbose, repeat_ground, repeat_clouds;
   int output_type; /* 0 image (-C), 1 flux map (-p) */
   double pos[3], tgt[3], up[3], fov; /* camera */
   int ortho;                         /* -P: orthographic camera */
@@ -243,9 +243,7 @@ parse_args(int argc, char** argv, struct args* a)
       case 'P': a->output_type = 0; a->ortho = 1; bad = parse_list(optarg, ortho_param, a); break;
       case 'p': a->output_type = 1; bad = parse_list(optarg, rect_param, a); break;
       case 'R': a->repeat_ground = 1; break;
-      case 'r':
-        fprintf(stderr, "%s: cloud repetition (-r) is not built\n", argv[0]);
-        return HTGX_BAD_OP;
+      case 'r': a->repeat_clouds = 1; break;
       case 's': bad = parse_list(optarg, spectral_param, a); break;
       case 'T': bad = parse_doubles(optarg, &a->thickness, 1) || a->thickness < 0; break;
       case 't': break; /* thread hint: the GPU path has no host threads */
@@ -354,6 +352,7 @@ main(int argc, char** argv)
   struct htgx_synth* synth = NULL;
   struct htgx_ctx* ctx = NULL;
   struct htgx_map_desc d;
+  struct htgx_scene_desc scene;
   struct htgx_accum tacc = {0, 0, 0, 0}, facc = {0, 0, 0, 0};
   const char* solver = getenv("HTGX_SOLVER");
   void* pixels = NULL;
@@ -443,7 +442,9 @@ main(int argc, char** argv)
     }
   }
 
-  if((res = htgx_ctx_create(htgx_synth_get_scene(synth), &ctx)) != HTGX_OK) {
+  scene = *htgx_synth_get_scene(synth);
+  scene.repeat_clouds = a.repeat_clouds;
+  if((res = htgx_ctx_create(&scene, &ctx)) != HTGX_OK) {
     fprintf(stderr, "%s: could not set up the scene on the GPU (%d)\n", argv[0], res);
     goto error;
   }

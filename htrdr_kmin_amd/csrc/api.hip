@@ -398,6 +398,8 @@ htgx_ctx_init(htgx_ctx* ctx, const htgx_scene_desc& sc)
       S.inv_bs[a] = svo.inv_bs[a];
       S.inv_cs[a] = svo.inv_cs[a];
     }
+    S.repeat = sc.repeat_clouds ? 1 : 0;
+    for(int a = 0; a < 2; ++a) S.period[a] = (double)svo.dims[0][a] * svo.bs[a];
     const size_t ncells = (size_t)cl.def[0] * cl.def[1] * cl.def[2];
     if((res = upload(ctx, cl.lwc, ncells, &S.lwc))) return res;
     if((res = upload(ctx, svo.lmap.data(), svo.lmap.size(), &S.lmap))) return res;

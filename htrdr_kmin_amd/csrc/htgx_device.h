@@ -56,6 +56,8 @@ struct DevScene {
   uint32_t dims0[3];
   double lower[3];
   double bs[3], inv_bs[3], inv_cs[3];
+  int repeat;        /* clouds repeated along X and Y (-r) */
+  double period[2];  /* repetition period: dims0 * bs */
   const float* lwc;
   const uint32_t* lmap;
   const float2* bounds;
@@ -324,7 +326,12 @@ struct Segment {
   int32_t layer;
   uint32_t leaf;        /* global leaf id */
   uint32_t lo[3], hi[3]; /* leaf range, in blocks */
+  double shift[2];      /* X/Y offset of the leaf's copy of the grid (-r) */
 };
+
+/* Cap on the copies of a repeated cloud grid one ray crosses (a horizontal
+ * ray inside the cloud layer never leaves it) */
+constexpr int64_t kMaxTiles = 1 << 16;
 
 HTGX_D double gas_k(const DevScene& S, int prop, uint32_t iband, uint32_t iquad, int32_t j)
 {
@@ -361,6 +368,19 @@ struct SegIter {
   int exit_axis;
   int32_t j;    /* gas layer, -1 below, nlayers above */
   int done;
+  /* repeated clouds: o[0..1] is the origin relative to the copy k of the
+   * grid, shift = k * period, ow the world origin */
+  int64_t k[2];
+  double shift[2], ow[2];
+  int64_t ntiles;
+
+  /* Enter copy k of the grid along axis a (0 or 1) */
+  HTGX_D void set_tile(const DevScene& S, int a, int64_t ka)
+  {
+    k[a] = ka;
+    shift[a] = (double)ka * S.period[a];
+    o[a] = ow[a] - shift[a];
+  }
 
   HTGX_D double plane(const DevScene& S, int a, uint32_t k) const
   {
@@ -412,12 +432,14 @@ struct SegIter {
 #pragma unroll
     for(int a = 0; a < 3; ++a) {
       if(a == exit_axis) {
-        if(d[a] > 0) {
-          if(hi[a] >= S.dims0[a]) { cstate = 2; return; }
-          c[a] = hi[a];
+        const bool out = d[a] > 0 ? hi[a] >= S.dims0[a] : lo[a] == 0;
+        if(out && (!S.repeat || a == 2 || ++ntiles > kMaxTiles)) { cstate = 2; return; }
+        if(out) {
+          /* into the next copy of the grid */
+          set_tile(S, a, k[a] + (d[a] > 0 ? 1 : -1));
+          c[a] = d[a] > 0 ? 0u : S.dims0[a] - 1;
         } else {
-          if(lo[a] == 0) { cstate = 2; return; }
-          c[a] = lo[a] - 1;
+          c[a] = d[a] > 0 ? hi[a] : lo[a] - 1;
         }
       } else {
         const double p = o[a] + t * d[a];
@@ -437,11 +459,17 @@ struct SegIter {
     tmax = r1;
     done = 0;
     cstate = 2;
+    k[0] = k[1] = 0;
+    shift[0] = shift[1] = 0.0;
+    ow[0] = o[0];
+    ow[1] = o[1];
+    ntiles = 0;
     if(S.has_clouds) {
       double tn = -INFINITY, tf = INFINITY;
       bool miss = false;
-#pragma unroll
-      for(int a = 0; a < 3; ++a) {
+      /* repeated clouds: an infinite layer along X and Y */
+      const int a0 = S.repeat ? 2 : 0;
+      for(int a = a0; a < 3; ++a) {
         const double plo = plane(S, a, 0), phi = plane(S, a, S.dims0[a]);
         if(d[a] != 0) {
           const double ta = (plo - o[a]) * inv[a];
@@ -455,6 +483,12 @@ struct SegIter {
         }
       }
       if(!miss && fmax(tn, t) < fmin(tf, tmax)) {
+        if(S.repeat) {
+          /* the copy of the grid holding the entry point */
+          const double te = fmax(tn, t);
+          for(int a = 0; a < 2; ++a)
+            set_tile(S, a, (int64_t)floor((ow[a] + te * d[a] - S.lower[a]) / S.period[a]));
+        }
         if(tn > t) { cstate = 0; tbox = tn; }
         else { cstate = 1; locate(S, t, cnt); }
       }
@@ -497,10 +531,13 @@ struct SegIter {
             seg.leaf = leaf;
 #pragma unroll
             for(int a = 0; a < 3; ++a) { seg.lo[a] = lo[a]; seg.hi[a] = hi[a]; }
+            seg.shift[0] = shift[0];
+            seg.shift[1] = shift[1];
           } else {
             seg.kmin = g;
             seg.kmax = g;
             seg.leaf = 0xFFFFFFFFu;
+            seg.shift[0] = seg.shift[1] = 0.0;
           }
           emit = true;
         }
@@ -525,13 +562,15 @@ struct SegIter {
   }
 };
 
-/* LWC of the cloud cell holding x, clamped into the segment's leaf */
+/* LWC of the cloud cell holding x (world), clamped into the segment's leaf
+ * of the segment's copy of the grid */
 HTGX_D float seg_lwc(const DevScene& S, const Segment& seg, const double x[3], Counters* cnt)
 {
   uint32_t ci[3];
 #pragma unroll
   for(int a = 0; a < 3; ++a) {
-    const double f = floor((x[a] - S.lower[a]) * S.inv_cs[a]);
+    const double xa = a < 2 ? x[a] - seg.shift[a] : x[a];
+    const double f = floor((xa - S.lower[a]) * S.inv_cs[a]);
     const double lo = (double)(seg.lo[a] * S.B);
     const double hi = (double)(seg.hi[a] * S.B - 1);
     ci[a] = f < lo ? seg.lo[a] * S.B : f > hi ? seg.hi[a] * S.B - 1 : (uint32_t)f;

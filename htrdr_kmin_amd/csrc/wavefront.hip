@@ -80,6 +80,8 @@ struct SceneF {
   uint32_t def[3];
   uint32_t dims0[3];
   float lower[3], bs[3], inv_bs[3], inv_cs[3];
+  int repeat;        /* clouds repeated along X and Y (-r) */
+  float period[2];   /* repetition period: dims0 * bs */
   const float* lwc;
   /* per finest block: x = level-map word (level << 28 | leaf id), y = the
    * leaf's LWC min (low half) and max (high half) as FP16 multiples of
@@ -192,8 +194,11 @@ struct LaneCold {
   uint32_t gh_mtl[kBlock];
   float tsum[kBlock], tsum2[kBlock];
   uint32_t n[kBlock], nfail[kBlock], chunk[kBlock], r_end[kBlock];
+  float shift[2][kBlock]; /* -r: world X/Y of the origin of the lane's frame */
 };
 __shared__ LaneCold g_cold;
+/* Cap on the copies of a repeated cloud grid one query crosses */
+constexpr uint32_t kMaxTilesF = 1u << 16;
 #define LC(f) (g_cold.f[threadIdx.x])
 #define LCA(f, a) (g_cold.f[a][threadIdx.x])
 
@@ -295,6 +300,19 @@ struct GroundF {
   float t;
   uint32_t nodes, ntris; /* BVH work of the trace (counting build) */
 };
+
+/* World position of a point of the lane's frame (-r moves the frame with
+ * the copies of the cloud grid); only a mesh ground needs it */
+__device__ __forceinline__ void world_posf(const SceneF& S, const float p[3], float w[3])
+{
+  w[0] = p[0];
+  w[1] = p[1];
+  w[2] = p[2];
+  if(S.repeat && S.ground_type == HTGX_GROUND_MESH) {
+    w[0] += LCA(shift, 0);
+    w[1] += LCA(shift, 1);
+  }
+}
 
 __device__ __forceinline__ GroundF ground_tracef(const SceneF& S, const float o[3], const float d[3],
                                                  float r1, int32_t prev_prim, bool prev_edge,
@@ -399,7 +417,7 @@ __device__ __forceinline__ void load_leaf(const SceneF& S, Lane& L, uint32_t cx,
  * the current leaf, other axes clamped into its block range); false when the
  * step leaves the grid. One code path for both, so that a wavefront pays for
  * a single leaf load per pass whatever mix of states its lanes are in. */
-__device__ __forceinline__ bool next_cell(const SceneF& S, const Lane& L, uint32_t c[3])
+__device__ __forceinline__ bool next_cell(const SceneF& S, Lane& L, uint32_t c[3])
 {
   const bool step = L.cstate == 4;
 #pragma unroll
@@ -407,12 +425,17 @@ __device__ __forceinline__ bool next_cell(const SceneF& S, const Lane& L, uint32
     const uint32_t lo = step ? L.lohi[a] & 0xFFFFu : 0u;
     const uint32_t hi = step ? L.lohi[a] >> 16 : S.dims0[a];
     if(step && a == L.exit_axis) {
-      if(L.dir[a] > 0.f) {
-        if(hi >= S.dims0[a]) return false;
-        c[a] = hi;
+      const bool out = L.dir[a] > 0.f ? hi >= S.dims0[a] : lo == 0;
+      if(out) {
+        /* -r: into the next copy of the grid along X or Y; the lane's frame
+         * moves with it (a bounded number of copies per query) */
+        if(a == 2 || !S.repeat || ++L.nsteps > kMaxTilesF) return false;
+        const float dp = L.dir[a] > 0.f ? S.period[a] : -S.period[a];
+        L.pos[a] -= dp;
+        LCA(shift, a) += dp;
+        c[a] = L.dir[a] > 0.f ? 0u : S.dims0[a] - 1;
       } else {
-        if(lo == 0) return false;
-        c[a] = lo - 1;
+        c[a] = L.dir[a] > 0.f ? hi : lo - 1;
       }
     } else {
       const float p = L.pos[a] + L.tit * L.dir[a];
@@ -440,6 +463,7 @@ __device__ __forceinline__ void iter_init(const SceneF& S, Lane& L)
     bool miss = false;
 #pragma unroll
     for(int a = 0; a < 3; ++a) {
+      if(S.repeat && a < 2) continue; /* -r: an infinite layer along X, Y */
       const float plo = planef(S, a, 0), phi = planef(S, a, S.dims0[a]);
       if(L.dir[a] != 0.f) {
         const float ta = (plo - L.pos[a]) * L.inv_dir[a];
@@ -451,6 +475,18 @@ __device__ __forceinline__ void iter_init(const SceneF& S, Lane& L)
       }
     }
     if(!miss && fmaxf(tn, 0.f) < fminf(tf, r1)) {
+      if(S.repeat) {
+        /* the frame of the copy of the grid holding the entry point */
+        const float te = fmaxf(tn, 0.f);
+#pragma unroll
+        for(int a = 0; a < 2; ++a) {
+          const float k = floorf((L.pos[a] + te * L.dir[a] - S.lower[a]) / S.period[a]);
+          if(k != 0.f) {
+            L.pos[a] -= k * S.period[a];
+            LCA(shift, a) += k * S.period[a];
+          }
+        }
+      }
       if(tn > 0.f) { L.cstate = 0; L.tnext = tn; }
       else L.cstate = 3;
     }
@@ -596,12 +632,15 @@ __device__ __forceinline__ void sw_begin(const SceneF& S, Lane& L, int mask, con
   LC(mask) = mask;
 #pragma unroll
   for(int a = 0; a < 3; ++a) { L.pos[a] = p[a]; L.dir[a] = d[a]; }
+  LCA(shift, 0) = LCA(shift, 1) = 0.f; /* a walk starts from a world position */
   LC(ksi) = 1.f;
   LC(w) = 0.f;
   LC(prev_prim) = -1;
   L.prev_edge = false;
   if((mask & HTGX_RADIANCE_DIRECT) && dot3f(L.dir, S.sun_dir) >= S.sun_cos_half) {
-    const GroundF g = ground_tracef(S, L.pos, L.dir, FLT_MAX, -1, false, true);
+    float wp[3];
+    world_posf(S, L.pos, wp);
+    const GroundF g = ground_tracef(S, wp, L.dir, FLT_MAX, -1, false, true);
     count_ground<COUNT>(L, g);
     if(!g.hit) {
       LC(stage) = ST_DIRECT;
@@ -690,7 +729,9 @@ __device__ __forceinline__ void consume(const SceneF& S, Lane& L, const float* u
       }
       bool shadowed = LC(R) <= 0.f;
       if(!shadowed) {
-        const GroundF g = ground_tracef(S, L.pos, sun_dir, FLT_MAX, LC(prev_prim), L.prev_edge, true);
+        float wp[3];
+        world_posf(S, L.pos, wp);
+        const GroundF g = ground_tracef(S, wp, sun_dir, FLT_MAX, LC(prev_prim), L.prev_edge, true);
         count_ground<COUNT>(L, g);
         shadowed = g.hit;
       }
@@ -1045,7 +1086,11 @@ __device__ __forceinline__ void run_scene(const SceneF& S, const DevMap& M, cons
       WF_TICK(2)
       if(L.lstate == LS_EVENT && LC(stage) != ST_FETCH) {
         const bool begin = LC(stage) == ST_BEGIN;
-        if(begin) { L.pair = 0; LC(t0) = (uint32_t)wall_clock64(); }
+        if(begin) {
+          L.pair = 0;
+          LC(t0) = (uint32_t)wall_clock64();
+          LCA(shift, 0) = LCA(shift, 1) = 0.f; /* sensor rays start in world coordinates */
+        }
         /* one set of canonicals per event, drawn at a single site */
         float u[2 * NP];
 #pragma unroll
@@ -1091,7 +1136,9 @@ __device__ __forceinline__ void run_scene(const SceneF& S, const DevMap& M, cons
         if(L.nq) {
           L.nq = false;
           if(L.nq_ground) {
-            const GroundF gh = ground_tracef(S, L.pos, L.dir, INFINITY, LC(prev_prim), L.prev_edge,
+            float wp[3];
+            world_posf(S, L.pos, wp);
+            const GroundF gh = ground_tracef(S, wp, L.dir, INFINITY, LC(prev_prim), L.prev_edge,
                                              false);
             count_ground<COUNT>(L, gh);
             L.gh_hit = gh.hit;
@@ -1415,6 +1462,9 @@ wavefront_build(htgx_ctx* ctx, htgx_wavefront_state* st)
       S.inv_cs[a] = (float)D.inv_cs[a];
       if(D.dims0[a] > 0xFFFFu) return HTGX_BAD_ARG;
     }
+    S.repeat = D.repeat;
+    S.period[0] = (float)D.period[0];
+    S.period[1] = (float)D.period[1];
     S.lwc = D.lwc;
     /* fused leaf records; the scale is the grid's largest LWC (the root's
      * upper bound, last node of the level-ordered bounds) */

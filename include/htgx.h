@@ -215,6 +215,9 @@ struct htgx_scene_desc {
   uint32_t leaf_block;  /* edge (in cells) of the finest octree leaf block:
                            1, 2, 4, ...; the cloud definition must be a
                            multiple of it */
+  int repeat_clouds;    /* option -r: the cloud grid repeats along X and Y
+                           with the period of its extent (htsky's
+                           repeat_clouds, src/atmosphere/htrdr_atmosphere.c:375) */
 };
 
 /*******************************************************************************

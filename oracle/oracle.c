@@ -514,6 +514,7 @@ struct oracle_sky {
   double cs[3], inv_cs[3];     /* cell size and inverse */
   double bs[3], inv_bs[3];     /* block size and inverse */
   double cext_max;
+  double period[2];            /* -r repetition period along X, Y */
   struct onode* root;
   uint64_t nnodes_alloc, nleaves;
   /* gas */
@@ -679,6 +680,8 @@ oracle_sky_create(const struct htgx_scene_desc* scene, struct oracle_sky** out)
         + (uint64_t)s->dims[l-1][0] * s->dims[l-1][1] * s->dims[l-1][2];
       s->nlevels++;
     }
+    s->period[0] = (double)s->dims[0][0] * s->bs[0];
+    s->period[1] = (double)s->dims[0][1] * s->bs[1];
     s->cext_max = 0;
     for(b = 0; b < scene->spectral.nbands; ++b)
       if(scene->clouds.cext[b] > s->cext_max) s->cext_max = scene->clouds.cext[b];
@@ -770,6 +773,10 @@ struct ray {
   double range[2];
 };
 
+/* Cap on the copies of a repeated cloud grid one ray crosses, as the GPU
+ * iterator's (a horizontal ray inside the cloud layer never leaves it) */
+#define ORACLE_MAX_TILES (1 << 16)
+
 /* Slab test of a box given by block planes */
 static void
 slab(const struct oracle_sky* s, const struct ray* r, const uint32_t lo[3],
@@ -800,6 +807,7 @@ struct leaf_hit {
   int64_t id;
   float mn, mx;
   uint32_t lo[3], hi[3]; /* block range */
+  double shift[2];  /* X/Y offset of the leaf's copy of the grid (-r) */
 };
 
 struct leaf_list {
@@ -832,6 +840,7 @@ traverse(const struct oracle_sky* s, const struct onode* n, const struct ray* r,
   if(n->leaf) {
     struct leaf_hit h;
     int a;
+    h.shift[0] = h.shift[1] = 0;
     h.tf = tf;
     h.id = (int64_t)leaf_id(s, n);
     h.mn = n->mn;
@@ -934,12 +943,61 @@ for_each_segment
   t = range[0];
   tmax = range[1];
 
-  if(s->has_clouds) {
+  if(s->has_clouds && !s->scene.repeat_clouds) {
     uint32_t lo[3] = {0,0,0};
     double tn, tf;
     slab(s, &r, lo, s->dims[0], &tn, &tf);
     if(fmax(tn, t) < fmin(tf, tmax)) {
       traverse(s, s->root, &r, &leaves);
+      if(leaves.n) {
+        if(tn > t) { cloud_state = 0; tbox = tn; } else { cloud_state = 1; }
+      }
+    }
+  } else if(s->has_clouds) {
+    /* -r: the grid repeats along X and Y with the period of its extent, an
+     * infinite layer of copies crossed one after the other, each traversed
+     * with the ray origin relative to it (the slab walk of
+     * src/core/htrdr_slab.c:35-137, here stepping to the next copy across
+     * the face the ray leaves the current one by) */
+    double tn = -INFINITY, tf = INFINITY;
+    const double plo = plane(s, 2, 0), phi = plane(s, 2, s->dims[0][2]);
+    if(dir[2] != 0) {
+      const double ta = (plo - org[2]) * r.inv[2], tb = (phi - org[2]) * r.inv[2];
+      tn = ta < tb ? ta : tb;
+      tf = ta < tb ? tb : ta;
+    } else if(org[2] < plo || org[2] > phi) {
+      tn = INFINITY;
+      tf = -INFINITY;
+    }
+    if(fmax(tn, t) < fmin(tf, tmax)) {
+      const double te = fmax(tn, t);
+      int64_t k[2], ntiles = 0;
+      for(a = 0; a < 2; ++a)
+        k[a] = (int64_t)floor((org[a] + te * dir[a] - s->scene.clouds.lower[a]) / s->period[a]);
+      for(;;) {
+        struct ray rl = r;
+        const size_t first = leaves.n;
+        double sh[2], tx[3];
+        size_t i;
+        int ax;
+        for(a = 0; a < 2; ++a) {
+          sh[a] = (double)k[a] * s->period[a];
+          rl.o[a] = org[a] - sh[a];
+        }
+        traverse(s, s->root, &rl, &leaves);
+        for(i = first; i < leaves.n; ++i) {
+          leaves.v[i].shift[0] = sh[0];
+          leaves.v[i].shift[1] = sh[1];
+        }
+        /* the face the ray leaves this copy by */
+        for(a = 0; a < 3; ++a) {
+          tx[a] = dir[a] == 0 ? INFINITY
+            : (plane(s, a, dir[a] > 0 ? s->dims[0][a] : 0) - rl.o[a]) * r.inv[a];
+        }
+        ax = tx[0] < tx[1] ? (tx[0] < tx[2] ? 0 : 2) : (tx[1] < tx[2] ? 1 : 2);
+        if(ax == 2 || tx[ax] >= tmax || ++ntiles > ORACLE_MAX_TILES) break;
+        k[ax] += dir[ax] > 0 ? 1 : -1;
+      }
       if(leaves.n) {
         if(tn > t) { cloud_state = 0; tbox = tn; } else { cloud_state = 1; }
       }
@@ -1061,7 +1119,9 @@ raw_in_seg(const struct oracle_sky* s, const struct oseg* seg, int prop,
     int64_t ci[3];
     float v;
     for(a = 0; a < 3; ++a) {
-      const double f = floor((x[a] - s->scene.clouds.lower[a]) * s->inv_cs[a]);
+      /* relative to the segment's copy of the grid (-r) */
+      const double xa = a < 2 ? x[a] - seg->leaf.shift[a] : x[a];
+      const double f = floor((xa - s->scene.clouds.lower[a]) * s->inv_cs[a]);
       const int64_t lo = (int64_t)seg->leaf.lo[a] * s->B;
       const int64_t hi = (int64_t)seg->leaf.hi[a] * s->B - 1;
       int64_t i = f < (double)lo ? lo : f > (double)hi ? hi : (int64_t)f;

@@ -42,12 +42,13 @@ class Scene:
     ka: [sum nquads][nlayers] m^-1, ks: [nbands][nlayers] m^-1, T: [nlayers] K.
     lwc: [nz][ny][nx] g/m^3 over [lower, upper]; cext/ssa/g: per band.
     ground: None, or ("plane", z) with one material (albedo, temperature).
+    repeat_clouds: the cloud grid repeats along X and Y (option -r).
     """
 
     def __init__(self, *, bands=((550.0, 550.0),), quad_weights=None, nlayers=0, z_bottom=0.0,
                  dz=400.0, ka=None, ks=None, T=None, lwc=None, lower=(0, 0, 0), upper=(1, 1, 1),
                  cext=None, ssa=None, g=None, ground=None, albedo=0.3, ground_T=0.0,
-                 sun=(0.0, 60.0), optical_thickness=1.0, leaf_block=1):
+                 sun=(0.0, 60.0), optical_thickness=1.0, leaf_block=1, repeat_clouds=False):
         nb = len(bands)
         if quad_weights is None:
             quad_weights = [[1.0]] * nb
@@ -102,6 +103,7 @@ class Scene:
         d.sun_azimuth, d.sun_elevation = sun
         d.optical_thickness = optical_thickness
         d.leaf_block = leaf_block
+        d.repeat_clouds = int(repeat_clouds)
         self.desc = d
         self.lwc = lwc
         self.lower = np.asarray(lower, np.float64)

@@ -3,6 +3,7 @@ reference's options (src/atmosphere/htrdr_atmosphere_args.c:124-255) and its
 htrdr-image output (doc/htrdr-image.5). CPU tests cover argument handling;
 the GPU test renders through the CLI and compares its output with the
 Python mirror's dump of the same map."""
+import ctypes as C
 import io
 import os
 import subprocess
@@ -43,7 +44,6 @@ def test_help():
     (("-a", "synth:0", "-P", "height=0"), "invalid option argument"),
     (("-a", "synth:0", "-C", "focal-length=0.05"), "focal-length is not supported"),
     (("-a", "synth:0", "-C", "lens-radius=-1"), "invalid option argument"),
-    (("-a", "synth:0", "-r"), "not built"),
     (("-a", "synth:9", "-s", "sw=550,550"), "unknown synthetic configuration"),
 ])
 def test_bad_arguments(args, msg):
@@ -72,7 +72,7 @@ def parse_image(text):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("kind", ["image", "flux", "cie", "ortho", "lens"])
+@pytest.mark.parametrize("kind", ["image", "flux", "cie", "ortho", "lens", "repeat"])
 def test_cli_matches_library(tmp_path, kind):
     """Same scene and map through the CLI and through Context.draw_map +
     dump_buffer: identical columns 1-6 (the time columns are wall clocks).
@@ -90,6 +90,8 @@ def test_cli_matches_library(tmp_path, kind):
     elif kind == "ortho":
         extra = ["-s", "sw=550,550", "-P",
                  "pos=3200,-2000,5000:tgt=3200,-1999.5,4999.133974596:up=0,0,1:height=3000"]
+    elif kind == "repeat":
+        extra = ["-r", "-s", "sw=550,550"] + cam
     elif kind == "lens":
         extra = ["-s", "sw=550,550", "-C", "pos=3200,-2000,5000:tgt=3200,-1999.5,4999.133974596:"
                  "up=0,0,1:fov=60:lens-radius=50:focal-dst=4000"]
@@ -111,7 +113,7 @@ def test_cli_matches_library(tmp_path, kind):
         d.ortho_height = 3000.0
     if kind == "lens":
         d.lens_radius, d.focal_dst = 50.0, 4000.0
-    if kind in ("image", "cie", "ortho", "lens"):
+    if kind in ("image", "cie", "ortho", "lens", "repeat"):
         d.pos[:] = (3200, -2000, 5000)
         d.tgt[:] = (3200, -1999.5, 4999.133974596)
         d.fov_y = 60.0
@@ -121,7 +123,9 @@ def test_cli_matches_library(tmp_path, kind):
         d.tgt[:] = (3200, 3200, 2001)
         d.up[:] = (0, 1, 0)
         d.rect_size[:] = (6400.0, 6400.0)
-    with A.Context(s.scene_ptr) as ctx:
+    scene = F.SceneDesc.from_buffer_copy(s.scene)
+    scene.repeat_clouds = int(kind == "repeat")
+    with A.Context(C.pointer(scene)) as ctx:
         px = ctx.draw_map(d)
     buf = io.StringIO()
     A.dump_buffer(px, d, buf)
