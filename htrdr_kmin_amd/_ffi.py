@@ -84,7 +84,8 @@ class SceneDesc(C.Structure):
     _fields_ = [("spectral", SpectralDesc), ("gas", GasDesc), ("clouds", CloudDesc),
                 ("ground", GroundDesc), ("sun_azimuth", C.c_double),
                 ("sun_elevation", C.c_double), ("optical_thickness", C.c_double),
-                ("leaf_block", C.c_uint32), ("repeat_clouds", C.c_int)]
+                ("leaf_block", C.c_uint32), ("repeat_clouds", C.c_int),
+                ("repeat_ground", C.c_int)]
 
 
 class MapDesc(C.Structure):

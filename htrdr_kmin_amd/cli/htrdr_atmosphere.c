@@ -444,6 +444,7 @@ main(int argc, char** argv)
 
   scene = *htgx_synth_get_scene(synth);
   scene.repeat_clouds = a.repeat_clouds;
+  scene.repeat_ground = a.repeat_ground;
   if((res = htgx_ctx_create(&scene, &ctx)) != HTGX_OK) {
     fprintf(stderr, "%s: could not set up the scene on the GPU (%d)\n", argv[0], res);
     goto error;

@@ -429,6 +429,8 @@ htgx_ctx_init(htgx_ctx* ctx, const htgx_scene_desc& sc)
                         sc.ground.ntris, bvh)) != HTGX_OK)
       return res;
     S.nbvh = (uint32_t)bvh.nodes.size();
+    S.ground_repeat = sc.repeat_ground ? 1 : 0;
+    for(int a = 0; a < 3; ++a) { S.glo[a] = bvh.lo[a]; S.ghi[a] = bvh.hi[a]; }
     if((res = upload(ctx, bvh.nodes.data(), bvh.nodes.size(), &S.bvh))) return res;
     if((res = upload(ctx, bvh.tris.data(), bvh.tris.size(), &S.tris))) return res;
     if((res = upload(ctx, bvh.mtl.data(), bvh.mtl.size(), &S.tri_mtl))) return res;

@@ -75,6 +75,8 @@ struct DevScene {
   uint32_t nbvh;
   const BvhTri* tris;
   const uint32_t* tri_mtl;
+  int ground_repeat;         /* -R: the mesh repeats along X, Y */
+  double glo[3], ghi[3];     /* the mesh's bounding box */
   /* sun */
   double sun_dir[3];
   double sun_cos_half;
@@ -710,8 +712,10 @@ HTGX_D GroundHit ground_trace(const DevScene& S, const double o[3], const double
     const float of[3] = {(float)o[0], (float)o[1], (float)o[2]};
     const float df[3] = {(float)d[0], (float)d[1], (float)d[2]};
     const float r1f = r1 > (double)FLT_MAX ? INFINITY : (float)r1;
-    const MeshHit m = mesh_trace(S.bvh, S.nbvh, S.tris, of, df, (float)r0, r1f, prev_prim,
-                                 prev_edge, any);
+    const MeshHit m = S.ground_repeat
+      ? mesh_trace_repeated<double>(S.bvh, S.nbvh, S.tris, S.glo, S.ghi, o, d, r0, r1, prev_prim,
+                                    prev_edge, any)
+      : mesh_trace(S.bvh, S.nbvh, S.tris, of, df, (float)r0, r1f, prev_prim, prev_edge, any);
     if(m.tri >= 0) {
       const BvhTri& T = S.tris[m.tri];
       const float n0 = T.e1[1] * T.e2[2] - T.e1[2] * T.e2[1];

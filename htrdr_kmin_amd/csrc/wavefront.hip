@@ -98,6 +98,8 @@ struct SceneF {
   uint32_t nbvh;
   const BvhTri* tris;
   const uint32_t* tri_mtl;
+  int ground_repeat;         /* -R: the mesh repeats along X, Y */
+  float glo[3], ghi[3];      /* the mesh's bounding box */
   const float* mtl_albedo;
   const float* mtl_T;
   float sun_dir[3];
@@ -330,7 +332,10 @@ __device__ __forceinline__ GroundF ground_tracef(const SceneF& S, const float o[
       if(tt > 0.f && tt < r1) { g.hit = true; g.t = tt; g.prim = 0; g.mtl = S.plane_mtl; }
     }
   } else if(S.ground_type == HTGX_GROUND_MESH) {
-    const MeshHit m = mesh_trace(S.bvh, S.nbvh, S.tris, o, d, 0.f, r1, prev_prim, prev_edge, any);
+    const MeshHit m = S.ground_repeat
+      ? mesh_trace_repeated<float>(S.bvh, S.nbvh, S.tris, S.glo, S.ghi, o, d, 0.f, r1, prev_prim,
+                                   prev_edge, any)
+      : mesh_trace(S.bvh, S.nbvh, S.tris, o, d, 0.f, r1, prev_prim, prev_edge, any);
     g.nodes = m.nodes;
     g.ntris = m.ntris;
     if(m.tri >= 0) {
@@ -1500,6 +1505,8 @@ wavefront_build(htgx_ctx* ctx, htgx_wavefront_state* st)
   S.nbvh = D.nbvh;
   S.tris = D.tris;
   S.tri_mtl = D.tri_mtl;
+  S.ground_repeat = D.ground_repeat;
+  for(int a = 0; a < 3; ++a) { S.glo[a] = (float)D.glo[a]; S.ghi[a] = (float)D.ghi[a]; }
   if(D.ground_type == HTGX_GROUND_PLANE || D.ground_type == HTGX_GROUND_MESH) {
     const uint32_t nm = ctx->nmtls;
     if((res = upload_as_float(st, D.mtl_albedo, nm, &S.mtl_albedo))) return res;

@@ -218,6 +218,11 @@ struct htgx_scene_desc {
   int repeat_clouds;    /* option -r: the cloud grid repeats along X and Y
                            with the period of its extent (htsky's
                            repeat_clouds, src/atmosphere/htrdr_atmosphere.c:375) */
+  int repeat_ground;    /* option -R: a mesh ground repeats along X and Y with
+                           the period of its bounding box, traced copy after
+                           copy as htrdr_atmosphere_ground_trace_ray does
+                           (src/atmosphere/htrdr_atmosphere_ground.c:197-210);
+                           the plane ground is infinite already */
 };
 
 /*******************************************************************************

@@ -515,6 +515,7 @@ struct oracle_sky {
   double bs[3], inv_bs[3];     /* block size and inverse */
   double cext_max;
   double period[2];            /* -r repetition period along X, Y */
+  double glo[3], ghi[3];       /* bounding box of the mesh ground (-R period) */
   struct onode* root;
   uint64_t nnodes_alloc, nleaves;
   /* gas */
@@ -654,6 +655,19 @@ oracle_sky_create(const struct htgx_scene_desc* scene, struct oracle_sky** out)
   }
   s->inv_dz_layer = 1.0 / scene->gas.layer_thickness;
   sun_setup(s);
+  if(scene->ground.type == HTGX_GROUND_MESH) {
+    /* FP32 bounds of the vertices of the triangles */
+    uint32_t t;
+    int a, k;
+    for(a = 0; a < 3; ++a) { s->glo[a] = FLT_MAX; s->ghi[a] = -FLT_MAX; }
+    for(t = 0; t < scene->ground.ntris; ++t)
+      for(k = 0; k < 3; ++k)
+        for(a = 0; a < 3; ++a) {
+          const double v = scene->ground.verts[3 * (size_t)scene->ground.tris[3 * (size_t)t + k] + a];
+          if(v < s->glo[a]) s->glo[a] = v;
+          if(v > s->ghi[a]) s->ghi[a] = v;
+        }
+  }
 
   s->has_clouds = scene->clouds.def[0] && scene->clouds.def[1]
                && scene->clouds.def[2] && scene->clouds.lwc;
@@ -1369,6 +1383,10 @@ tri_hit_f(const float v0[3], const float e1[3], const float e2[3], const float o
  * plane is the quad repeated to infinity with -R, a single primitive; a mesh
  * is cast against every triangle in FP32, as the reference converts the ray
  * (src/core/htrdr_geometry.c:698-700). */
+static struct ghit mesh_trace_once
+  (const struct oracle_sky* s, const double o[3], const double d[3], const double range[2],
+   int64_t pprim, int pedge);
+
 static struct ghit
 ground_trace(const struct oracle_sky* s, const double o[3], const double d[3],
              const double range[2], const struct ghit* prev)
@@ -1386,7 +1404,53 @@ ground_trace(const struct oracle_sky* s, const double o[3], const double d[3],
       h.mtl = g->plane_mtl;
       h.prim = 0;
     }
+  } else if(g->type == HTGX_GROUND_MESH && s->scene.repeat_ground) {
+    /* -R: the copies of the mesh along X and Y, as htrdr_slab_trace_ray
+     * walks them (src/core/htrdr_slab.c:35-137; called at
+     * src/atmosphere/htrdr_atmosphere_ground.c:197-210 with 32 steps) */
+    const double* lo = s->glo;
+    const double* hi = s->ghi;
+    double tz0 = (lo[2] - o[2]) / d[2], tz1 = (hi[2] - o[2]) / d[2];
+    double sz[2], tmax[3], tdelta[2];
+    int64_t xy[2];
+    int incr[2], a, step;
+    if(tz0 > tz1) { const double tt = tz0; tz0 = tz1; tz1 = tt; }
+    tz0 = fmax(tz0, range[0]);
+    tz1 = fmin(tz1, range[1]);
+    if(tz0 > tz1) return h;
+    tmax[2] = tz1;
+    for(a = 0; a < 2; ++a) {
+      double low;
+      sz[a] = hi[a] - lo[a];
+      xy[a] = (int64_t)floor((o[a] + tz0 * d[a] - lo[a]) / sz[a]);
+      incr[a] = d[a] < 0 ? -1 : 1;
+      low = lo[a] + (double)xy[a] * sz[a];
+      tmax[a] = ((d[a] < 0 ? low : low + sz[a]) - o[a]) / d[a];
+      tdelta[a] = (d[a] < 0 ? -sz[a] : sz[a]) / d[a];
+    }
+    for(step = 0; step < 32; ++step) {
+      const double oc[3] = {o[0] - (double)xy[0] * sz[0], o[1] - (double)xy[1] * sz[1], o[2]};
+      struct ghit m = mesh_trace_once(s, oc, d, range, pprim, pedge);
+      if(m.hit) return m;
+      a = tmax[0] < tmax[1] ? (tmax[0] < tmax[2] ? 0 : 2) : (tmax[1] < tmax[2] ? 1 : 2);
+      if(a == 2 || tmax[a] >= range[1]) break;
+      tmax[a] += tdelta[a];
+      xy[a] += incr[a];
+    }
   } else if(g->type == HTGX_GROUND_MESH) {
+    return mesh_trace_once(s, o, d, range, pprim, pedge);
+  }
+  return h;
+}
+
+/* Closest mesh hit by brute force over every triangle, the ray in FP32 */
+static struct ghit
+mesh_trace_once(const struct oracle_sky* s, const double o[3], const double d[3],
+                const double range[2], int64_t pprim, int pedge)
+{
+  struct ghit h = GHIT_NONE;
+  const struct htgx_ground_desc* g = &s->scene.ground;
+  {
     const float of[3] = {(float)o[0], (float)o[1], (float)o[2]};
     const float df[3] = {(float)d[0], (float)d[1], (float)d[2]};
     const float r0 = (float)range[0];

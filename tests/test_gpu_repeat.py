@@ -89,3 +89,34 @@ def test_infinite_absorbing_layer(solver):
     E, SE = A.estimate(p["flux_sum"], p["flux_sum2"], p["flux_n"])
     rel = 2e-4 if solver == F.SOLVER_PLAIN_FP64 else 5e-4
     assert np.all(np.abs(E - expect) <= 4 * SE + rel * expect), (E.min(), E.max(), expect)
+
+
+@pytest.mark.parametrize("solver", [F.SOLVER_PLAIN_FP64, F.SOLVER_WAVEFRONT_FP32])
+def test_repeated_mesh_tiles_the_plane(solver):
+    from scenes import camera
+    from test_oracle_repeat import plane_radiance, repeated_quad_scene
+    sc = repeated_quad_scene(True)
+    d = camera(8, 8, 128, (5030.0, -2970.0, 800.0), (5035.0, -2968.0, 0.0), up=(0.0, 1.0, 0.0),
+               solver=solver)
+    with A.Context(sc.ptr) as ctx:
+        p = ctx.draw_map(d)
+    L = plane_radiance(60.0)
+    rel = 1e-9 if solver == F.SOLVER_PLAIN_FP64 else 2e-4
+    assert np.all(np.abs(p["radiance_E"] - L) <= 4 * p["radiance_SE"] + rel * L)
+
+
+def test_repeated_terrain_realisations():
+    """cfg0's clouds over a repeated 17x17-vertex fBm terrain (seams
+    included), looking across many copies of it"""
+    from test_oracle_ground import terrain
+    s, _, _, _ = terrain(res=17, relief=300.0, seed=4)
+    d = F.SceneDesc.from_buffer_copy(s.scene)
+    d.repeat_ground = 1
+    ptr = C.pointer(d)
+    with A.Context(ptr) as ctx:
+        sky = O.Sky(ptr)
+        desc = A.map_preset(0, width=24, height=16, spp=4, pos=(3200.0, -9000.0, 1500.0),
+                            tgt=(3200.0, -8000.0, 1200.0))
+        assert realisation_parity(ctx, sky, desc, n=500) >= 0.97
+        sky.close()
+    s.close()

@@ -84,3 +84,69 @@ def test_infinite_absorbing_layer_direct_flux():
         else:
             assert np.all(E > expect * 1.1)  # unshaded: the only copy is far away
         sky.close()
+
+
+def repeated_quad_scene(repeat, sun=(0.0, 60.0)):
+    """A 100 m x 60 m flat quad mesh at z = 0; repeated (-R), it tiles the
+    plane"""
+    from test_oracle_ground import with_mesh
+    v = [(0.0, 0.0, 0.0), (100.0, 0.0, 0.0), (100.0, 60.0, 0.0), (0.0, 60.0, 0.0)]
+    sc = with_mesh(Scene(sun=sun), v, [(0, 1, 2), (0, 2, 3)], [0, 0], [0.3], [0.0])
+    sc.desc.repeat_ground = int(repeat)
+    return sc
+
+
+def plane_radiance(el):
+    return L_SUN_550 * sun_solid_angle() * 0.3 / math.pi * cone_mean_cos(el) / 1e9
+
+
+def test_repeated_mesh_tiles_the_plane():
+    """Looking down far from the quad: nothing without -R; with -R, the
+    radiance of an infinite Lambertian plane, exactly (the copies tile it)"""
+    from scenes import camera
+    d = camera(4, 4, 64, (5030.0, -2970.0, 800.0), (5035.0, -2968.0, 0.0), up=(0.0, 1.0, 0.0))
+    for repeat in (False, True):
+        sc = repeated_quad_scene(repeat)  # the oracle reads its arrays
+        sky = O.Sky(sc.ptr)
+        acc = sky.draw_map(d, nthreads=4)
+        E, SE = accum_estimate(acc)
+        if repeat:
+            assert np.all(np.abs(E - plane_radiance(60.0)) <= 4 * SE + 1e-9 * plane_radiance(60.0))
+        else:
+            assert np.all(E == 0)
+        sky.close()
+
+
+def test_repeated_mesh_ray_casts():
+    """Ground casts through the copies: a downward ray hits the plane z = 0
+    at the distance of the plane, the copy's triangle it falls in (shifted
+    back into the original quad) being the hit triangle; upward rays from
+    below the copies hit them too (double-sided)"""
+    sc = repeated_quad_scene(True)
+    sky = O.Sky(sc.ptr)
+    rng = np.random.default_rng(4)
+    import ctypes as C
+    nhit = 0
+    for _ in range(300):
+        o = np.array([rng.uniform(-5000, 5000), rng.uniform(-5000, 5000), rng.uniform(10, 3000)])
+        d = rng.normal(size=3)
+        d[2] = -abs(d[2]) - 0.05
+        d /= np.linalg.norm(d)
+        if rng.random() < 0.3:  # from below, going up
+            o[2], d[2] = -o[2], -d[2]
+        ray = np.concatenate([o, d, [0.0, 1e30]])
+        t, prim, edge = C.c_double(), C.c_int64(), C.c_int()
+        hit = O.lib().oracle_ground_trace(sky.h, O.dptr(ray), -1, 0, C.byref(t), C.byref(prim), C.byref(edge))
+        if not hit:
+            # the walk spans at most 32 copies (src/core/htrdr_slab.c)
+            continue
+        nhit += 1
+        t_plane = -o[2] / d[2]
+        assert abs(t.value - t_plane) <= 1e-3 * max(1.0, t_plane), (t.value, t_plane)
+        p = o + t.value * d
+        u, v = (p[0] % 100.0) / 100.0, (p[1] % 60.0) / 60.0
+        below_diag = v <= u  # triangle (0, 1, 2) holds the points under v = u
+        if abs(u - v) > 1e-3:
+            assert prim.value == (0 if below_diag else 1)
+    assert nhit > 150
+    sky.close()
