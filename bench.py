@@ -55,9 +55,9 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--seed", type=int, default=0)
-    p.add_argument("--autotune-ms", type=float, default=6000.0,
+    p.add_argument("--autotune-ms", type=float, default=12000.0,
                    help="device-time budget of the launch-parameter autotuner run before the "
-                        "warm-up (FP32 solver, 8x for --config 3; 0: default parameters)")
+                        "warm-up (FP32 solver, 4x for --config 3; 0: default parameters)")
     p.add_argument("--launch-params", default="",
                    help="EV,CHUNK,GRID: fixed FP32 launch parameters instead of the autotuner "
                         "(0 = default for a field; profiles use the values the tuner picked)")
@@ -186,7 +186,7 @@ def bench_batch(args):
     tuned = None
     if args.autotune_ms > 0:
         # trials: the whole batch at 1/8 of each map's realisations per pixel
-        budget = 8.0 * args.autotune_ms
+        budget = 4.0 * args.autotune_ms
         params, rate, ntrials = A.batch_autotune(step_items, budget, spp_div=8)
         tuned = {"params": params, "trials": ntrials, "budget_ms": budget,
                  "trial_rate_mreal_s": round(rate / 1e6, 3)}

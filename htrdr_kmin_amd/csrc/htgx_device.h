@@ -258,9 +258,10 @@ HTGX_HD double cie_band_sample(int channel, double lmin, double len, double r1)
  * and htrdr_ran_wlen_cie_xyz_sample_X/Y/Z, htrdr_ran_wlen_cie_xyz.c:346-397.
  * The pdf is in nm^-1; for a CIE channel it is the constant 1 / integral of
  * the channel, as the reference returns it. */
+template<bool CIE = true>
 HTGX_D double sample_wavelength(const DevMap& M, double r0, double r1, double* pdf)
 {
-  if(M.wmode == 3) {
+  if(CIE && M.wmode == 3) {
     uint32_t lo = 0, hi = M.nwbands - 1;
     while(lo < hi) {
       const uint32_t mid = (lo + hi) >> 1;
@@ -709,13 +710,8 @@ HTGX_D GroundHit ground_trace(const DevScene& S, const double o[3], const double
       h.prim = 0;
     }
   } else if(S.ground_type == HTGX_GROUND_MESH) {
-    const float of[3] = {(float)o[0], (float)o[1], (float)o[2]};
-    const float df[3] = {(float)d[0], (float)d[1], (float)d[2]};
-    const float r1f = r1 > (double)FLT_MAX ? INFINITY : (float)r1;
-    const MeshHit m = S.ground_repeat
-      ? mesh_trace_repeated<double>(S.bvh, S.nbvh, S.tris, S.glo, S.ghi, o, d, r0, r1, prev_prim,
-                                    prev_edge, any)
-      : mesh_trace(S.bvh, S.nbvh, S.tris, of, df, (float)r0, r1f, prev_prim, prev_edge, any);
+    const MeshHit m = mesh_trace_copies<double>(S.bvh, S.nbvh, S.tris, S.ground_repeat != 0, S.glo,
+                                                S.ghi, o, d, r0, r1, prev_prim, prev_edge, any);
     if(m.tri >= 0) {
       const BvhTri& T = S.tris[m.tri];
       const float n0 = T.e1[1] * T.e2[2] - T.e1[2] * T.e2[1];

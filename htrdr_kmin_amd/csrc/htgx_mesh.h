@@ -119,68 +119,71 @@ HTGX_HD MeshHit mesh_trace(const BvhNode* nodes, uint32_t nnodes, const BvhTri* 
   return h;
 }
 
-/* The closest accepted hit on the copies of a mesh repeated along X and Y
- * with the period of its bounding box [lo, hi]: the walk of
+/* The closest accepted hit on a mesh, or with `repeat' on its copies along
+ * X and Y with the period of its bounding box [lo, hi]: the walk of
  * htrdr_slab_trace_ray (src/core/htrdr_slab.c:35-137) from the copy where
  * the ray enters the box's Z range, across at most 32 copies, each traced
  * with the origin relative to it and the whole range; the first copy with a
  * hit holds the closest one (htrdr_atmosphere_ground_trace_ray,
- * src/atmosphere/htrdr_atmosphere_ground.c:197-210). The previous hit is the
+ * src/atmosphere/htrdr_atmosphere_ground.c:185-210). The previous hit is the
  * same triangle in every copy, as in the reference. Real is the precision of
- * the walk (the ray is converted to FP32 per copy); the returned tri is -1
- * without a hit. */
+ * the walk; the ray is converted to FP32 per copy, as the reference does
+ * (src/core/htrdr_geometry.c:698-700). One trace site serves both cases, so
+ * the BVH walk is inlined once. The returned tri is -1 without a hit. */
 template<class Real>
-HTGX_HD MeshHit mesh_trace_repeated(const BvhNode* nodes, uint32_t nnodes, const BvhTri* tris,
-                                    const Real lo[3], const Real hi[3], const Real o[3],
-                                    const Real d[3], Real r0, Real r1, int32_t prev_tri,
-                                    bool prev_edge, bool any)
+HTGX_HD MeshHit mesh_trace_copies(const BvhNode* nodes, uint32_t nnodes, const BvhTri* tris,
+                                  bool repeat, const Real lo[3], const Real hi[3], const Real o[3],
+                                  const Real d[3], Real r0, Real r1, int32_t prev_tri,
+                                  bool prev_edge, bool any)
 {
-  MeshHit none;
-  none.t = 0.f;
-  none.u = none.v = 0.f;
-  none.tri = -1;
-  none.nodes = none.ntris = 0;
-  Real tz0 = (lo[2] - o[2]) / d[2], tz1 = (hi[2] - o[2]) / d[2];
-  if(tz0 > tz1) { const Real t = tz0; tz0 = tz1; tz1 = t; }
-  tz0 = tz0 > r0 ? tz0 : r0;
-  tz1 = tz1 < r1 ? tz1 : r1;
-  if(tz0 > tz1) return none;
-  Real sz[2], tmax[3], tdelta[2];
-  int64_t xy[2];
-  int incr[2];
-  tmax[2] = tz1;
+  Real sz[2] = {0, 0}, tmax[3] = {0, 0, 0}, tdelta[2] = {0, 0};
+  int64_t xy[2] = {0, 0};
+  int incr[2] = {1, 1};
+  if(repeat) {
+    Real tz0 = (lo[2] - o[2]) / d[2], tz1 = (hi[2] - o[2]) / d[2];
+    if(tz0 > tz1) { const Real t = tz0; tz0 = tz1; tz1 = t; }
+    tz0 = tz0 > r0 ? tz0 : r0;
+    tz1 = tz1 < r1 ? tz1 : r1;
+    if(tz0 > tz1) {
+      MeshHit none;
+      none.t = 0.f;
+      none.u = none.v = 0.f;
+      none.tri = -1;
+      none.nodes = none.ntris = 0;
+      return none;
+    }
+    tmax[2] = tz1;
 #pragma unroll
-  for(int a = 0; a < 2; ++a) {
-    sz[a] = hi[a] - lo[a];
-    xy[a] = (int64_t)floor((o[a] + tz0 * d[a] - lo[a]) / sz[a]);
-    incr[a] = d[a] < 0 ? -1 : 1;
-    const Real low = lo[a] + (Real)xy[a] * sz[a];
-    tmax[a] = ((d[a] < 0 ? low : low + sz[a]) - o[a]) / d[a];
-    tdelta[a] = (d[a] < 0 ? -sz[a] : sz[a]) / d[a];
+    for(int a = 0; a < 2; ++a) {
+      sz[a] = hi[a] - lo[a];
+      xy[a] = (int64_t)floor((o[a] + tz0 * d[a] - lo[a]) / sz[a]);
+      incr[a] = d[a] < 0 ? -1 : 1;
+      const Real low = lo[a] + (Real)xy[a] * sz[a];
+      tmax[a] = ((d[a] < 0 ? low : low + sz[a]) - o[a]) / d[a];
+      tdelta[a] = (d[a] < 0 ? -sz[a] : sz[a]) / d[a];
+    }
   }
   const float df[3] = {(float)d[0], (float)d[1], (float)d[2]};
   const float r0f = (float)r0;
   const float r1f = (double)r1 > (double)FLT_MAX ? INFINITY : (float)r1;
   uint32_t nodes_acc = 0, tris_acc = 0;
+  MeshHit m;
   for(int step = 0; step < 32; ++step) {
+    /* without repetition xy and sz are 0: the origin as given */
     const float of[3] = {(float)(o[0] - (Real)xy[0] * sz[0]), (float)(o[1] - (Real)xy[1] * sz[1]),
                          (float)o[2]};
-    MeshHit m = mesh_trace(nodes, nnodes, tris, of, df, r0f, r1f, prev_tri, prev_edge, any);
+    m = mesh_trace(nodes, nnodes, tris, of, df, r0f, r1f, prev_tri, prev_edge, any);
     nodes_acc += m.nodes;
     tris_acc += m.ntris;
-    if(m.tri >= 0) {
-      m.nodes = nodes_acc;
-      m.ntris = tris_acc;
-      return m;
-    }
+    if(m.tri >= 0 || !repeat) break;
     const int ax = tmax[0] < tmax[1] ? (tmax[0] < tmax[2] ? 0 : 2) : (tmax[1] < tmax[2] ? 1 : 2);
     if(ax == 2 || tmax[ax] >= r1) break;
     tmax[ax] += tdelta[ax];
     xy[ax] += incr[ax];
   }
-  none.nodes = nodes_acc;
-  none.ntris = tris_acc;
-  return none;
+  m.nodes = nodes_acc;
+  m.ntris = tris_acc;
+  return m;
 }
 
 } /* namespace htgx */

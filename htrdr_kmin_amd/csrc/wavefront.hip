@@ -332,10 +332,8 @@ __device__ __forceinline__ GroundF ground_tracef(const SceneF& S, const float o[
       if(tt > 0.f && tt < r1) { g.hit = true; g.t = tt; g.prim = 0; g.mtl = S.plane_mtl; }
     }
   } else if(S.ground_type == HTGX_GROUND_MESH) {
-    const MeshHit m = S.ground_repeat
-      ? mesh_trace_repeated<float>(S.bvh, S.nbvh, S.tris, S.glo, S.ghi, o, d, 0.f, r1, prev_prim,
-                                   prev_edge, any)
-      : mesh_trace(S.bvh, S.nbvh, S.tris, o, d, 0.f, r1, prev_prim, prev_edge, any);
+    const MeshHit m = mesh_trace_copies<float>(S.bvh, S.nbvh, S.tris, S.ground_repeat != 0, S.glo,
+                                               S.ghi, o, d, 0.f, r1, prev_prim, prev_edge, any);
     g.nodes = m.nodes;
     g.ntris = m.ntris;
     if(m.tri >= 0) {
@@ -799,7 +797,7 @@ __device__ __forceinline__ void consume(const SceneF& S, Lane& L, const float* u
 
 /* Sensor ray, wavelength, band and k-point, then the first walk
  * (..._draw_map.c:348-400 and :233-302) */
-template<int SENSOR, int SPECTRAL, bool COUNT>
+template<int SENSOR, int SPECTRAL, bool COUNT, bool FULL>
 __device__ __forceinline__ void real_begin(const SceneF& S, const DevMap& M, Lane& L, const float* u)
 {
   const uint32_t px = L.pixel % M.width, py = L.pixel / M.width;
@@ -862,7 +860,7 @@ __device__ __forceinline__ void real_begin(const SceneF& S, const DevMap& M, Lan
   }
 
   double pdf;
-  const double wlen = sample_wavelength(M, (double)u[4], (double)u[5], &pdf);
+  const double wlen = sample_wavelength<FULL>(M, (double)u[4], (double)u[5], &pdf);
   LC(pdf) = (float)(pdf * 1e9);
   LC(wlen) = (float)wlen;
   uint32_t band;
@@ -950,10 +948,22 @@ __device__ __forceinline__ uint64_t wave_sum64(uint64_t v)
   return v;
 }
 
-/* The persistent loop of one wavefront over the chunks of one map */
-template<int SENSOR, int SPECTRAL, bool COUNT>
-__device__ __forceinline__ void run_scene(const SceneF& S, const DevMap& M, const Launch& P)
+/* The persistent loop of one wavefront over the chunks of one map. Without
+ * FULL, the features that few maps use (cloud and ground repetition,
+ * orthographic and thin-lens cameras, CIE XYZ wavelength sampling) are
+ * compiled out: their flags become constants, so that their code and
+ * registers do not weigh on the common kernel. */
+template<int SENSOR, int SPECTRAL, bool COUNT, bool FULL>
+__device__ __forceinline__ void run_scene(const SceneF& S0, const DevMap& M0, const Launch& P)
 {
+  SceneF S = S0;
+  DevMap M = M0;
+  if(!FULL) {
+    S.repeat = 0;
+    S.ground_repeat = 0;
+    M.ortho = 0;
+    M.lens_radius = 0.0;
+  }
   /* canonicals per event: 8, plus 2 for the sun of a SW flux map */
   constexpr int NP = (SENSOR == HTGX_SENSOR_RECTANGLE && SPECTRAL == HTGX_SPECTRAL_SW) ? 5 : 4;
   Lane L;
@@ -1102,7 +1112,7 @@ __device__ __forceinline__ void run_scene(const SceneF& S, const DevMap& M, cons
         for(int i = 0; i < NP; ++i) { const float2 v = draw2(L, key); u[2 * i] = v.x; u[2 * i + 1] = v.y; }
         WF_TICK(3)
         if(begin) {
-          real_begin<SENSOR, SPECTRAL, COUNT>(S, M, L, u);
+          real_begin<SENSOR, SPECTRAL, COUNT, FULL>(S, M, L, u);
         }
         WF_TICK(4)
         if(!begin && L.pending) {
@@ -1191,11 +1201,11 @@ __device__ __forceinline__ void run_scene(const SceneF& S, const DevMap& M, cons
 
 /* 3 waves per SIMD: the register cap (168) spills a little but hides more
  * latency than 2 waves without spills (measured 14% faster at 1024^2) */
-template<int SENSOR, int SPECTRAL, bool COUNT>
+template<int SENSOR, int SPECTRAL, bool COUNT, bool FULL>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 wavefront_kernel(SceneF S, DevMap M, Launch P)
 {
-  run_scene<SENSOR, SPECTRAL, COUNT>(S, M, P);
+  run_scene<SENSOR, SPECTRAL, COUNT, FULL>(S, M, P);
 }
 
 /* One map of a batched launch */
@@ -1211,7 +1221,7 @@ struct BatchScene {
  * queues with the per-scene loop, then moves on, so the tails of the scenes
  * overlap instead of idling the device between launches. The scene is
  * uniform over a wavefront, so its constants stay scalar. */
-template<int SENSOR, int SPECTRAL>
+template<int SENSOR, int SPECTRAL, bool FULL>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 wavefront_batch_kernel(const BatchScene* __restrict__ scenes, uint32_t nscenes)
 {
@@ -1220,7 +1230,7 @@ wavefront_batch_kernel(const BatchScene* __restrict__ scenes, uint32_t nscenes)
     const unsigned int ex = __hip_atomic_load(B.P.exhausted, __ATOMIC_RELAXED,
                                               __HIP_MEMORY_SCOPE_AGENT);
     if(__builtin_amdgcn_readfirstlane(ex)) continue;
-    run_scene<SENSOR, SPECTRAL, false>(B.S, B.M, B.P);
+    run_scene<SENSOR, SPECTRAL, false, FULL>(B.S, B.M, B.P);
   }
 }
 
@@ -1340,17 +1350,31 @@ int launch_variant(F kernel, const SceneF& S, const DevMap& M, const Launch& P, 
   return HTGX_OK;
 }
 
-template<bool COUNT>
-int launch(const SceneF& S, const DevMap& M, const Launch& P, uint32_t grid, hipStream_t stream)
+/* Whether a map needs the FULL kernels (features compiled out of the
+ * common ones) */
+bool needs_full(const SceneF& S, const DevMap& M)
+{
+  return S.repeat || S.ground_repeat || M.ortho || M.lens_radius > 0 || M.wmode == 3;
+}
+
+template<bool COUNT, bool FULL>
+int launch_full(const SceneF& S, const DevMap& M, const Launch& P, uint32_t grid, hipStream_t stream)
 {
   if(M.sensor == HTGX_SENSOR_CAMERA) {
     if(M.spectral_type == HTGX_SPECTRAL_LW)
-      return launch_variant(wavefront_kernel<HTGX_SENSOR_CAMERA, HTGX_SPECTRAL_LW, COUNT>, S, M, P, grid, stream);
-    return launch_variant(wavefront_kernel<HTGX_SENSOR_CAMERA, HTGX_SPECTRAL_SW, COUNT>, S, M, P, grid, stream);
+      return launch_variant(wavefront_kernel<HTGX_SENSOR_CAMERA, HTGX_SPECTRAL_LW, COUNT, FULL>, S, M, P, grid, stream);
+    return launch_variant(wavefront_kernel<HTGX_SENSOR_CAMERA, HTGX_SPECTRAL_SW, COUNT, FULL>, S, M, P, grid, stream);
   }
   if(M.spectral_type == HTGX_SPECTRAL_LW)
-    return launch_variant(wavefront_kernel<HTGX_SENSOR_RECTANGLE, HTGX_SPECTRAL_LW, COUNT>, S, M, P, grid, stream);
-  return launch_variant(wavefront_kernel<HTGX_SENSOR_RECTANGLE, HTGX_SPECTRAL_SW, COUNT>, S, M, P, grid, stream);
+    return launch_variant(wavefront_kernel<HTGX_SENSOR_RECTANGLE, HTGX_SPECTRAL_LW, COUNT, FULL>, S, M, P, grid, stream);
+  return launch_variant(wavefront_kernel<HTGX_SENSOR_RECTANGLE, HTGX_SPECTRAL_SW, COUNT, FULL>, S, M, P, grid, stream);
+}
+
+template<bool COUNT>
+int launch(const SceneF& S, const DevMap& M, const Launch& P, uint32_t grid, hipStream_t stream)
+{
+  return needs_full(S, M) ? launch_full<COUNT, true>(S, M, P, grid, stream)
+                          : launch_full<COUNT, false>(S, M, P, grid, stream);
 }
 
 } /* namespace */
@@ -1522,7 +1546,7 @@ wavefront_build(htgx_ctx* ctx, htgx_wavefront_state* st)
   HIP_OK(hipGetDevice(&dev));
   HIP_OK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
   HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu,
-    wavefront_kernel<HTGX_SENSOR_CAMERA, HTGX_SPECTRAL_SW, false>, 256, 0));
+    wavefront_kernel<HTGX_SENSOR_CAMERA, HTGX_SPECTRAL_SW, false, false>, 256, 0));
   st->grid = st->grid_max = (uint32_t)std::max(1, ncu * std::max(1, per_cu));
   if(const char* e = std::getenv("HTGX_WF_GRID")) st->grid = (uint32_t)std::max(1, atoi(e));
   if(const char* e = std::getenv("HTGX_WF_EVTHRESH")) st->ev_thresh = (uint32_t)std::max(1, atoi(e));
@@ -1615,6 +1639,7 @@ struct htgx_batch_plan {
   int device = 0;
   uint32_t n = 0;
   int sensor = 0, spectral = 0;
+  bool full = false;  /* some map needs the FULL kernels */
   uint32_t grid = 0;
   uint32_t maxpix = 0;
   std::vector<uint32_t> order;      /* table slot -> item */
@@ -1650,6 +1675,23 @@ static int plan_alloc(htgx_batch_plan* p, size_t n, T** out)
   return HTGX_OK;
 }
 
+template<bool FULL>
+static void launch_batch(const htgx_batch_plan* p, hipStream_t stream)
+{
+  const uint32_t n = p->n;
+  if(p->sensor == HTGX_SENSOR_CAMERA) {
+    if(p->spectral == HTGX_SPECTRAL_LW)
+      wavefront_batch_kernel<HTGX_SENSOR_CAMERA, HTGX_SPECTRAL_LW, FULL><<<p->grid, 256, 0, stream>>>(p->table, n);
+    else
+      wavefront_batch_kernel<HTGX_SENSOR_CAMERA, HTGX_SPECTRAL_SW, FULL><<<p->grid, 256, 0, stream>>>(p->table, n);
+  } else {
+    if(p->spectral == HTGX_SPECTRAL_LW)
+      wavefront_batch_kernel<HTGX_SENSOR_RECTANGLE, HTGX_SPECTRAL_LW, FULL><<<p->grid, 256, 0, stream>>>(p->table, n);
+    else
+      wavefront_batch_kernel<HTGX_SENSOR_RECTANGLE, HTGX_SPECTRAL_SW, FULL><<<p->grid, 256, 0, stream>>>(p->table, n);
+  }
+}
+
 /* Launches of one step on `stream': no allocation, no host copy, no memset
  * (so that a HIP graph can record them) */
 static int plan_launch_step(htgx_batch_plan* p, hipStream_t stream)
@@ -1660,17 +1702,8 @@ static int plan_launch_step(htgx_batch_plan* p, hipStream_t stream)
   zero_u32_kernel<<<(2 * n + 255) / 256, 256, 0, stream>>>(p->flags, 2 * n);
   batch_tick_kernel<<<1, 1, 0, stream>>>(p->ticks + n, p->flags + 2 * n);
   HIP_OK(hipGetLastError());
-  if(p->sensor == HTGX_SENSOR_CAMERA) {
-    if(p->spectral == HTGX_SPECTRAL_LW)
-      wavefront_batch_kernel<HTGX_SENSOR_CAMERA, HTGX_SPECTRAL_LW><<<p->grid, 256, 0, stream>>>(p->table, n);
-    else
-      wavefront_batch_kernel<HTGX_SENSOR_CAMERA, HTGX_SPECTRAL_SW><<<p->grid, 256, 0, stream>>>(p->table, n);
-  } else {
-    if(p->spectral == HTGX_SPECTRAL_LW)
-      wavefront_batch_kernel<HTGX_SENSOR_RECTANGLE, HTGX_SPECTRAL_LW><<<p->grid, 256, 0, stream>>>(p->table, n);
-    else
-      wavefront_batch_kernel<HTGX_SENSOR_RECTANGLE, HTGX_SPECTRAL_SW><<<p->grid, 256, 0, stream>>>(p->table, n);
-  }
+  if(p->full) launch_batch<true>(p, stream);
+  else launch_batch<false>(p, stream);
   HIP_OK(hipGetLastError());
   batch_merge_kernel<<<dim3((p->maxpix + 255) / 256, n), 256, 0, stream>>>(p->table);
   batch_advance_kernel<<<(n + 255) / 256, 256, 0, stream>>>(p->table, n, p->flags + 2 * n);
@@ -1737,6 +1770,7 @@ int wf_plan_build(htgx_ctx* const* ctxs, const DevMap* Ms, htgx_mc_accum* const*
       }
     }
     p->maxpix = std::max(p->maxpix, Ms[i].width * Ms[i].height);
+    p->full = p->full || needs_full(e.S, e.M);
     p->us_per_tick[k] = ctxs[i]->us_per_tick;
   }
   HIP_OK(hipMemcpy(p->table, p->table_h.data(), n * sizeof(BatchScene), hipMemcpyHostToDevice));
@@ -1845,12 +1879,12 @@ struct TrialClock {
   }
 };
 
-/* Coordinate descent over the event threshold, then the chunk size, then the
- * grid. trial(p, &ms) runs one timed launch with parameters p; a candidate
- * replaces the current parameters only when more than 1% faster (trial
- * timings repeat to a fraction of that). The first launch is an untimed
- * warm-up; the search stops before a trial that would exceed the budget,
- * estimated from the last one. */
+/* Coordinate descent over the chunk size, then the event threshold, then the
+ * grid. trial(p, &ms) runs one timed launch with parameters p; each
+ * candidate is timed twice and keeps its faster time (single launches vary
+ * by 1-2%), and replaces the current parameters only when more than 1%
+ * faster. The first launch is an untimed warm-up; the search stops before a
+ * trial that would exceed the budget, estimated from the last one. */
 template<class Trial>
 static int tune_search(Trial&& trial, double budget_ms, uint32_t grid_max, htgx_wf_params* cur,
                        double* cur_ms, uint32_t* ntrials)
@@ -1858,27 +1892,31 @@ static int tune_search(Trial&& trial, double budget_ms, uint32_t grid_max, htgx_
   static const uint32_t thresholds[] = {8, 16, 24, 32, 40, 48};
   static const uint32_t chunks[] = {2, 4, 8};
   const uint32_t grids[] = {grid_max, std::max(1u, grid_max * 3 / 4), std::max(1u, grid_max / 2)};
-  double ms = 0;
+  double ms = 0, ms2 = 0;
   int res;
   if((res = trial(*cur, &ms)) != HTGX_OK) return res;
   if((res = trial(*cur, &ms)) != HTGX_OK) return res;
-  double spent = ms, last = ms;
+  if((res = trial(*cur, &ms2)) != HTGX_OK) return res;
+  ms = std::min(ms, ms2);
+  double spent = 2 * ms, last = 2 * ms;
   uint32_t n = 1;
   *cur_ms = ms;
   for(int dim = 0; dim < 3; ++dim) {
-    const uint32_t* vals = dim == 0 ? thresholds : dim == 1 ? chunks : grids;
-    const int nv = dim == 0 ? 6 : 3;
+    const uint32_t* vals = dim == 0 ? chunks : dim == 1 ? thresholds : grids;
+    const int nv = dim == 1 ? 6 : 3;
     const htgx_wf_params base = *cur;
     for(int i = 0; i < nv; ++i) {
       htgx_wf_params p = base;
-      if(dim == 0) p.ev_thresh = vals[i];
-      else if(dim == 1) p.chunk = vals[i];
+      if(dim == 0) p.chunk = vals[i];
+      else if(dim == 1) p.ev_thresh = vals[i];
       else p.grid = vals[i];
       if(p.ev_thresh == base.ev_thresh && p.chunk == base.chunk && p.grid == base.grid) continue;
       if(spent + last > budget_ms) { dim = 3; break; }
       if((res = trial(p, &ms)) != HTGX_OK) return res;
-      spent += ms;
-      last = ms;
+      if((res = trial(p, &ms2)) != HTGX_OK) return res;
+      spent += ms + ms2;
+      last = ms + ms2;
+      ms = std::min(ms, ms2);
       ++n;
       if(ms < 0.99 * *cur_ms) { *cur = p; *cur_ms = ms; }
     }
