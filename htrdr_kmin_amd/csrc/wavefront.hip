@@ -1048,7 +1048,7 @@ __device__ __forceinline__ void run_scene(const SceneF& S0, const DevMap& M0, co
           const float proba = (L.mode == MODE_TR ? k - L.s_kmin : k) * L.inv_maj;
           L.ev = uu.x < proba ? EV_HIT : EV_NONE;
           /* gas vs particle scattering at the event, ..._sw.c:427-438 */
-          if(L.mode == MODE_SC) LC(gas_ratio) = gks / ks;
+          if(L.mode == MODE_SC && L.ev != EV_NONE) LC(gas_ratio) = gks / ks;
         }
         if(L.ev != EV_NONE) {
           LC(ev_dst) = L.tc;
