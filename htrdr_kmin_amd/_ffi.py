@@ -22,6 +22,8 @@ Ka, Ks, Kext = 0, 1, 2
 SENSOR_CAMERA, SENSOR_RECTANGLE = 0, 1
 SOLVER_PLAIN_FP64, SOLVER_WAVEFRONT_FP32 = 0, 1
 GROUND_NONE, GROUND_PLANE, GROUND_MESH = 0, 1, 2
+BRDF_LAMBERTIAN, BRDF_SPECULAR = 0, 1
+BRDF_LIST_WAVELENGTHS, BRDF_LIST_BANDS = 0, 1
 
 c_double_p = C.POINTER(C.c_double)
 c_uint32_p = C.POINTER(C.c_uint32)
@@ -73,11 +75,18 @@ class CloudDesc(C.Structure):
                 ("hg_g", c_double_p)]
 
 
+class Brdf(C.Structure):
+    _fields_ = [("wlen", C.c_double * 2), ("type", C.c_int), ("reflectivity", C.c_double)]
+
+
 class GroundDesc(C.Structure):
     _fields_ = [("type", C.c_int), ("plane_z", C.c_double), ("plane_mtl", C.c_uint32),
                 ("nverts", C.c_uint32), ("ntris", C.c_uint32), ("verts", c_float_p),
                 ("tris", c_uint32_p), ("tri_mtl", c_uint32_p), ("nmtls", C.c_uint32),
-                ("mtl_albedo", c_double_p), ("mtl_temperature", c_double_p)]
+                ("mtl_albedo", c_double_p), ("mtl_temperature", c_double_p),
+                ("mtl_brdf_first", c_uint32_p), ("mtl_brdf_list", C.POINTER(C.c_int32)),
+                ("brdfs", C.POINTER(Brdf)), ("ninterfaces", C.c_uint32),
+                ("interfaces", C.POINTER(C.c_int32)), ("sky_mtl", C.c_int32)]
 
 
 class SceneDesc(C.Structure):
@@ -178,6 +187,12 @@ SIGNATURES = {
     "htgx_synth_create": (C.c_int, [C.POINTER(SynthParams), C.POINTER(C.c_void_p)]),
     "htgx_synth_get_scene": (C.POINTER(SceneDesc), [C.c_void_p]),
     "htgx_synth_destroy": (C.c_int, [C.c_void_p]),
+    "htgx_ground_files_load": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p)]),
+    "htgx_ground_files_get_desc": (C.POINTER(GroundDesc), [C.c_void_p]),
+    "htgx_ground_files_material_name": (C.c_char_p, [C.c_void_p, C.c_uint32]),
+    "htgx_ground_files_interface_name": (C.c_char_p, [C.c_void_p, C.c_uint32]),
+    "htgx_ground_files_destroy": (C.c_int, [C.c_void_p]),
+    "htgx_files_last_error": (C.c_char_p, []),
 }
 
 _lib = None

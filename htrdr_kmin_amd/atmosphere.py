@@ -48,6 +48,54 @@ def accum_planes(desc):
 assert ACCUM_DTYPE.itemsize == C.sizeof(F.McAccum) == 64
 
 
+class GroundFiles:
+    """The ground of option -g (an htrdr-obj file) with the materials of
+    option -M (an htrdr-materials file and its mrumtl files) and the sky
+    material of option -n, read by the library (include/htgx_files.h).
+
+    ``desc`` is an htgx_ground_desc owned by this object; ``with_scene``
+    returns a copy of a scene description with this ground.
+    """
+
+    def __init__(self, obj_path, materials_path, sky_mtl="air"):
+        lib = F.load()
+        self._lib = lib
+        self._h = C.c_void_p()
+        res = lib.htgx_ground_files_load(str(obj_path).encode(), str(materials_path).encode(),
+                                         None if sky_mtl is None else sky_mtl.encode(), C.byref(self._h))
+        if res != F.OK:
+            raise F.HtgxError(f"htgx_ground_files_load ({lib.htgx_files_last_error().decode()})", res)
+        self.desc = lib.htgx_ground_files_get_desc(self._h).contents
+        self.material_names = [lib.htgx_ground_files_material_name(self._h, m).decode()
+                               for m in range(self.desc.nmtls)]
+        self.interface_names = [lib.htgx_ground_files_interface_name(self._h, i).decode()
+                                for i in range(self.desc.ninterfaces)]
+
+    def with_scene(self, scene):
+        """A copy of the htgx_scene_desc ``scene`` whose ground is this one
+        (valid while this object is open)"""
+        d = F.SceneDesc.from_buffer_copy(scene)
+        d.ground = self.desc
+        return d
+
+    def close(self):
+        if self._h:
+            self._lib.htgx_ground_files_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Synth:
     """Seeded synthetic scene (benchmark configuration ``config``)."""
 

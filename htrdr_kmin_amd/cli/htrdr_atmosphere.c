@@ -5,23 +5,27 @@
  * (src/atmosphere/htrdr_atmosphere_draw_map.c:593-641) and writing the same
  * htrdr-image (doc/htrdr-image.5) as dump_buffer (:477-587).
  *
- * The atmosphere inputs: the htgop (-a), htcp (-c), htmie (-m) and
- * htrdr-obj/-materials (-g, -M) files are read by htsky and star-3d in the
- * reference; no reader for these formats exists in this build, so those
- * options take a synthetic source instead of a path:
+ * The ground of -g is an htrdr-obj file with the htrdr-materials list of -M
+ * and the sky material of -n, read by htgx_ground_files_load
+ * (include/htgx_files.h). The atmosphere inputs, the htgop (-a), htcp (-c)
+ * and htmie (-m) files, are read by htsky in the reference; no reader for
+ * these formats exists in this build, so those options take a synthetic
+ * source instead of a path:
  *   -a synth:<config>[,<seed>]   gas layers and spectral data of a seeded
  *                                benchmark configuration (0, 1, 2, 4)
  *   -c synth                     its cloud field (the Mie data come with it,
  *                                so -m is not required)
- *   -g plane | synth             an infinite plane (the quad repeated with -R)
- *                                or the configuration's fBm heightfield
+ * and -g also takes
+ *   plane | synth                an infinite plane (the quad repeated with -R)
+ *                                or the configuration's fBm heightfield.
  * Options without a counterpart on the GPU path are rejected with an error
- * (-d, -C focal-length) or accepted and ignored (-t, -V, -O, -n).
+ * (-d, -C focal-length) or accepted and ignored (-t, -V, -O).
  * The solver is the FP32 wavefront one; HTGX_SOLVER=plain selects the FP64
  * plain one.
  */
 #define _POSIX_C_SOURCE 200809L
 #include "htgx.h"
+#include "htgx_files.h"
 #include "htgx_synth.h"
 
 #include <errno.h>
@@ -39,6 +43,7 @@ struct args {
   const char* mie;
   const char* ground;
   const char* mtl;
+  const char* sky_mtl;
   const char* output;
   int force, verbose, repeat_ground, repeat_clouds;
   int output_type; /* 0 image (-C), 1 flux map (-p) */
@@ -65,7 +70,7 @@ usage(void)
 "                        [-c clouds [-m mie]] [-O cache] [-o output] [-t threads]\n"
 "                        -a gas\n"
 "Sources without file readers on this GPU path: -a synth:<config>[,<seed>],\n"
-"-c synth, -g plane|synth. HTGX_SOLVER=plain selects the FP64 solver.\n");
+"-c synth; -g also takes plane|synth. HTGX_SOLVER=plain selects the FP64 solver.\n");
 }
 
 static int
@@ -237,7 +242,7 @@ parse_args(int argc, char** argv, struct args* a)
       case 'i': bad = parse_list(optarg, image_param, a); break;
       case 'M': a->mtl = optarg; break;
       case 'm': a->mie = optarg; break;
-      case 'n': break; /* the sky material is implicit */
+      case 'n': a->sky_mtl = optarg; break;
       case 'O': break; /* no octree cache: the octree is rebuilt in a fraction of a second */
       case 'o': a->output = optarg; break;
       case 'P': a->output_type = 0; a->ortho = 1; bad = parse_list(optarg, ortho_param, a); break;
@@ -259,6 +264,10 @@ parse_args(int argc, char** argv, struct args* a)
   }
   if(!a->gas) {
     fprintf(stderr, "missing the path of the gas optical properties file -- option '-a'\n");
+    return HTGX_BAD_ARG;
+  }
+  if(a->ground && strcmp(a->ground, "synth") && strcmp(a->ground, "plane") && !a->mtl) {
+    fprintf(stderr, "missing the path of the file listing the ground materials -- option '-M'\n");
     return HTGX_BAD_ARG;
   }
   if(a->clouds && strcmp(a->clouds, "synth") && !a->mie) {
@@ -351,6 +360,7 @@ main(int argc, char** argv)
   struct htgx_synth_params sp;
   struct htgx_synth* synth = NULL;
   struct htgx_ctx* ctx = NULL;
+  struct htgx_ground_files* ground = NULL;
   struct htgx_map_desc d;
   struct htgx_scene_desc scene;
   struct htgx_accum tacc = {0, 0, 0, 0}, facc = {0, 0, 0, 0};
@@ -375,9 +385,10 @@ main(int argc, char** argv)
     return 1;
   }
   if(a.ground && strcmp(a.ground, "synth") && strcmp(a.ground, "plane")) {
-    fprintf(stderr, "%s: `%s': htrdr-obj files cannot be read on this path; use plane or synth\n",
-            argv[0], a.ground);
-    return 1;
+    if(htgx_ground_files_load(a.ground, a.mtl, a.sky_mtl ? a.sky_mtl : "air", &ground) != HTGX_OK) {
+      fprintf(stderr, "%s: %s\n", argv[0], htgx_files_last_error());
+      return 1;
+    }
   }
   if(htgx_synth_params_preset(config, &sp) != HTGX_OK) {
     fprintf(stderr, "%s: unknown synthetic configuration %d\n", argv[0], config);
@@ -388,7 +399,7 @@ main(int argc, char** argv)
   sp.sun_elevation = a.sun_el;
   sp.optical_thickness = a.thickness;
   if(!a.clouds) sp.cloud_def[0] = sp.cloud_def[1] = sp.cloud_def[2] = 0;
-  if(!a.ground) sp.ground_type = HTGX_GROUND_NONE;
+  if(!a.ground || ground) sp.ground_type = HTGX_GROUND_NONE;
   else if(!strcmp(a.ground, "plane")) sp.ground_type = HTGX_GROUND_PLANE;
   else {
     if(sp.ground_type != HTGX_GROUND_MESH) {
@@ -398,6 +409,7 @@ main(int argc, char** argv)
   }
   if((res = htgx_synth_create(&sp, &synth)) != HTGX_OK) {
     fprintf(stderr, "%s: could not create the synthetic atmosphere (%d)\n", argv[0], res);
+    htgx_ground_files_destroy(ground);
     return 1;
   }
 
@@ -438,11 +450,13 @@ main(int argc, char** argv)
       fprintf(stderr, "%s: cannot open `%s' for writing%s\n", argv[0], a.output,
               a.force ? "" : " (use -f to overwrite)");
       htgx_synth_destroy(synth);
+      htgx_ground_files_destroy(ground);
       return 1;
     }
   }
 
   scene = *htgx_synth_get_scene(synth);
+  if(ground) scene.ground = *htgx_ground_files_get_desc(ground);
   scene.repeat_clouds = a.repeat_clouds;
   scene.repeat_ground = a.repeat_ground;
   if((res = htgx_ctx_create(&scene, &ctx)) != HTGX_OK) {
@@ -478,6 +492,7 @@ error:
   free(pixels);
   if(ctx) htgx_ctx_destroy(ctx);
   htgx_synth_destroy(synth);
+  htgx_ground_files_destroy(ground);
   if(out != stdout) fclose(out);
   return res == HTGX_OK ? 0 : 1;
 }

@@ -284,6 +284,53 @@ int upload(htgx_ctx* ctx, const T* host, size_t n, const T** out)
 /*******************************************************************************
  * Context
  ******************************************************************************/
+/* Ground materials: grey albedos, or BRDF lists (sorted, reflectivities in
+ * [0, 1]); interfaces whose materials exist and of which one reflects */
+static int
+check_materials(const htgx_ground_desc& g)
+{
+  if(!g.nmtls) return HTGX_BAD_ARG;
+  if(g.mtl_brdf_first) {
+    if(!g.mtl_brdf_list || g.mtl_brdf_first[0] != 0) return HTGX_BAD_ARG;
+    if(g.mtl_brdf_first[g.nmtls] && !g.brdfs) return HTGX_BAD_ARG;
+    for(uint32_t m = 0; m < g.nmtls; ++m) {
+      const uint32_t b0 = g.mtl_brdf_first[m], b1 = g.mtl_brdf_first[m + 1];
+      const int list = g.mtl_brdf_list[m];
+      if(b1 < b0) return HTGX_BAD_ARG;
+      if(list != HTGX_BRDF_LIST_WAVELENGTHS && list != HTGX_BRDF_LIST_BANDS) return HTGX_BAD_ARG;
+      for(uint32_t i = b0; i < b1; ++i) {
+        const htgx_brdf& e = g.brdfs[i];
+        if(e.type != HTGX_BRDF_LAMBERTIAN && e.type != HTGX_BRDF_SPECULAR) return HTGX_BAD_ARG;
+        if(!(e.reflectivity >= 0 && e.reflectivity <= 1)) return HTGX_BAD_ARG;
+        if(!(e.wlen[0] >= 0)) return HTGX_BAD_ARG;
+        if(list == HTGX_BRDF_LIST_WAVELENGTHS ? e.wlen[1] != e.wlen[0] : !(e.wlen[1] >= e.wlen[0]))
+          return HTGX_BAD_ARG;
+        if(i > b0) {
+          const htgx_brdf& p = g.brdfs[i - 1];
+          if(list == HTGX_BRDF_LIST_WAVELENGTHS ? !(e.wlen[0] > p.wlen[0]) : !(e.wlen[0] >= p.wlen[1]))
+            return HTGX_BAD_ARG;
+        }
+      }
+    }
+  } else {
+    if(!g.mtl_albedo) return HTGX_BAD_ARG;
+  }
+  if(g.interfaces) {
+    if(!g.ninterfaces) return HTGX_BAD_ARG;
+    if(g.sky_mtl < -1 || g.sky_mtl >= (int32_t)g.nmtls) return HTGX_BAD_ARG;
+    auto reflects = [&](int32_t m) {
+      return !g.mtl_brdf_first || g.mtl_brdf_first[m + 1] > g.mtl_brdf_first[m];
+    };
+    for(uint32_t i = 0; i < g.ninterfaces; ++i) {
+      const int32_t* f = g.interfaces + 3 * (size_t)i;
+      if(f[0] < 0 || f[0] >= (int32_t)g.nmtls || f[1] < 0 || f[1] >= (int32_t)g.nmtls) return HTGX_BAD_ARG;
+      if(f[2] < -1 || f[2] >= (int32_t)g.nmtls) return HTGX_BAD_ARG;
+      if(!reflects(f[0]) && !reflects(f[1]) && !(f[2] >= 0 && reflects(f[2]))) return HTGX_BAD_ARG;
+    }
+  }
+  return HTGX_OK;
+}
+
 static int
 check_scene(const htgx_scene_desc* s)
 {
@@ -302,16 +349,19 @@ check_scene(const htgx_scene_desc* s)
   const bool clouds = c.def[0] || c.def[1] || c.def[2];
   if(clouds && (!c.def[0] || !c.def[1] || !c.def[2] || !c.lwc || !c.cext || !c.ssa || !c.hg_g))
     return HTGX_BAD_ARG;
-  if(s->ground.type == HTGX_GROUND_PLANE) {
-    if(!s->ground.nmtls || s->ground.plane_mtl >= s->ground.nmtls || !s->ground.mtl_albedo)
-      return HTGX_BAD_ARG;
-  } else if(s->ground.type == HTGX_GROUND_MESH) {
+  if(s->ground.type == HTGX_GROUND_PLANE || s->ground.type == HTGX_GROUND_MESH) {
     const htgx_ground_desc& g = s->ground;
-    if(!g.nverts || !g.ntris || !g.verts || !g.tris || !g.tri_mtl || !g.nmtls || !g.mtl_albedo)
-      return HTGX_BAD_ARG;
-    for(uint32_t t = 0; t < g.ntris; ++t) {
-      if(g.tri_mtl[t] >= g.nmtls) return HTGX_BAD_ARG;
-      for(int k = 0; k < 3; ++k) if(g.tris[3 * (size_t)t + k] >= g.nverts) return HTGX_BAD_ARG;
+    int res = check_materials(g);
+    if(res != HTGX_OK) return res;
+    const uint32_t nsurf = g.interfaces ? g.ninterfaces : g.nmtls;
+    if(g.type == HTGX_GROUND_PLANE) {
+      if(g.plane_mtl >= nsurf) return HTGX_BAD_ARG;
+    } else {
+      if(!g.nverts || !g.ntris || !g.verts || !g.tris || !g.tri_mtl) return HTGX_BAD_ARG;
+      for(uint32_t t = 0; t < g.ntris; ++t) {
+        if(g.tri_mtl[t] >= nsurf) return HTGX_BAD_ARG;
+        for(int k = 0; k < 3; ++k) if(g.tris[3 * (size_t)t + k] >= g.nverts) return HTGX_BAD_ARG;
+      }
     }
   } else if(s->ground.type != HTGX_GROUND_NONE) {
     return HTGX_BAD_ARG;
@@ -416,12 +466,55 @@ htgx_ctx_init(htgx_ctx* ctx, const htgx_scene_desc& sc)
   /* Ground */
   S.ground_type = sc.ground.type;
   if(sc.ground.type == HTGX_GROUND_PLANE || sc.ground.type == HTGX_GROUND_MESH) {
-    S.plane_z = sc.ground.plane_z;
-    S.plane_mtl = sc.ground.plane_mtl;
-    if((res = upload(ctx, sc.ground.mtl_albedo, sc.ground.nmtls, &S.mtl_albedo))) return res;
-    if(sc.ground.mtl_temperature)
-      if((res = upload(ctx, sc.ground.mtl_temperature, sc.ground.nmtls, &S.mtl_T))) return res;
-    ctx->nmtls = sc.ground.nmtls;
+    const htgx_ground_desc& g = sc.ground;
+    S.plane_z = g.plane_z;
+    S.plane_mtl = g.plane_mtl;
+    if(g.mtl_albedo)
+      if((res = upload(ctx, g.mtl_albedo, g.nmtls, &S.mtl_albedo))) return res;
+    if(g.mtl_temperature)
+      if((res = upload(ctx, g.mtl_temperature, g.nmtls, &S.mtl_T))) return res;
+    ctx->nmtls = g.nmtls;
+    S.gmat = g.mtl_brdf_first || g.interfaces;
+    if(S.gmat) {
+      /* the general model: every material a BRDF list (a grey albedo is a
+       * one-entry Lambertian list), every surface an interface (material m
+       * alone is {front m, back the sky}, the sky a material with no BRDF
+       * appended after the others) */
+      std::vector<DevMtl> mtls(g.nmtls + 1);
+      std::vector<DevBrdf> brdfs;
+      for(uint32_t m = 0; m < g.nmtls; ++m) {
+        DevMtl& d = mtls[m];
+        d.first = (uint32_t)brdfs.size();
+        d.T = g.mtl_temperature ? g.mtl_temperature[m] : 0.0;
+        if(g.mtl_brdf_first) {
+          d.list = g.mtl_brdf_list[m];
+          for(uint32_t i = g.mtl_brdf_first[m]; i < g.mtl_brdf_first[m + 1]; ++i) {
+            const htgx_brdf& e = g.brdfs[i];
+            brdfs.push_back(DevBrdf{e.wlen[0], e.wlen[1], e.reflectivity, e.type, 0});
+          }
+        } else {
+          d.list = HTGX_BRDF_LIST_WAVELENGTHS;
+          brdfs.push_back(DevBrdf{0.0, 0.0, g.mtl_albedo[m], HTGX_BRDF_LAMBERTIAN, 0});
+        }
+        d.count = (uint32_t)brdfs.size() - d.first;
+      }
+      mtls[g.nmtls] = DevMtl{(uint32_t)brdfs.size(), 0, HTGX_BRDF_LIST_WAVELENGTHS, 0, 0.0};
+      std::vector<DevIfc> ifcs;
+      if(g.interfaces) {
+        for(uint32_t i = 0; i < g.ninterfaces; ++i)
+          ifcs.push_back(DevIfc{g.interfaces[3 * i], g.interfaces[3 * i + 1], g.interfaces[3 * i + 2], 0});
+        S.sky_mtl = g.sky_mtl;
+        S.explicit_ifc = 1;
+      } else {
+        for(uint32_t m = 0; m < g.nmtls; ++m) ifcs.push_back(DevIfc{(int32_t)m, (int32_t)g.nmtls, -1, 0});
+        S.sky_mtl = (int32_t)g.nmtls;
+        S.explicit_ifc = 0;
+      }
+      if(brdfs.empty()) brdfs.push_back(DevBrdf{0.0, 0.0, 0.0, 0, 0}); /* no zero-size upload */
+      if((res = upload(ctx, mtls.data(), mtls.size(), &S.mtls))) return res;
+      if((res = upload(ctx, brdfs.data(), brdfs.size(), &S.brdfs))) return res;
+      if((res = upload(ctx, ifcs.data(), ifcs.size(), &S.ifcs))) return res;
+    }
   }
   if(sc.ground.type == HTGX_GROUND_MESH) {
     BvhHost bvh;

@@ -37,6 +37,25 @@ constexpr double kPi = 3.14159265358979323846;
 constexpr uint32_t kLeafIdBits = 28;
 constexpr uint32_t kLeafIdMask = (1u << kLeafIdBits) - 1u;
 
+/* Ground materials in the general model (spectral BRDF lists and two-sided
+ * interfaces, as htrdr-materials, mrumtl and htrdr-obj files give them) */
+struct DevMtl {
+  uint32_t first, count; /* BRDF entries; none: semi-transparent (the sky) */
+  int32_t list;          /* enum htgx_brdf_list */
+  int32_t pad;
+  double T;              /* emission temperature, K */
+};
+
+struct DevBrdf {
+  double lo, hi;         /* nm: [w, w] or the band */
+  double refl;
+  int32_t type, pad;     /* enum htgx_brdf_type */
+};
+
+struct DevIfc {
+  int32_t front, back, thin, pad; /* material indices; thin -1: none */
+};
+
 struct DevScene {
   /* spectral */
   uint32_t nbands;
@@ -77,6 +96,13 @@ struct DevScene {
   const uint32_t* tri_mtl;
   int ground_repeat;         /* -R: the mesh repeats along X, Y */
   double glo[3], ghi[3];     /* the mesh's bounding box */
+  /* general materials: when set, tri_mtl and plane_mtl index ifcs */
+  int gmat;
+  const DevMtl* mtls;
+  const DevBrdf* brdfs;
+  const DevIfc* ifcs;
+  int32_t sky_mtl;           /* the sky's material (-1: none) */
+  int explicit_ifc;          /* interfaces given, not made from materials */
   /* sun */
   double sun_dir[3];
   double sun_cos_half;
@@ -729,6 +755,63 @@ HTGX_D GroundHit ground_trace(const DevScene& S, const double o[3], const double
   return h;
 }
 
+/* Material reflecting a hit of interface f by a ray along dir, front_hit
+ * telling that the ray meets the interface's front side, i.e. that
+ * N . dir > 0 for the normal N = (v1 - v0) x (v2 - v0) (the front normal of
+ * htrdr-obj(5) is -N): the thin material if it has a BRDF, else the material
+ * of the other side, else the hit side's (htrdr_interface_fetch_hit_mtl,
+ * src/core/htrdr_interface.h:45-87) */
+HTGX_D int32_t interface_mtl(const DevIfc& f, const DevMtl* mtls, bool front_hit)
+{
+  if(f.thin >= 0 && mtls[f.thin].count) return f.thin;
+  const int32_t m = front_hit ? f.back : f.front;
+  return mtls[m].count ? m : (front_hit ? f.front : f.back);
+}
+
+/* BRDF entry of material m at wavelength wlen (nm) with the canonical r, as
+ * mrumtl_fetch_brdf selects it (called at src/core/htrdr_materials.c:430-433):
+ * the band holding wlen, or one of the two entries around it, the upper one
+ * with probability u = wlen's linear position between them (r < u); the
+ * first or last entry outside the list */
+HTGX_D const DevBrdf& brdf_fetch(const DevMtl& m, const DevBrdf* brdfs, double wlen, double r)
+{
+  const DevBrdf* e = brdfs + m.first;
+  const uint32_t n = m.count;
+  const bool bands = m.list == HTGX_BRDF_LIST_BANDS;
+  /* first entry whose upper bound is >= wlen */
+  uint32_t lo = 0, hi = n;
+  while(lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if(e[mid].hi < wlen) lo = mid + 1; else hi = mid;
+  }
+  if(lo == n) return e[n - 1];
+  if(lo == 0 || (bands && e[lo].lo <= wlen)) return e[lo];
+  const double below = bands ? e[lo - 1].hi : e[lo - 1].lo;
+  const double u = (wlen - below) / (e[lo].lo - below);
+  return r < u ? e[lo] : e[lo - 1];
+}
+
+/* Mirror direction of dir about the unit normal N */
+HTGX_D void reflect3(const double dir[3], const double N[3], double out[3])
+{
+  const double c = 2.0 * (dir[0] * N[0] + dir[1] * N[1] + dir[2] * N[2]);
+  out[0] = dir[0] - c * N[0];
+  out[1] = dir[1] - c * N[1];
+  out[2] = dir[2] - c * N[2];
+}
+
+/* A rectangle sensor's sample whose up ray hits the ground with `h' lies
+ * inside the ground if the side the ray travels in is not the sky's
+ * (..._draw_map.c:75-100): with interfaces, the front (cos(up, -N) < 0) or
+ * back material is compared with the sky's; else the ray must not meet the
+ * side N points to from below (N . up >= 0) */
+HTGX_D bool sensor_under_ground(const DevScene& S, const GroundHit& h)
+{
+  if(!S.explicit_ifc) return h.N[2] >= 0;
+  const DevIfc& f = S.ifcs[h.mtl];
+  return (h.N[2] > 0 ? f.front : f.back) != S.sky_mtl;
+}
+
 HTGX_D bool in_solar_cone(const DevScene& S, const double dir[3])
 {
   return dot3(dir, S.sun_dir) >= S.sun_cos_half;
@@ -808,7 +891,7 @@ struct Path {
   bool rejected;
   /* walk */
   int stage, mask;
-  bool surface, rayleigh, gh_hit;
+  bool surface, rayleigh, gh_hit, specular;
   bool prev_edge, gh_edge;
   int32_t prev_prim, gh_prim; /* previous surface hit (-1: none) and current hit */
   uint32_t gh_mtl;
@@ -923,13 +1006,27 @@ HTGX_D void stage_consume(const DevScene& S, Path& P, Stream& rng, const TrackOu
     case ST_ABS: {
       P.Tr_abs = out.Tr;
       if(P.Tr_abs <= 0) { P.stage = ST_WALK_DONE; break; }
+      P.specular = false;
       if(P.surface) {
+        if(S.gmat) {
+          /* the hit side's material and its BRDF at the wavelength,
+           * ..._sw.c:403-411 */
+          const int32_t m = interface_mtl(S.ifcs[P.gh_mtl], S.mtls, dot3(P.N, P.dir) > 0);
+          const DevBrdf& b = brdf_fetch(S.mtls[m], S.brdfs, P.wlen, rng.canonical());
+          P.albedo = b.refl;
+          P.specular = b.type == HTGX_BRDF_SPECULAR;
+        } else {
+          P.albedo = S.mtl_albedo[P.gh_mtl];
+        }
         if(-(P.dir[0]*P.N[0] + P.dir[1]*P.N[1] + P.dir[2]*P.N[2]) < 0) {
           P.N[0] = -P.N[0]; P.N[1] = -P.N[1]; P.N[2] = -P.N[2];
         }
-        P.albedo = S.mtl_albedo[P.gh_mtl];
-        const double u0 = rng.canonical(), u1 = rng.canonical();
-        cos_hemisphere(P.N, u0, u1, P.dir_next);
+        if(P.specular) {
+          reflect3(P.dir, P.N, P.dir_next);
+        } else {
+          const double u0 = rng.canonical(), u1 = rng.canonical();
+          cos_hemisphere(P.N, u0, u1, P.dir_next);
+        }
         P.refl = P.albedo;
       } else {
         P.rayleigh = rng.canonical() < P.gas_ratio;
@@ -938,11 +1035,17 @@ HTGX_D void stage_consume(const DevScene& S, Path& P, Stream& rng, const TrackOu
         local_to_world(P.dir, mu, 2.0 * kPi * u1, P.dir_next);
         P.refl = 1;
       }
-      P.sun_pdf = sun_sample(S, rng, P.sun_dir);
-      if(P.surface) {
+      if(P.specular) {
+        /* the mirror direction is the sun direction, ..._sw.c:446-455 */
+        P.sun_pdf = 1;
+        P.sun_dir[0] = P.dir_next[0]; P.sun_dir[1] = P.dir_next[1]; P.sun_dir[2] = P.dir_next[2];
+        P.R = !in_solar_cone(S, P.sun_dir) || dot3(P.N, P.sun_dir) < 0 ? 0 : P.albedo;
+      } else if(P.surface) {
+        P.sun_pdf = sun_sample(S, rng, P.sun_dir);
         const double c = dot3(P.N, P.sun_dir);
         P.R = c < 0 ? 0 : P.albedo / kPi * c;
       } else {
+        P.sun_pdf = sun_sample(S, rng, P.sun_dir);
         const double mu = dot3(P.dir, P.sun_dir);
         P.R = P.rayleigh ? rayleigh_eval(mu) : hg_eval(P.g, mu);
       }
@@ -975,14 +1078,28 @@ HTGX_D void stage_consume(const DevScene& S, Path& P, Stream& rng, const TrackOu
         P.prev_prim = -1; /* ..._lw.c:236 */
         P.prev_edge = false;
       } else {
+        double refl, T;
+        bool specular = false;
+        if(S.gmat) { /* ..._lw.c:249-252 */
+          const int32_t m = interface_mtl(S.ifcs[P.gh_mtl], S.mtls, dot3(P.N, P.dir) > 0);
+          const DevBrdf& b = brdf_fetch(S.mtls[m], S.brdfs, P.wlen, rng.canonical());
+          refl = b.refl;
+          specular = b.type == HTGX_BRDF_SPECULAR;
+          T = S.mtls[m].T;
+        } else {
+          refl = S.mtl_albedo[P.gh_mtl];
+          T = S.mtl_T ? S.mtl_T[P.gh_mtl] : 0.0;
+        }
         if(-(P.dir[0]*P.N[0] + P.dir[1]*P.N[1] + P.dir[2]*P.N[2]) < 0) {
           P.N[0] = -P.N[0]; P.N[1] = -P.N[1]; P.N[2] = -P.N[2];
         }
-        const double u0 = rng.canonical(), u1 = rng.canonical();
-        cos_hemisphere(P.N, u0, u1, P.dir_next);
-        const double refl = S.mtl_albedo[P.gh_mtl];
+        if(specular) {
+          reflect3(P.dir, P.N, P.dir_next);
+        } else {
+          const double u0 = rng.canonical(), u1 = rng.canonical();
+          cos_hemisphere(P.N, u0, u1, P.dir_next);
+        }
         if(rng.canonical() >= refl) {
-          const double T = S.mtl_T ? S.mtl_T[P.gh_mtl] : 0.0;
           P.w = T > 0 ? planck_mono(P.wlen * 1e-9, T) : 0;
           P.stage = ST_WALK_DONE;
           break;
@@ -1064,7 +1181,7 @@ HTGX_D void real_begin(const DevScene& S, const DevMap& M, Path& P, Stream& rng,
       P.org[a] = M.pos[a] + M.axis_x[a] * (sx * 2.0 - 1.0) + M.axis_y[a] * (sy * 2.0 - 1.0);
     const double up[3] = {0, 0, 1};
     const GroundHit h = ground_trace(S, P.org, up, 0.0, DBL_MAX, -1, false);
-    if(h.hit && dot3(up, h.N) >= 0) { P.rejected = true; P.stage = ST_REAL_DONE; return; }
+    if(h.hit && sensor_under_ground(S, h)) { P.rejected = true; P.stage = ST_REAL_DONE; return; }
     const double u0 = rng.canonical(), u1 = rng.canonical();
     cos_hemisphere(M.z, u0, u1, P.dir0);
   }

@@ -102,6 +102,14 @@ struct SceneF {
   float glo[3], ghi[3];      /* the mesh's bounding box */
   const float* mtl_albedo;
   const float* mtl_T;
+  /* general materials (the FP64 solver's tables): tri_mtl and plane_mtl
+   * then index interfaces */
+  int gmat;
+  int explicit_ifc;
+  int32_t sky_mtl;
+  const DevMtl* mtls;
+  const DevBrdf* brdfs;
+  const DevIfc* ifcs;
   float sun_dir[3];
   float sun_cos_half;
   float sun_solid_angle;
@@ -363,6 +371,14 @@ __device__ __forceinline__ void ground_normalf(const SceneF& S, int32_t prim, fl
     N[0] = N[1] = 0.f;
     N[2] = 1.f;
   }
+}
+
+/* Mirror direction of dir about the unit normal N */
+__device__ __forceinline__ void reflectf(const float dir[3], const float N[3], float out[3])
+{
+  const float c = 2.f * dot3f(dir, N);
+#pragma unroll
+  for(int a = 0; a < 3; ++a) out[a] = dir[a] - c * N[a];
 }
 
 /* The normal turned to face the incoming direction's origin side
@@ -704,14 +720,28 @@ __device__ __forceinline__ void consume(const SceneF& S, Lane& L, const float* u
       float N[3] = {0.f, 0.f, 1.f};
 #pragma unroll
       for(int a = 0; a < 3; ++a) L.pos[a] = L.pos[a] + L.dir[a] * dst;
+      bool specular = false;
       if(L.surface) {
         ground_normalf(S, LC(gh_prim), N);
+        if(S.gmat) {
+          /* the hit side's material and its BRDF at the wavelength,
+           * ..._sw.c:403-411 (the canonical of the phase choice) */
+          const int32_t m = interface_mtl(S.ifcs[LC(gh_mtl)], S.mtls, dot3f(N, L.dir) > 0.f);
+          const DevBrdf& b = brdf_fetch(S.mtls[m], S.brdfs, (double)LC(wlen), (double)u[4]);
+          LC(refl) = (float)b.refl;
+          specular = b.type == HTGX_BRDF_SPECULAR;
+        } else {
+          LC(refl) = S.mtl_albedo[LC(gh_mtl)];
+        }
         face_forwardf(N, L.dir);
         float dn[3];
-        local_to_worldf(N, sqrtf(1.f - u[0]), 2.f * kPif * u[1], dn);
+        if(specular) {
+          reflectf(L.dir, N, dn);
+        } else {
+          local_to_worldf(N, sqrtf(1.f - u[0]), 2.f * kPif * u[1], dn);
+        }
 #pragma unroll
         for(int a = 0; a < 3; ++a) LCA(dir_next, a) = dn[a];
-        LC(refl) = S.mtl_albedo[LC(gh_mtl)];
       } else {
         L.rayleigh = u[4] < LC(gas_ratio);
         const float mu = L.rayleigh ? rayleigh_cosf(u[0]) : hg_cosf(LC(g), u[0]);
@@ -723,7 +753,14 @@ __device__ __forceinline__ void consume(const SceneF& S, Lane& L, const float* u
       }
       float sun_dir[3];
       local_to_worldf(S.sun_dir, 1.f - u[2] * (1.f - S.sun_cos_half), 2.f * kPif * u[3], sun_dir);
-      if(L.surface) {
+      if(specular) {
+        /* the mirror direction is the sun direction, of pdf 1 instead of
+         * 1 / solid angle (..._sw.c:446-455) */
+#pragma unroll
+        for(int a = 0; a < 3; ++a) sun_dir[a] = LCA(dir_next, a);
+        const bool lit = dot3f(sun_dir, S.sun_dir) >= S.sun_cos_half && dot3f(N, sun_dir) >= 0.f;
+        LC(R) = lit ? LC(refl) / S.sun_solid_angle : 0.f;
+      } else if(L.surface) {
         const float c = dot3f(N, sun_dir);
         LC(R) = c < 0.f ? 0.f : LC(refl) / kPif * c;
       } else {
@@ -774,10 +811,25 @@ __device__ __forceinline__ void consume(const SceneF& S, Lane& L, const float* u
       } else {
         float N[3];
         ground_normalf(S, LC(gh_prim), N);
+        float refl, T;
+        bool specular = false;
+        if(S.gmat) { /* ..._lw.c:249-252 */
+          const int32_t m = interface_mtl(S.ifcs[LC(gh_mtl)], S.mtls, dot3f(N, L.dir) > 0.f);
+          const DevBrdf& b = brdf_fetch(S.mtls[m], S.brdfs, (double)LC(wlen), (double)u[4]);
+          refl = (float)b.refl;
+          specular = b.type == HTGX_BRDF_SPECULAR;
+          T = (float)S.mtls[m].T;
+        } else {
+          refl = S.mtl_albedo[LC(gh_mtl)];
+          T = S.mtl_T ? S.mtl_T[LC(gh_mtl)] : 0.f;
+        }
         face_forwardf(N, L.dir);
-        local_to_worldf(N, sqrtf(1.f - u[0]), 2.f * kPif * u[1], dn);
-        if(u[6] >= S.mtl_albedo[LC(gh_mtl)]) {
-          const float T = S.mtl_T ? S.mtl_T[LC(gh_mtl)] : 0.f;
+        if(specular) {
+          reflectf(L.dir, N, dn);
+        } else {
+          local_to_worldf(N, sqrtf(1.f - u[0]), 2.f * kPif * u[1], dn);
+        }
+        if(u[6] >= refl) {
           LC(w) = T > 0.f ? planckf(LC(wlen), T) : 0.f;
           LC(stage) = ST_WALK_DONE;
           break;
@@ -852,8 +904,17 @@ __device__ __forceinline__ void real_begin(const SceneF& S, const DevMap& M, Lan
     if(gu.hit) {
       float N[3];
       ground_normalf(S, gu.prim, N);
-      /* cos(up, N) >= 0: the up ray travels in the ground's back side */
-      if(N[2] >= 0.f) { LC(stage) = ST_REAL_DONE; LC(nfail)++; LC(w) = -1.f; return; }
+      /* with interfaces: the material of the side the up ray travels in
+       * (front if cos(up, -N) < 0) must be the sky's; else the ray must not
+       * meet the side N points to from below */
+      bool under;
+      if(S.explicit_ifc) {
+        const DevIfc& f = S.ifcs[gu.mtl];
+        under = (N[2] > 0.f ? f.front : f.back) != S.sky_mtl;
+      } else {
+        under = N[2] >= 0.f;
+      }
+      if(under) { LC(stage) = ST_REAL_DONE; LC(nfail)++; LC(w) = -1.f; return; }
     }
     const float z[3] = {(float)M.z[0], (float)M.z[1], (float)M.z[2]};
     local_to_worldf(z, sqrtf(1.f - u[2]), 2.f * kPif * u[3], dir);
@@ -950,7 +1011,8 @@ __device__ __forceinline__ uint64_t wave_sum64(uint64_t v)
 
 /* The persistent loop of one wavefront over the chunks of one map. Without
  * FULL, the features that few maps use (cloud and ground repetition,
- * orthographic and thin-lens cameras, CIE XYZ wavelength sampling) are
+ * spectral BRDFs and interfaces, orthographic and thin-lens cameras, CIE
+ * XYZ wavelength sampling) are
  * compiled out: their flags become constants, so that their code and
  * registers do not weigh on the common kernel. */
 template<int SENSOR, int SPECTRAL, bool COUNT, bool FULL>
@@ -961,6 +1023,8 @@ __device__ __forceinline__ void run_scene(const SceneF& S0, const DevMap& M0, co
   if(!FULL) {
     S.repeat = 0;
     S.ground_repeat = 0;
+    S.gmat = 0;
+    S.explicit_ifc = 0;
     M.ortho = 0;
     M.lens_radius = 0.0;
   }
@@ -1354,7 +1418,7 @@ int launch_variant(F kernel, const SceneF& S, const DevMap& M, const Launch& P, 
  * common ones) */
 bool needs_full(const SceneF& S, const DevMap& M)
 {
-  return S.repeat || S.ground_repeat || M.ortho || M.lens_radius > 0 || M.wmode == 3;
+  return S.repeat || S.ground_repeat || S.gmat || M.ortho || M.lens_radius > 0 || M.wmode == 3;
 }
 
 template<bool COUNT, bool FULL>
@@ -1533,9 +1597,15 @@ wavefront_build(htgx_ctx* ctx, htgx_wavefront_state* st)
   for(int a = 0; a < 3; ++a) { S.glo[a] = (float)D.glo[a]; S.ghi[a] = (float)D.ghi[a]; }
   if(D.ground_type == HTGX_GROUND_PLANE || D.ground_type == HTGX_GROUND_MESH) {
     const uint32_t nm = ctx->nmtls;
-    if((res = upload_as_float(st, D.mtl_albedo, nm, &S.mtl_albedo))) return res;
+    if(D.mtl_albedo && (res = upload_as_float(st, D.mtl_albedo, nm, &S.mtl_albedo))) return res;
     if(D.mtl_T && (res = upload_as_float(st, D.mtl_T, nm, &S.mtl_T))) return res;
   }
+  S.gmat = D.gmat;
+  S.explicit_ifc = D.explicit_ifc;
+  S.sky_mtl = D.sky_mtl;
+  S.mtls = D.mtls;
+  S.brdfs = D.brdfs;
+  S.ifcs = D.ifcs;
   for(int a = 0; a < 3; ++a) S.sun_dir[a] = (float)D.sun_dir[a];
   S.sun_cos_half = (float)D.sun_cos_half;
   S.sun_solid_angle = (float)D.sun_solid_angle;

@@ -186,22 +186,77 @@ struct htgx_cloud_desc {
   const double* hg_g; /* [nbands] */
 };
 
+/* One entry of a material's spectral BRDF list (an mrumtl file: a BRDF per
+ * wavelength or per band, src/core/htrdr_materials.c:415-463) */
+enum htgx_brdf_type {
+  HTGX_BRDF_LAMBERTIAN = 0, /* ssf_lambertian_reflection */
+  HTGX_BRDF_SPECULAR = 1    /* ssf_specular_reflection, constant Fresnel term */
+};
+
+enum htgx_brdf_list {
+  HTGX_BRDF_LIST_WAVELENGTHS = 0, /* entries at wavelengths, strictly increasing */
+  HTGX_BRDF_LIST_BANDS = 1        /* entries over [lo, hi] bands, increasing,
+                                     not overlapping (they may touch) */
+};
+
+struct htgx_brdf {
+  double wlen[2];      /* nm: [w, w] in a wavelength list, [lo, hi] in a band list */
+  int type;            /* enum htgx_brdf_type */
+  double reflectivity; /* in [0, 1] */
+};
+
 /* Ground. The plane ground is one material; a mesh carries one material index
- * per triangle. Materials are Lambertian reflectors with an albedo and a
- * temperature (the emission temperature used by the longwave walk,
- * ..._lw.c:266-270). */
+ * per triangle. By default materials are grey Lambertian reflectors with an
+ * albedo and a temperature (the emission temperature used by the longwave
+ * walk, ..._lw.c:266-270), seen the same from both sides of a surface.
+ *
+ * Materials as htrdr-materials, mrumtl and htrdr-obj files give them
+ * (src/core/htrdr_materials.c, src/core/htrdr_geometry.c:177-566) use the
+ * optional fields at the end, each left zero for the default:
+ *  - spectral BRDFs: with mtl_brdf_first set, material m reflects with the
+ *    entries brdfs[mtl_brdf_first[m] .. mtl_brdf_first[m + 1]), a list of
+ *    kind mtl_brdf_list[m], and mtl_albedo is ignored. At wavelength lambda
+ *    the list gives the entry of the band holding lambda, or one of the two
+ *    entries around lambda, the upper one with the probability of lambda's
+ *    linear position between them (mrumtl_fetch_brdf with a canonical,
+ *    src/core/htrdr_materials.c:430-433); the first or last entry outside
+ *    the list. A material with no entry is semi-transparent (`none', e.g.
+ *    the air);
+ *  - interfaces: with `interfaces' set, tri_mtl[t] and plane_mtl index
+ *    interfaces, not materials. Interface i is the material of its front
+ *    side interfaces[3i], of its back side interfaces[3i+1], and a thin
+ *    material interfaces[3i+2] (-1: none). The front side of triangle
+ *    (v0, v1, v2) is the side from which its vertices turn clockwise, the
+ *    one (v2 - v0) x (v1 - v0) points to (htrdr-obj(5)); the plane's front
+ *    side is below it. A hit from one side reflects with the material of
+ *    the other side, or with the thin material if it has a BRDF; if the
+ *    material of the other side has none, with the material of the hit side
+ *    (src/core/htrdr_interface.h:45-87). At least one of the three must
+ *    have a BRDF. Without `interfaces', material m is the interface whose
+ *    front is m and whose back is the sky;
+ *  - sky_mtl: with `interfaces' set, the material of the sky (option -n).
+ *    A rectangle sensor's sample whose up ray meets the ground from a side
+ *    whose material is not the sky is rejected (..._draw_map.c:75-100). */
 struct htgx_ground_desc {
   int type;              /* enum htgx_ground_type */
   double plane_z;        /* HTGX_GROUND_PLANE */
-  uint32_t plane_mtl;    /* HTGX_GROUND_PLANE material index */
+  uint32_t plane_mtl;    /* HTGX_GROUND_PLANE material (or interface) index */
   uint32_t nverts;       /* HTGX_GROUND_MESH */
   uint32_t ntris;
   const float* verts;    /* [nverts][3] m */
   const uint32_t* tris;  /* [ntris][3] */
-  const uint32_t* tri_mtl; /* [ntris] */
+  const uint32_t* tri_mtl; /* [ntris] material (or interface) index */
   uint32_t nmtls;
-  const double* mtl_albedo;      /* [nmtls] */
-  const double* mtl_temperature; /* [nmtls] K */
+  const double* mtl_albedo;      /* [nmtls]; may be NULL with mtl_brdf_first */
+  const double* mtl_temperature; /* [nmtls] K; may be NULL (0 K) */
+  /* spectral BRDFs */
+  const uint32_t* mtl_brdf_first; /* [nmtls + 1], mtl_brdf_first[0] = 0 */
+  const int32_t* mtl_brdf_list;   /* [nmtls] enum htgx_brdf_list */
+  const struct htgx_brdf* brdfs;  /* [mtl_brdf_first[nmtls]] */
+  /* interfaces */
+  uint32_t ninterfaces;
+  const int32_t* interfaces;      /* [ninterfaces][3] front, back, thin */
+  int32_t sky_mtl;                /* -1: no material is the sky */
 };
 
 struct htgx_scene_desc {

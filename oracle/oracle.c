@@ -1511,6 +1511,124 @@ oracle_ground_trace(const struct oracle_sky* s, const double ray[8], int64_t pre
 }
 
 /*******************************************************************************
+ * Ground materials: spectral BRDF lists and two-sided interfaces
+ ******************************************************************************/
+static int
+general_materials(const struct htgx_ground_desc* g)
+{
+  return g->mtl_brdf_first != NULL || g->interfaces != NULL;
+}
+
+/* BRDF entries of material m; m = -1 is the sky side of a surface given by a
+ * lone material (an interface {m, sky}) */
+static uint32_t
+brdf_count(const struct htgx_ground_desc* g, int32_t m)
+{
+  if(m < 0) return 0;
+  if(!g->mtl_brdf_first) return 1;
+  return g->mtl_brdf_first[m + 1] - g->mtl_brdf_first[m];
+}
+
+static void
+interface_of(const struct htgx_ground_desc* g, uint32_t i, int32_t f[3])
+{
+  if(g->interfaces) {
+    f[0] = g->interfaces[3 * (size_t)i];
+    f[1] = g->interfaces[3 * (size_t)i + 1];
+    f[2] = g->interfaces[3 * (size_t)i + 2];
+  } else {
+    f[0] = (int32_t)i; f[1] = -1; f[2] = -1;
+  }
+}
+
+/* src/core/htrdr_interface.h:45-87. N is the hit normal (v1 - v0) x (v2 - v0);
+ * the front normal of the s3d hit, towards the side from which the vertices
+ * turn clockwise (htrdr-obj(5)), is -N. */
+static int32_t
+hit_material(const struct htgx_ground_desc* g, uint32_t ifc, const double N[3], const double dir[3])
+{
+  const double Nf[3] = {-N[0], -N[1], -N[2]};
+  int32_t f[3], m;
+  int back_hit;
+  interface_of(g, ifc, f);
+  if(f[2] >= 0 && brdf_count(g, f[2])) return f[2];   /* thin material */
+  back_hit = !(dot3(Nf, dir) < 0);                     /* :67 */
+  m = back_hit ? f[0] : f[1];         /* the material on the other side */
+  if(!brdf_count(g, m)) m = back_hit ? f[1] : f[0];    /* :75-84 */
+  return m;
+}
+
+/* mrumtl_fetch_brdf(mrumtl, wavelength, r), src/core/htrdr_materials.c:430-433
+ * (mrumtl 0.2, not vendored; restated from its documented behaviour): in a
+ * band list, the band holding wlen; otherwise, between the entries i-1 and i
+ * around wlen (wavelengths, or the upper end of band i-1 and the lower end
+ * of band i), entry i if r < u = wlen's linear position between them, else
+ * i-1; before the first entry the first, after the last the last. A grey
+ * albedo is one Lambertian entry. */
+static void
+brdf_at(const struct htgx_ground_desc* g, int32_t m, double wlen, double r, int* type,
+        double* refl)
+{
+  const struct htgx_brdf* e;
+  uint32_t n, i;
+  if(!g->mtl_brdf_first) { *type = HTGX_BRDF_LAMBERTIAN; *refl = g->mtl_albedo[m]; return; }
+  e = g->brdfs + g->mtl_brdf_first[m];
+  n = brdf_count(g, m);
+  if(g->mtl_brdf_list[m] == HTGX_BRDF_LIST_BANDS) {
+    for(i = 0; i < n && e[i].wlen[1] < wlen; ++i) {}
+    if(i == n) {
+      i = n - 1;
+    } else if(i > 0 && wlen < e[i].wlen[0]) {
+      const double u = (wlen - e[i - 1].wlen[1]) / (e[i].wlen[0] - e[i - 1].wlen[1]);
+      if(!(r < u)) --i;
+    }
+  } else {
+    for(i = 0; i < n && e[i].wlen[0] < wlen; ++i) {}
+    if(i == n) {
+      i = n - 1;
+    } else if(i > 0) {
+      const double u = (wlen - e[i - 1].wlen[0]) / (e[i].wlen[0] - e[i - 1].wlen[0]);
+      if(!(r < u)) --i;
+    }
+  }
+  *type = e[i].type;
+  *refl = e[i].reflectivity;
+}
+
+static double
+mtl_temperature(const struct htgx_ground_desc* g, int32_t m)
+{
+  return g->mtl_temperature ? g->mtl_temperature[m] : 0.0;
+}
+
+int
+oracle_brdf_fetch(const struct htgx_ground_desc* g, int32_t m, double wlen, double r,
+                  double* refl)
+{
+  int type;
+  brdf_at(g, m, wlen, r, &type, refl);
+  return type;
+}
+
+int32_t
+oracle_hit_material(const struct htgx_ground_desc* g, uint32_t ifc, const double N[3],
+                    const double dir[3])
+{
+  return hit_material(g, ifc, N, dir);
+}
+
+/* specular reflection of the incoming direction about N (ssf_specular_reflection:
+ * wi = 2 (wo . N) N - wo with wo = -dir) */
+static void
+reflect_dir(const double dir[3], const double N[3], double out[3])
+{
+  const double c = 2.0 * dot3(dir, N);
+  out[0] = dir[0] - c * N[0];
+  out[1] = dir[1] - c * N[1];
+  out[2] = dir[2] - c * N[2];
+}
+
+/*******************************************************************************
  * Sun, phase functions, BRDF
  ******************************************************************************/
 static int
@@ -1627,7 +1745,7 @@ radiance_sw
     struct ghit gh;
     double range[2], pos_next[3], dir_next[3], sun_dir[3], N[3] = {0,0,0};
     double Tr_abs, Tr, R, sun_pdf, refl = 1, dst, albedo = 0;
-    int surface, phase_kind = 0;
+    int surface, phase_kind = 0, specular = 0;
     const double r_inf[2] = {0, DBL_MAX};
 
     gh = ground_trace(s, pos, dir, r_inf, &hit_prev);     /* :355-356 */
@@ -1652,13 +1770,24 @@ radiance_sw
     if(Tr_abs <= 0) break;
 
     if(surface) {                                         /* :402-420 */
+      const struct htgx_ground_desc* gd = &s->scene.ground;
       const double wo[3] = {-dir[0], -dir[1], -dir[2]};
-      double u0, u1;
       N[0] = gh.N[0]; N[1] = gh.N[1]; N[2] = gh.N[2];
+      if(general_materials(gd)) {                         /* :405-407 */
+        int type;
+        const int32_t m = hit_material(gd, gh.mtl, gh.N, dir);
+        brdf_at(gd, m, wlen, canonical(rng), &type, &albedo);
+        specular = type == HTGX_BRDF_SPECULAR;
+      } else {
+        albedo = gd->mtl_albedo[gh.mtl];
+      }
       if(dot3(N, wo) < 0) { N[0] = -N[0]; N[1] = -N[1]; N[2] = -N[2]; }
-      albedo = s->scene.ground.mtl_albedo[gh.mtl];
-      u0 = canonical(rng); u1 = canonical(rng);
-      cos_hemisphere(N, u0, u1, dir_next);
+      if(specular) {
+        reflect_dir(dir, N, dir_next);
+      } else {
+        const double u0 = canonical(rng), u1 = canonical(rng);
+        cos_hemisphere(N, u0, u1, dir_next);
+      }
       refl = albedo;
     } else {                                              /* :422-443 */
       const double ks_gas = raw_in_seg(s, &sc.seg, HTGX_Ks, 1, iband, iquad, pos_next);
@@ -1671,11 +1800,17 @@ radiance_sw
       local_to_world(dir, mu, 2.0 * PI * u1, dir_next);
     }
 
-    sun_pdf = sun_sample(s, rng, sun_dir);                /* :457-466 */
-    if(surface) {
-      const double c = dot3(N, sun_dir);
+    if(specular) {                                        /* :446-455 */
+      memcpy(sun_dir, dir_next, sizeof(sun_dir));
+      R = !in_solar_cone(s, sun_dir) ? 0 : dot3(N, sun_dir) < 0 ? 0 : albedo;
+      sun_pdf = 1.0;
+    } else if(surface) {                                  /* :457-466 */
+      double c;
+      sun_pdf = sun_sample(s, rng, sun_dir);
+      c = dot3(N, sun_dir);
       R = c < 0 ? 0 : albedo / PI * c;
     } else {
+      sun_pdf = sun_sample(s, rng, sun_dir);
       const double mu = dot3(dir, sun_dir);
       R = phase_kind ? rayleigh_eval(mu) : hg_eval(g, mu);
     }
@@ -1754,16 +1889,29 @@ radiance_lw
       local_to_world(dir, hg_cos_sample(g, u0), 2.0 * PI * u1, dir_next);
       hit_prev = GHIT_NONE;                               /* :236 */
     } else {                                              /* :239-273 */
+      const struct htgx_ground_desc* gd = &s->scene.ground;
       const double wo[3] = {-dir[0], -dir[1], -dir[2]};
-      double N[3], u0, u1, refl;
+      double N[3], refl, T;
+      int specular = 0;
       N[0] = gh.N[0]; N[1] = gh.N[1]; N[2] = gh.N[2];
+      if(general_materials(gd)) {                         /* :250-252 */
+        int type;
+        const int32_t m = hit_material(gd, gh.mtl, gh.N, dir);
+        brdf_at(gd, m, wlen, canonical(rng), &type, &refl);
+        specular = type == HTGX_BRDF_SPECULAR;
+        T = mtl_temperature(gd, m);
+      } else {
+        refl = gd->mtl_albedo[gh.mtl];
+        T = mtl_temperature(gd, (int32_t)gh.mtl);
+      }
       if(dot3(N, wo) < 0) { N[0] = -N[0]; N[1] = -N[1]; N[2] = -N[2]; }
-      u0 = canonical(rng); u1 = canonical(rng);
-      cos_hemisphere(N, u0, u1, dir_next);
-      refl = s->scene.ground.mtl_albedo[gh.mtl];
-      if(canonical(rng) >= refl) {
-        const double T = s->scene.ground.mtl_temperature
-          ? s->scene.ground.mtl_temperature[gh.mtl] : 0.0;
+      if(specular) {
+        reflect_dir(dir, N, dir_next);
+      } else {
+        const double u0 = canonical(rng), u1 = canonical(rng);
+        cos_hemisphere(N, u0, u1, dir_next);
+      }
+      if(canonical(rng) >= refl) {                        /* :266-270 */
         w = T > 0 ? oracle_planck_monochromatic(wlen_m, T) : 0;
         break;
       }
@@ -1920,7 +2068,20 @@ realisation
     for(a = 0; a < 3; ++a)
       org[a] = sn->pos[a] + sn->axis_x[a] * (sx*2.0 - 1.0) + sn->axis_y[a] * (sy*2.0 - 1.0);
     h = ground_trace(s, org, up, r_inf, NULL);                /* :75-100 */
-    if(h.hit && dot3(up, h.N) >= 0) { *rejected = 1; return 0; }
+    if(h.hit) {
+      const struct htgx_ground_desc* gd = &s->scene.ground;
+      int under;
+      if(gd->interfaces) {
+        /* the side the up ray travels in: front if cos(up, -N) < 0 (:93);
+         * rejected unless its material is the sky's (:96) */
+        int32_t f[3];
+        interface_of(gd, h.mtl, f);
+        under = (-h.N[2] < 0 ? f[0] : f[1]) != gd->sky_mtl;
+      } else {
+        under = dot3(up, h.N) >= 0; /* the up ray meets the sky side from below */
+      }
+      if(under) { *rejected = 1; return 0; }
+    }
     {
       const double u0 = canonical(&rng), u1 = canonical(&rng);
       cos_hemisphere(sn->z, u0, u1, dir);                     /* :103-104 */

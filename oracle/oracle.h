@@ -145,6 +145,17 @@ int oracle_ground_trace
   (const struct oracle_sky* sky, const double ray[8], int64_t prev_prim,
    int prev_edge, double* t, int64_t* prim, int* edge);
 
+/* Ground materials of a scene's ground description: the BRDF entry of
+ * material m at wavelength wlen (nm) for the canonical r, returning its type
+ * (enum htgx_brdf_type) and reflectivity (mrumtl_fetch_brdf, called at
+ * src/core/htrdr_materials.c:430-433); the material reflecting a hit of
+ * interface ifc by a ray along dir, N = (v1 - v0) x (v2 - v0) being the hit
+ * triangle's normal (src/core/htrdr_interface.h:45-87) */
+int oracle_brdf_fetch
+  (const struct htgx_ground_desc* ground, int32_t m, double wlen, double r, double* refl);
+int32_t oracle_hit_material
+  (const struct htgx_ground_desc* ground, uint32_t ifc, const double N[3], const double dir[3]);
+
 /* CIE 1931 XYZ sampler of one channel (0 X, 1 Y, 2 Z), range within
  * [380, 780] nm in nbands bands; sampled with oracle_wlen_planck_sample (pdf
  * = 1 / integral of the channel, nm^-1) and freed with

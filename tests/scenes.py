@@ -178,3 +178,52 @@ def cells_along(lwc, lower, upper, org, direction, t0, t1):
         if inside:
             out.append((ta, tb, tuple(idx)))
     return out
+
+
+def with_materials(sc, mtls, interfaces=None, sky_mtl=-1):
+    """Give a Scene's ground (plane or mesh, already set) general materials.
+
+    mtls: one entry per material, (list_kind, [(lo, hi, type, reflectivity),
+    ...], temperature) with list_kind F.BRDF_LIST_WAVELENGTHS or
+    F.BRDF_LIST_BANDS, or (kind, [], T) for a semi-transparent material.
+    interfaces: [(front, back, thin)] (thin -1: none), or None (the ground's
+    material indices stay material indices).
+    """
+    g = sc.desc.ground
+    first = np.zeros(len(mtls) + 1, np.uint32)
+    kinds = np.zeros(len(mtls), np.int32)
+    temps = np.zeros(len(mtls), np.float64)
+    entries = []
+    for m, (kind, lst, T) in enumerate(mtls):
+        kinds[m] = kind
+        temps[m] = T
+        entries += lst
+        first[m + 1] = len(entries)
+    brdfs = (F.Brdf * max(1, len(entries)))()
+    for i, (lo, hi, typ, refl) in enumerate(entries):
+        brdfs[i].wlen[0], brdfs[i].wlen[1] = lo, hi
+        brdfs[i].type, brdfs[i].reflectivity = typ, refl
+    sc._keep += [first, kinds, temps, brdfs]
+    g.nmtls = len(mtls)
+    g.mtl_albedo = None
+    g.mtl_temperature = temps.ctypes.data_as(F.c_double_p)
+    g.mtl_brdf_first = first.ctypes.data_as(F.c_uint32_p)
+    g.mtl_brdf_list = kinds.ctypes.data_as(C.POINTER(C.c_int32))
+    g.brdfs = C.cast(brdfs, C.POINTER(F.Brdf))
+    if interfaces is not None:
+        ifc = np.ascontiguousarray(interfaces, np.int32).reshape(-1, 3)
+        sc._keep.append(ifc)
+        g.ninterfaces = len(ifc)
+        g.interfaces = ifc.ctypes.data_as(C.POINTER(C.c_int32))
+        g.sky_mtl = sky_mtl
+    return sc
+
+
+def lambertian(albedo, T=0.0):
+    """A grey Lambertian material for with_materials"""
+    return (F.BRDF_LIST_WAVELENGTHS, [(550.0, 550.0, F.BRDF_LAMBERTIAN, albedo)], T)
+
+
+def air():
+    """A semi-transparent material (no BRDF) for with_materials"""
+    return (F.BRDF_LIST_WAVELENGTHS, [], 0.0)

@@ -45,6 +45,9 @@ def test_help():
     (("-a", "synth:0", "-C", "focal-length=0.05"), "focal-length is not supported"),
     (("-a", "synth:0", "-C", "lens-radius=-1"), "invalid option argument"),
     (("-a", "synth:9", "-s", "sw=550,550"), "unknown synthetic configuration"),
+    (("-a", "synth:0", "-g", "ground.obj"), "option '-M'"),
+    (("-a", "synth:0", "-g", "/nonexistent/ground.obj", "-M", "/nonexistent/materials"),
+     "cannot open the materials file"),
 ])
 def test_bad_arguments(args, msg):
     r = run(*args)
@@ -131,3 +134,69 @@ def test_cli_matches_library(tmp_path, kind):
     A.dump_buffer(px, d, buf)
     _, _, lib = parse_image(buf.getvalue())
     assert np.array_equal(cli[..., :6], lib[..., :6])
+
+
+def write_ground(tmp_path):
+    """A 9x9-vertex terrain over the cloud grid, its south half `air:grass'
+    (a wavelength list), its north half `air:ice' (a band list mixing a
+    specular and a Lambertian entry around 550 nm)"""
+    (tmp_path / "grass.mrumtl").write_text("wavelengths 2\n500 lambertian 0.1\n600 lambertian 0.3\n")
+    (tmp_path / "ice.mrumtl").write_text("bands 2\n400 540 specular 0.7\n560 800 lambertian 0.8\n")
+    (tmp_path / "materials").write_text(
+        f"grass {tmp_path / 'grass.mrumtl'} 290\nice {tmp_path / 'ice.mrumtl'} 260\nair none\n")
+    rng = np.random.default_rng(0)
+    n = 9
+    lines = []
+    for j in range(n):
+        for i in range(n):
+            lines.append(f"v {6400.0 * i / (n - 1)} {6400.0 * j / (n - 1)} {rng.uniform(0, 300):.3f}")
+    for half, mtl in ((range(0, (n - 1) // 2), "air:grass"), (range((n - 1) // 2, n - 1), "air:ice")):
+        lines.append(f"usemtl {mtl}")
+        for j in half:
+            for i in range(n - 1):
+                a, b, c, d = j * n + i + 1, j * n + i + 2, (j + 1) * n + i + 2, (j + 1) * n + i + 1
+                # clockwise seen from above: the front side, the air, is up
+                lines += [f"f {a} {c} {b}", f"f {a} {d} {c}"]
+    (tmp_path / "ground.obj").write_text("\n".join(lines) + "\n")
+    return tmp_path / "ground.obj", tmp_path / "materials"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["image", "flux"])
+def test_cli_ground_files(tmp_path, kind):
+    """-g ground.obj -M materials -n air: the CLI's map equals the library's
+    on the same scene with the ground read by GroundFiles"""
+    obj, mats = write_ground(tmp_path)
+    out = tmp_path / "out.txt"
+    common = ["-a", "synth:0,3", "-c", "synth", "-g", str(obj), "-M", str(mats), "-n", "air",
+              "-D", "0,60", "-f", "-o", str(out), "-i", "def=24x16:spp=8", "-s", "sw=550,550"]
+    if kind == "image":
+        extra = ["-C", "pos=3200,-2000,2500:tgt=3200,-1999.5,2499.4:up=0,0,1:fov=60"]
+    else:
+        extra = ["-p", "pos=3200,3200,150:tgt=3200,3200,151:up=0,1,0:sz=6000,6000"]
+    r = run(*(common + extra))
+    assert r.returncode == 0, r.stderr
+    _, _, cli = parse_image(out.read_text())
+
+    s = A.Synth(0, seed=3, sun_azimuth=0.0, sun_elevation=60.0, ground_type=F.GROUND_NONE)
+    d = A.map_preset(0, width=24, height=16, spp=8, solver=F.SOLVER_WAVEFRONT_FP32)
+    if kind == "image":
+        d.pos[:] = (3200, -2000, 2500)
+        d.tgt[:] = (3200, -1999.5, 2499.4)
+        d.fov_y = 60.0
+    else:
+        d.sensor = F.SENSOR_RECTANGLE
+        d.pos[:] = (3200, 3200, 150)
+        d.tgt[:] = (3200, 3200, 151)
+        d.up[:] = (0, 1, 0)
+        d.rect_size[:] = (6000.0, 6000.0)
+    with A.GroundFiles(obj, mats, "air") as g:
+        scene = g.with_scene(s.scene)
+        with A.Context(C.pointer(scene)) as ctx:
+            px = ctx.draw_map(d)
+    buf = io.StringIO()
+    A.dump_buffer(px, d, buf)
+    _, _, lib = parse_image(buf.getvalue())
+    assert np.array_equal(cli[..., :6], lib[..., :6])
+    if kind == "flux":  # the rectangle cuts through the terrain
+        assert 0 < px["flux_n"].sum() < 24 * 16 * 8
