@@ -162,7 +162,10 @@ struct Lane {
   float pos[3], dir[3];
   /* query */
   int mode, prop;
-  bool nq, nq_ground;   /* a query is requested; its range comes from a ground trace */
+  bool nq;              /* a query is requested */
+  int nq_ground;        /* its range comes from a ground cast (1), or the query
+                           resumes after a late cast (2) */
+  bool lazy;            /* the range ends at the mesh's top, not yet cast */
   float qr;             /* end of the query range */
   float inv_dir[3];
   float tc, Ts, maj, inv_maj;
@@ -339,7 +342,8 @@ __device__ __forceinline__ GroundF ground_tracef(const SceneF& S, const float o[
       const float tt = (S.plane_z - o[2]) / d[2];
       if(tt > 0.f && tt < r1) { g.hit = true; g.t = tt; g.prim = 0; g.mtl = S.plane_mtl; }
     }
-  } else if(S.ground_type == HTGX_GROUND_MESH) {
+  } else if(S.ground_type == HTGX_GROUND_MESH && !(o[2] > S.ghi[2] && d[2] >= 0.f)) {
+    /* (a ray from above the mesh's top going up or level meets none of it) */
     const MeshHit m = mesh_trace_copies<float>(S.bvh, S.nbvh, S.tris, S.ground_repeat != 0, S.glo,
                                                S.ghi, o, d, 0.f, r1, prev_prim, prev_edge, any);
     g.nodes = m.nodes;
@@ -620,7 +624,7 @@ __device__ __forceinline__ void request(Lane& L, int mode, int prop, float r1)
   L.prop = prop;
   L.qr = r1;
   L.nq = true;
-  L.nq_ground = false;
+  L.nq_ground = 0;
 }
 
 __device__ __forceinline__ void request_ground(Lane& L, int mode, int prop)
@@ -628,7 +632,7 @@ __device__ __forceinline__ void request_ground(Lane& L, int mode, int prop)
   L.mode = mode;
   L.prop = prop;
   L.nq = true;
-  L.nq_ground = true;
+  L.nq_ground = 1;
 }
 
 __device__ __forceinline__ void sw_after_direct(Lane& L)
@@ -1009,17 +1013,24 @@ __device__ __forceinline__ uint64_t wave_sum64(uint64_t v)
   return v;
 }
 
-/* The persistent loop of one wavefront over the chunks of one map. Without
- * FULL, the features that few maps use (cloud and ground repetition,
- * spectral BRDFs and interfaces, orthographic and thin-lens cameras, CIE
- * XYZ wavelength sampling) are
- * compiled out: their flags become constants, so that their code and
- * registers do not weigh on the common kernel. */
-template<int SENSOR, int SPECTRAL, bool COUNT, bool FULL>
+/* Kernel features: without kFeatRare, the features that few maps use (cloud
+ * and ground repetition, spectral BRDFs and interfaces, orthographic and
+ * thin-lens cameras, CIE XYZ wavelength sampling) are compiled out; without
+ * kFeatMesh, the mesh ground (its BVH casts, its deferred casts). Their flags
+ * become constants, so that their code and registers do not weigh on the
+ * kernels of the maps that do without them. */
+enum { kFeatRare = 1, kFeatMesh = 2 };
+
+/* The persistent loop of one wavefront over the chunks of one map */
+template<int SENSOR, int SPECTRAL, bool COUNT, int FEAT>
 __device__ __forceinline__ void run_scene(const SceneF& S0, const DevMap& M0, const Launch& P)
 {
+  constexpr bool FULL = (FEAT & kFeatRare) != 0;
   SceneF S = S0;
   DevMap M = M0;
+  /* no mesh: the ground type is NONE or PLANE (0 or 1), so that every test
+   * for HTGX_GROUND_MESH (2) folds to false */
+  if(!(FEAT & kFeatMesh)) S.ground_type &= 1;
   if(!FULL) {
     S.repeat = 0;
     S.ground_repeat = 0;
@@ -1035,6 +1046,7 @@ __device__ __forceinline__ void run_scene(const SceneF& S0, const DevMap& M0, co
   LC(stage) = ST_FETCH;
   L.pending = false;
   L.nq = false;
+  L.lazy = false;
   L.c_leaf = L.c_seg = L.c_coll = L.c_lwc = L.c_ev = L.c_bvh = L.c_tri = 0;
   LC(n) = LC(nfail) = 0;
   LC(sum) = LC(sum2) = 0.0;
@@ -1077,6 +1089,13 @@ __device__ __forceinline__ void run_scene(const SceneF& S0, const DevMap& M0, co
           L.tc = t0 + L.Ts * L.inv_maj;
           L.lstate = LS_COLL;
         }
+      } else if(L.lazy) {
+        /* the query reached the top of the mesh ground without an event:
+         * cast the ray on the ground now, then go on up to the hit */
+        L.lazy = false;
+        L.nq = true;
+        L.nq_ground = 2;
+        L.lstate = LS_EVENT;
       } else {
         L.ev = EV_NONE;
         L.pending = true;
@@ -1214,22 +1233,40 @@ __device__ __forceinline__ void run_scene(const SceneF& S0, const DevMap& M0, co
         /* set-up of the requested query (single site) */
         if(L.nq) {
           L.nq = false;
+          const bool resume = L.nq_ground == 2;
+          L.lazy = false; /* a query that ended in the medium leaves no cast pending */
           if(L.nq_ground) {
             float wp[3];
             world_posf(S, L.pos, wp);
-            const GroundF gh = ground_tracef(S, wp, L.dir, INFINITY, LC(prev_prim), L.prev_edge,
-                                             false);
-            count_ground<COUNT>(L, gh);
-            L.gh_hit = gh.hit;
-            LC(gh_mtl) = gh.mtl;
-            LC(gh_prim) = gh.prim;
-            L.gh_edge = gh.edge;
-            L.qr = gh.t;
+            if(S.ground_type == HTGX_GROUND_MESH && !resume && wp[2] > S.ghi[2] && L.dir[2] < 0.f) {
+              /* from above the mesh, downwards: the tracking runs up to the
+               * mesh's top first, and most queries end in the medium before
+               * it; the cast waits until the tracking gets there */
+              L.lazy = true;
+              L.gh_hit = false;
+              L.qr = (S.ghi[2] - wp[2]) / L.dir[2];
+            } else {
+              const GroundF gh = ground_tracef(S, wp, L.dir, INFINITY, LC(prev_prim), L.prev_edge,
+                                               false);
+              count_ground<COUNT>(L, gh);
+              L.gh_hit = gh.hit;
+              LC(gh_mtl) = gh.mtl;
+              LC(gh_prim) = gh.prim;
+              L.gh_edge = gh.edge;
+              /* resuming: the tracking goes on from the mesh's top, the
+               * iterator's leaf and layer being those at that point */
+              if(resume) L.tit = L.qr;
+              L.qr = gh.t;
+            }
           }
-          L.Ts = exp1f(u[2 * NP - 1]);
-          L.Tmin = 0.0;
-          L.pending = false;
-          iter_init(S, L);
+          if(resume) {
+            L.itdone = false;
+          } else {
+            L.Ts = exp1f(u[2 * NP - 1]);
+            L.Tmin = 0.0;
+            L.pending = false;
+            iter_init(S, L);
+          }
           L.lstate = LS_SEG;
         }
         WF_TICK(8)
@@ -1265,11 +1302,11 @@ __device__ __forceinline__ void run_scene(const SceneF& S0, const DevMap& M0, co
 
 /* 3 waves per SIMD: the register cap (168) spills a little but hides more
  * latency than 2 waves without spills (measured 14% faster at 1024^2) */
-template<int SENSOR, int SPECTRAL, bool COUNT, bool FULL>
+template<int SENSOR, int SPECTRAL, bool COUNT, int FEAT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 wavefront_kernel(SceneF S, DevMap M, Launch P)
 {
-  run_scene<SENSOR, SPECTRAL, COUNT, FULL>(S, M, P);
+  run_scene<SENSOR, SPECTRAL, COUNT, FEAT>(S, M, P);
 }
 
 /* One map of a batched launch */
@@ -1285,7 +1322,7 @@ struct BatchScene {
  * queues with the per-scene loop, then moves on, so the tails of the scenes
  * overlap instead of idling the device between launches. The scene is
  * uniform over a wavefront, so its constants stay scalar. */
-template<int SENSOR, int SPECTRAL, bool FULL>
+template<int SENSOR, int SPECTRAL, int FEAT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 wavefront_batch_kernel(const BatchScene* __restrict__ scenes, uint32_t nscenes)
 {
@@ -1294,7 +1331,7 @@ wavefront_batch_kernel(const BatchScene* __restrict__ scenes, uint32_t nscenes)
     const unsigned int ex = __hip_atomic_load(B.P.exhausted, __ATOMIC_RELAXED,
                                               __HIP_MEMORY_SCOPE_AGENT);
     if(__builtin_amdgcn_readfirstlane(ex)) continue;
-    run_scene<SENSOR, SPECTRAL, false, FULL>(B.S, B.M, B.P);
+    run_scene<SENSOR, SPECTRAL, false, FEAT>(B.S, B.M, B.P);
   }
 }
 
@@ -1414,31 +1451,36 @@ int launch_variant(F kernel, const SceneF& S, const DevMap& M, const Launch& P, 
   return HTGX_OK;
 }
 
-/* Whether a map needs the FULL kernels (features compiled out of the
- * common ones) */
-bool needs_full(const SceneF& S, const DevMap& M)
+/* The kernel features a map needs (kFeatRare, kFeatMesh) */
+int features(const SceneF& S, const DevMap& M)
 {
-  return S.repeat || S.ground_repeat || S.gmat || M.ortho || M.lens_radius > 0 || M.wmode == 3;
+  const bool rare = S.repeat || S.ground_repeat || S.gmat || M.ortho || M.lens_radius > 0 ||
+                    M.wmode == 3;
+  return (rare ? kFeatRare : 0) | (S.ground_type == HTGX_GROUND_MESH ? kFeatMesh : 0);
 }
 
-template<bool COUNT, bool FULL>
+template<bool COUNT, int FEAT>
 int launch_full(const SceneF& S, const DevMap& M, const Launch& P, uint32_t grid, hipStream_t stream)
 {
   if(M.sensor == HTGX_SENSOR_CAMERA) {
     if(M.spectral_type == HTGX_SPECTRAL_LW)
-      return launch_variant(wavefront_kernel<HTGX_SENSOR_CAMERA, HTGX_SPECTRAL_LW, COUNT, FULL>, S, M, P, grid, stream);
-    return launch_variant(wavefront_kernel<HTGX_SENSOR_CAMERA, HTGX_SPECTRAL_SW, COUNT, FULL>, S, M, P, grid, stream);
+      return launch_variant(wavefront_kernel<HTGX_SENSOR_CAMERA, HTGX_SPECTRAL_LW, COUNT, FEAT>, S, M, P, grid, stream);
+    return launch_variant(wavefront_kernel<HTGX_SENSOR_CAMERA, HTGX_SPECTRAL_SW, COUNT, FEAT>, S, M, P, grid, stream);
   }
   if(M.spectral_type == HTGX_SPECTRAL_LW)
-    return launch_variant(wavefront_kernel<HTGX_SENSOR_RECTANGLE, HTGX_SPECTRAL_LW, COUNT, FULL>, S, M, P, grid, stream);
-  return launch_variant(wavefront_kernel<HTGX_SENSOR_RECTANGLE, HTGX_SPECTRAL_SW, COUNT, FULL>, S, M, P, grid, stream);
+    return launch_variant(wavefront_kernel<HTGX_SENSOR_RECTANGLE, HTGX_SPECTRAL_LW, COUNT, FEAT>, S, M, P, grid, stream);
+  return launch_variant(wavefront_kernel<HTGX_SENSOR_RECTANGLE, HTGX_SPECTRAL_SW, COUNT, FEAT>, S, M, P, grid, stream);
 }
 
 template<bool COUNT>
 int launch(const SceneF& S, const DevMap& M, const Launch& P, uint32_t grid, hipStream_t stream)
 {
-  return needs_full(S, M) ? launch_full<COUNT, true>(S, M, P, grid, stream)
-                          : launch_full<COUNT, false>(S, M, P, grid, stream);
+  switch(features(S, M)) {
+    case 0: return launch_full<COUNT, 0>(S, M, P, grid, stream);
+    case kFeatRare: return launch_full<COUNT, kFeatRare>(S, M, P, grid, stream);
+    case kFeatMesh: return launch_full<COUNT, kFeatMesh>(S, M, P, grid, stream);
+    default: return launch_full<COUNT, kFeatRare | kFeatMesh>(S, M, P, grid, stream);
+  }
 }
 
 } /* namespace */
@@ -1616,7 +1658,7 @@ wavefront_build(htgx_ctx* ctx, htgx_wavefront_state* st)
   HIP_OK(hipGetDevice(&dev));
   HIP_OK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
   HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu,
-    wavefront_kernel<HTGX_SENSOR_CAMERA, HTGX_SPECTRAL_SW, false, false>, 256, 0));
+    wavefront_kernel<HTGX_SENSOR_CAMERA, HTGX_SPECTRAL_SW, false, 0>, 256, 0));
   st->grid = st->grid_max = (uint32_t)std::max(1, ncu * std::max(1, per_cu));
   if(const char* e = std::getenv("HTGX_WF_GRID")) st->grid = (uint32_t)std::max(1, atoi(e));
   if(const char* e = std::getenv("HTGX_WF_EVTHRESH")) st->ev_thresh = (uint32_t)std::max(1, atoi(e));
@@ -1709,7 +1751,7 @@ struct htgx_batch_plan {
   int device = 0;
   uint32_t n = 0;
   int sensor = 0, spectral = 0;
-  bool full = false;  /* some map needs the FULL kernels */
+  int feat = 0;       /* kernel features the maps need, together */
   uint32_t grid = 0;
   uint32_t maxpix = 0;
   std::vector<uint32_t> order;      /* table slot -> item */
@@ -1745,20 +1787,20 @@ static int plan_alloc(htgx_batch_plan* p, size_t n, T** out)
   return HTGX_OK;
 }
 
-template<bool FULL>
+template<int FEAT>
 static void launch_batch(const htgx_batch_plan* p, hipStream_t stream)
 {
   const uint32_t n = p->n;
   if(p->sensor == HTGX_SENSOR_CAMERA) {
     if(p->spectral == HTGX_SPECTRAL_LW)
-      wavefront_batch_kernel<HTGX_SENSOR_CAMERA, HTGX_SPECTRAL_LW, FULL><<<p->grid, 256, 0, stream>>>(p->table, n);
+      wavefront_batch_kernel<HTGX_SENSOR_CAMERA, HTGX_SPECTRAL_LW, FEAT><<<p->grid, 256, 0, stream>>>(p->table, n);
     else
-      wavefront_batch_kernel<HTGX_SENSOR_CAMERA, HTGX_SPECTRAL_SW, FULL><<<p->grid, 256, 0, stream>>>(p->table, n);
+      wavefront_batch_kernel<HTGX_SENSOR_CAMERA, HTGX_SPECTRAL_SW, FEAT><<<p->grid, 256, 0, stream>>>(p->table, n);
   } else {
     if(p->spectral == HTGX_SPECTRAL_LW)
-      wavefront_batch_kernel<HTGX_SENSOR_RECTANGLE, HTGX_SPECTRAL_LW, FULL><<<p->grid, 256, 0, stream>>>(p->table, n);
+      wavefront_batch_kernel<HTGX_SENSOR_RECTANGLE, HTGX_SPECTRAL_LW, FEAT><<<p->grid, 256, 0, stream>>>(p->table, n);
     else
-      wavefront_batch_kernel<HTGX_SENSOR_RECTANGLE, HTGX_SPECTRAL_SW, FULL><<<p->grid, 256, 0, stream>>>(p->table, n);
+      wavefront_batch_kernel<HTGX_SENSOR_RECTANGLE, HTGX_SPECTRAL_SW, FEAT><<<p->grid, 256, 0, stream>>>(p->table, n);
   }
 }
 
@@ -1772,8 +1814,12 @@ static int plan_launch_step(htgx_batch_plan* p, hipStream_t stream)
   zero_u32_kernel<<<(2 * n + 255) / 256, 256, 0, stream>>>(p->flags, 2 * n);
   batch_tick_kernel<<<1, 1, 0, stream>>>(p->ticks + n, p->flags + 2 * n);
   HIP_OK(hipGetLastError());
-  if(p->full) launch_batch<true>(p, stream);
-  else launch_batch<false>(p, stream);
+  switch(p->feat) {
+    case 0: launch_batch<0>(p, stream); break;
+    case kFeatRare: launch_batch<kFeatRare>(p, stream); break;
+    case kFeatMesh: launch_batch<kFeatMesh>(p, stream); break;
+    default: launch_batch<kFeatRare | kFeatMesh>(p, stream); break;
+  }
   HIP_OK(hipGetLastError());
   batch_merge_kernel<<<dim3((p->maxpix + 255) / 256, n), 256, 0, stream>>>(p->table);
   batch_advance_kernel<<<(n + 255) / 256, 256, 0, stream>>>(p->table, n, p->flags + 2 * n);
@@ -1840,7 +1886,7 @@ int wf_plan_build(htgx_ctx* const* ctxs, const DevMap* Ms, htgx_mc_accum* const*
       }
     }
     p->maxpix = std::max(p->maxpix, Ms[i].width * Ms[i].height);
-    p->full = p->full || needs_full(e.S, e.M);
+    p->feat |= features(e.S, e.M);
     p->us_per_tick[k] = ctxs[i]->us_per_tick;
   }
   HIP_OK(hipMemcpy(p->table, p->table_h.data(), n * sizeof(BatchScene), hipMemcpyHostToDevice));

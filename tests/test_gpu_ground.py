@@ -173,3 +173,31 @@ def test_device_bvh_walk_matches_host_and_oracle():
     assert np.array_equal(t2, t2_o) and (q2 != q2_o).mean() < 1e-3
     sky.close()
     ctx.close()
+
+
+@pytest.mark.parametrize("spectral", [F.SPECTRAL_SW, F.SPECTRAL_LW])
+def test_deferred_mesh_casts(spectral):
+    """The FP32 solver casts a downward query from above the mesh only once
+    its tracking reaches the mesh's top, then resumes the query there with
+    the same optical-depth state. Scattering and absorbing gas layers
+    straddle the top of a 300 m terrain, so that many queries resume inside
+    the medium: FP32 vs FP64 (which casts first) per pixel."""
+    _, v, t, _ = terrain(res=17, relief=300.0, seed=6)
+    if spectral == F.SPECTRAL_SW:
+        sc = Scene(nlayers=10, dz=100.0, ka=np.full((1, 10), 3e-4), ks=np.full((1, 10), 2e-3),
+                   sun=(30.0, 50.0))
+        wlen, Tref = (550.0, 550.0), 0.0
+    else:
+        sc = Scene(bands=((9500.0, 10500.0),), nlayers=10, dz=100.0, ka=np.full((1, 10), 1e-3),
+                   ks=np.full((1, 10), 1e-3), T=np.linspace(288.0, 282.0, 10))
+        wlen, Tref = (10000.0, 10000.0), 285.0
+    sc = with_mesh(sc, v, t, np.zeros(len(t)), [0.4], [290.0])
+    out = {}
+    with A.Context(sc.ptr) as ctx:
+        for solver, seed in ((F.SOLVER_PLAIN_FP64, 0), (F.SOLVER_WAVEFRONT_FP32, 1)):
+            d = camera(24, 24, 64, pos=(3200.0, 3200.0, 900.0), tgt=(3200.3, 3200.2, 0.0), fov=70.0,
+                       wlen=wlen, spectral=spectral, Tref=Tref, solver=solver, seed=seed)
+            out[solver] = ctx.draw_map(d)
+    a, b = out[F.SOLVER_WAVEFRONT_FP32], out[F.SOLVER_PLAIN_FP64]
+    w, zm, zs = zstats(a["radiance_E"], a["radiance_SE"], b["radiance_E"], b["radiance_SE"])
+    assert w >= 0.99 and abs(zm) < 0.1 and zs < 1.2
