@@ -193,6 +193,7 @@ SIGNATURES = {
     "htgx_ground_files_interface_name": (C.c_char_p, [C.c_void_p, C.c_uint32]),
     "htgx_ground_files_destroy": (C.c_int, [C.c_void_p]),
     "htgx_files_last_error": (C.c_char_p, []),
+    "htgx_dump_octrees_vtk": (C.c_int, [C.POINTER(SceneDesc), C.c_int]),
 }
 
 _lib = None

@@ -48,6 +48,19 @@ def accum_planes(desc):
 assert ACCUM_DTYPE.itemsize == C.sizeof(F.McAccum) == 64
 
 
+def dump_octrees_vtk(scene_ptr, out):
+    """Option -d: the octrees of every band and k-point as legacy VTK, each
+    followed by a line `---' (htgx_dump_octrees_vtk), into the open file
+    ``out`` (anything with fileno()) or the file at path ``out``."""
+    lib = F.load()
+    if hasattr(out, "fileno"):
+        out.flush()
+        F.check("htgx_dump_octrees_vtk", lib.htgx_dump_octrees_vtk(scene_ptr, out.fileno()))
+        return
+    with open(out, "w") as f:
+        F.check("htgx_dump_octrees_vtk", lib.htgx_dump_octrees_vtk(scene_ptr, f.fileno()))
+
+
 class GroundFiles:
     """The ground of option -g (an htrdr-obj file) with the materials of
     option -M (an htrdr-materials file and its mrumtl files) and the sky

@@ -18,8 +18,9 @@
  * and -g also takes
  *   plane | synth                an infinite plane (the quad repeated with -R)
  *                                or the configuration's fBm heightfield.
+ * -d writes the octrees (htgx_dump_octrees_vtk) instead of rendering.
  * Options without a counterpart on the GPU path are rejected with an error
- * (-d, -C focal-length) or accepted and ignored (-t, -V, -O).
+ * (-C focal-length) or accepted and ignored (-t, -V, -O).
  * The solver is the FP32 wavefront one; HTGX_SOLVER=plain selects the FP64
  * plain one.
  */
@@ -45,7 +46,7 @@ struct args {
   const char* mtl;
   const char* sky_mtl;
   const char* output;
-  int force, verbose, repeat_ground, repeat_clouds;
+  int force, verbose, repeat_ground, repeat_clouds, dump;
   int output_type; /* 0 image (-C), 1 flux map (-p) */
   double pos[3], tgt[3], up[3], fov; /* camera */
   int ortho;                         /* -P: orthographic camera */
@@ -233,9 +234,7 @@ parse_args(int argc, char** argv, struct args* a)
         bad = parse_doubles(optarg, d, 2) || d[0] < 0 || d[0] >= 360 || d[1] < 0 || d[1] > 90;
         a->sun_az = d[0]; a->sun_el = d[1];
       } break;
-      case 'd':
-        fprintf(stderr, "%s: octree dumps (-d) are not built\n", argv[0]);
-        return HTGX_BAD_OP;
+      case 'd': a->dump = 1; break;
       case 'f': a->force = 1; break;
       case 'g': a->ground = optarg; break;
       case 'h': usage(); return -1;
@@ -459,6 +458,13 @@ main(int argc, char** argv)
   if(ground) scene.ground = *htgx_ground_files_get_desc(ground);
   scene.repeat_clouds = a.repeat_clouds;
   scene.repeat_ground = a.repeat_ground;
+  if(a.dump) { /* dump_volumetric_acceleration_structure, ..._atmosphere.c:228-255 */
+    fflush(out);
+    res = htgx_dump_octrees_vtk(&scene, fileno(out));
+    if(res == HTGX_BAD_OP) fprintf(stderr, "%s: no clouds, no octree to write (-d)\n", argv[0]);
+    else if(res != HTGX_OK) fprintf(stderr, "%s: could not write the octrees (%d)\n", argv[0], res);
+    goto error;
+  }
   if((res = htgx_ctx_create(&scene, &ctx)) != HTGX_OK) {
     fprintf(stderr, "%s: could not set up the scene on the GPU (%d)\n", argv[0], res);
     goto error;

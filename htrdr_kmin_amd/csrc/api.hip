@@ -15,7 +15,10 @@
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <string>
 #include <vector>
+
+#include <unistd.h>
 
 using namespace htgx;
 
@@ -367,6 +370,114 @@ check_scene(const htgx_scene_desc* s)
     return HTGX_BAD_ARG;
   }
   if(!(s->optical_thickness >= 0)) return HTGX_BAD_ARG;
+  return HTGX_OK;
+}
+
+/* Option -d: the octree's leaves per band and k-point as legacy VTK */
+extern "C" int
+htgx_dump_octrees_vtk(const htgx_scene_desc* scene, int fd)
+{
+  int res;
+  if(fd < 0) return HTGX_BAD_ARG;
+  if((res = check_scene(scene)) != HTGX_OK) return res;
+  const htgx_scene_desc& sc = *scene;
+  const htgx_cloud_desc& cl = sc.clouds;
+  if(!(cl.def[0] && cl.def[1] && cl.def[2])) return HTGX_BAD_OP;
+  SvoHost svo;
+  if((res = svo_build(sc, svo)) != HTGX_OK) return res;
+
+  /* the leaves: (level, node) from the first block of each */
+  struct Leaf { uint32_t lo[3], hi[3]; float2 b; };
+  std::vector<Leaf> leaves;
+  const uint32_t* d0 = svo.dims[0];
+  for(uint32_t z = 0; z < d0[2]; ++z)
+    for(uint32_t y = 0; y < d0[1]; ++y)
+      for(uint32_t x = 0; x < d0[0]; ++x) {
+        const uint32_t w = svo.lmap[((size_t)z * d0[1] + y) * d0[0] + x];
+        const uint32_t L = w >> kLeafIdBits, m = (1u << L) - 1u;
+        if((x & m) || (y & m) || (z & m)) continue;
+        Leaf f;
+        const uint32_t c[3] = {x, y, z};
+        for(int a = 0; a < 3; ++a) { f.lo[a] = c[a]; f.hi[a] = std::min(c[a] + (1u << L), d0[a]); }
+        f.b = svo.bounds[w & kLeafIdMask];
+        leaves.push_back(f);
+      }
+
+  const htgx_spectral_desc& sp = sc.spectral;
+  const htgx_gas_desc& gas = sc.gas;
+  std::string out;
+  auto flush = [&]() {
+    size_t off = 0;
+    while(off < out.size()) {
+      const ssize_t n = ::write(fd, out.data() + off, out.size() - off);
+      if(n <= 0) return false;
+      off += (size_t)n;
+    }
+    out.clear();
+    return true;
+  };
+  auto put = [&](const char* fmt, auto... v) {
+    char buf[256];
+    const int n = snprintf(buf, sizeof(buf), fmt, v...);
+    out.append(buf, (size_t)n);
+  };
+  uint32_t q0 = 0;
+  for(uint32_t b = 0; b < sp.nbands; ++b) {
+    for(uint32_t q = 0; q < sp.nquads[b]; ++q) {
+      const uint32_t iq = q0 + q;
+      out.append("# vtk DataFile Version 2.0\n");
+      put("octree of band %u (%g-%g nm), k-point %u: kext bounds of its leaves\n", b,
+          sp.band_lo[b], sp.band_hi[b], q);
+      put("ASCII\nDATASET UNSTRUCTURED_GRID\nPOINTS %zu double\n", leaves.size() * 8);
+      for(const Leaf& f : leaves) {
+        double p[2][3];
+        for(int a = 0; a < 3; ++a) {
+          p[0][a] = cl.lower[a] + f.lo[a] * svo.bs[a];
+          p[1][a] = cl.lower[a] + f.hi[a] * svo.bs[a];
+        }
+        for(int k = 0; k < 8; ++k) /* VTK_VOXEL corner order: x fastest */
+          put("%.9g %.9g %.9g\n", p[k & 1][0], p[(k >> 1) & 1][1], p[(k >> 2) & 1][2]);
+        if(out.size() > (1u << 22) && !flush()) return HTGX_IO_ERR;
+      }
+      put("CELLS %zu %zu\n", leaves.size(), leaves.size() * 9);
+      for(size_t i = 0; i < leaves.size(); ++i) {
+        const size_t o = 8 * i;
+        put("8 %zu %zu %zu %zu %zu %zu %zu %zu\n", o, o + 1, o + 2, o + 3, o + 4, o + 5, o + 6, o + 7);
+        if(out.size() > (1u << 22) && !flush()) return HTGX_IO_ERR;
+      }
+      put("CELL_TYPES %zu\n", leaves.size());
+      for(size_t i = 0; i < leaves.size(); ++i) {
+        out.append("11\n");
+        if(out.size() > (1u << 22) && !flush()) return HTGX_IO_ERR;
+      }
+      put("CELL_DATA %zu\n", leaves.size());
+      for(int which = 0; which < 2; ++which) {
+        put("SCALARS %s double 1\nLOOKUP_TABLE default\n", which ? "kext_max" : "kext_min");
+        for(const Leaf& f : leaves) {
+          /* gas extinction bounds over the layers the leaf overlaps (0 where
+           * it reaches outside the layers) */
+          double gmin = INFINITY, gmax = -INFINITY;
+          const double z0 = cl.lower[2] + f.lo[2] * svo.bs[2], z1 = cl.lower[2] + f.hi[2] * svo.bs[2];
+          if(!gas.nlayers || z0 < gas.z_bottom || z1 > gas.z_bottom + gas.nlayers * gas.layer_thickness) {
+            gmin = 0; gmax = std::max(gmax, 0.0);
+          }
+          for(uint32_t j = 0; j < gas.nlayers; ++j) {
+            const double lz0 = gas.z_bottom + j * gas.layer_thickness, lz1 = lz0 + gas.layer_thickness;
+            if(lz1 <= z0 || lz0 >= z1) continue;
+            const double k = gas.ka[(size_t)iq * gas.nlayers + j] + gas.ks[(size_t)b * gas.nlayers + j];
+            gmin = std::min(gmin, k);
+            gmax = std::max(gmax, k);
+          }
+          const double kc = cl.cext[b] * (which ? (double)f.b.y : (double)f.b.x);
+          put("%.9g\n", kc + (which ? gmax : gmin));
+          if(out.size() > (1u << 22) && !flush()) return HTGX_IO_ERR;
+        }
+      }
+      out.append("---\n");
+      if(!flush()) return HTGX_IO_ERR;
+    }
+    q0 += sp.nquads[b];
+  }
   return HTGX_OK;
 }
 

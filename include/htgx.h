@@ -325,6 +325,21 @@ int htgx_ctx_get_svo_info
    uint64_t* nnodes,
    uint64_t* nleaves);
 
+/* Option -d of htrdr-atmosphere (dump_volumetric_acceleration_structure,
+ * src/atmosphere/htrdr_atmosphere.c:228-255, calling htsky_dump_cloud_vtk per
+ * band and k-point): writes to the file descriptor fd, for every band and
+ * k-point in order, the octree over the clouds as a legacy VTK file, each
+ * followed by a line `---'. A tree is the unstructured grid of its leaves
+ * (VTK_VOXEL cells, corners in metres) with the cell scalars kext_min and
+ * kext_max (m^-1): the cloud extinction cext * LWC bounds of the leaf plus
+ * the bounds of the gas extinction over the layers the leaf overlaps. This
+ * build keeps one tree of LWC bounds for all bands; the trees written are
+ * its per-band images. htsky 0.3 is not in the reference, so this is the
+ * content the option documents (doc/htrdr-atmosphere.1.in:222-230), not
+ * htsky's byte layout. Host only (no device involved); HTGX_BAD_OP for a
+ * scene without clouds, HTGX_IO_ERR if a write fails. */
+int htgx_dump_octrees_vtk(const struct htgx_scene_desc* scene, int fd);
+
 /* Replaces htrdr_draw_map (src/core/htrdr_draw_map.c:497-621) as called by
  * atmosphere_draw_map (src/atmosphere/htrdr_atmosphere_draw_map.c:593-616):
  * renders desc->spp realisations per pixel and writes one pixel struct per

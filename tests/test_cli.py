@@ -40,7 +40,7 @@ def test_help():
     (("-a", "synth:0", "-C", "fov=180"), "invalid option argument"),
     (("-a", "synth:0", "-s", "sw=600,500"), "invalid option argument"),
     (("-a", "synth:0", "-T", "-1"), "invalid option argument"),
-    (("-a", "synth:0", "-d"), "not built"),
+    (("-a", "synth:0", "-d"), "no clouds"),
     (("-a", "synth:0", "-P", "height=0"), "invalid option argument"),
     (("-a", "synth:0", "-C", "focal-length=0.05"), "focal-length is not supported"),
     (("-a", "synth:0", "-C", "lens-radius=-1"), "invalid option argument"),
