@@ -39,6 +39,9 @@ sys.path.insert(0, ROOT)
 METRIC = "Mrealisations/s & time-to-0.5%-std-error, FP32 batched vs FP64 plain"
 SOLVER_NAMES = {0: "FP64 plain", 1: "FP32 wavefront"}
 HBM_PEAK_GBS = 8000.0
+# (solver, config) -> rocprofv3 summary of the kernel on that workload, for
+# its 16-realisation launches
+PROFILED = {("wavefront", 1): "traffic_wavefront_sw_image_1024.json"}
 
 
 def parse():
@@ -377,11 +380,15 @@ def main():
                   + 36.0 * counters["ntri"]) / creal + 64.0 / S
     avg_launch_s = total_s / args.steps
     achieved = bytes_real * npix * S / avg_launch_s / 1e9
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_{args.solver}_cfg{args.config}.json")
-    if os.path.exists(tfile):
+    # rocprofv3 counters of the same launches, summarised under profiles/
+    # (named for the kernel and workload they cover)
+    traffic, counters_prof = None, {}
+    tname = PROFILED.get((args.solver, args.config))
+    tfile = os.path.join(ROOT, "profiles", tname) if tname else None
+    if tfile and os.path.exists(tfile) and S == 16 and desc.width == 1024:
         with open(tfile) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            counters_prof = json.load(f)
+        traffic = counters_prof.get("bytes_per_launch")
 
     out = {
         "metric": METRIC,
@@ -414,6 +421,11 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 6),
             "traffic": traffic,
             "bytes_per_realisation": round(bytes_real, 2),
+            # the kernel's real bound (rocprofv3 counters, same launches)
+            "valu_busy": counters_prof.get("valu_busy"),
+            "valu_lane_utilisation": counters_prof.get("valu_lane_utilisation"),
+            "l2_hit_rate": counters_prof.get("l2_hit_rate"),
+            "counters_source": counters_prof.get("source"),
             "events_per_realisation": {
                 "leaf_lookups": round(counters["nleaf"] / creal, 2),
                 "segments": round(counters["nseg"] / creal, 2),
