@@ -249,7 +249,8 @@ htgx_ground_files_load(const char* obj_path, const char* materials_path, const c
     if(t.empty()) continue;
     if(t[0] == "v") {
       double x[3];
-      if(t.size() < 4 || t.size() > 5 || !to_double(t[1], &x[0]) || !to_double(t[2], &x[1]) ||
+      /* x y z, then an optional w or vertex colour, ignored */
+      if(t.size() < 4 || !to_double(t[1], &x[0]) || !to_double(t[2], &x[1]) ||
          !to_double(t[3], &x[2]))
         return done(fail(HTGX_BAD_ARG, "%s:%zu: invalid vertex", obj_path, lnum));
       for(int a = 0; a < 3; ++a) f->verts.push_back((float)x[a]);

@@ -163,3 +163,37 @@ def test_flux_lw_thick_isothermal(solver):
                              spectral=F.SPECTRAL_LW, Tref=T, solver=solver))
     E, SE = A.estimate(p["flux_sum"], p["flux_sum2"], p["flux_n"])
     assert np.allclose(E, math.pi * planck(LW_WLEN * 1e-9, T) / 1e9, rtol=DET_RTOL[solver], atol=0)
+
+
+@pytest.mark.parametrize("solver", SOLVERS)
+@pytest.mark.parametrize("config", [0, 2])
+def test_looser_majorants_leave_the_estimate_unchanged(solver, config):
+    """Null-collision estimators are unbiased for any majorant: the octree
+    built with threshold 0 (per-block bounds) and with threshold 1e9 (one
+    leaf: the whole grid's bounds) give the same images (full SW walks with
+    delta tracking on Ks, LW walks on Kext), while the loose tree draws many
+    more tentative collisions."""
+    out = []
+    for T in (0.0, 1e9):
+        s = A.Synth(config, seed=2, optical_thickness=T, cloud_def=(32, 32, 16),
+                    cloud_upper=(3200.0, 3200.0, 2600.0))
+        if config == 0:
+            d = A.map_preset(0, width=16, height=16, spp=256, solver=solver, pos=(1600.0, -1000.0, 3000.0),
+                             tgt=(1600.0, -999.5, 2999.134))
+        else:
+            d = A.map_preset(2, width=16, height=16, spp=128, solver=solver, pos=(1600.0, 1600.0, 2000.0),
+                             tgt=(1600.0, 1600.0, 2001.0), rect_size=(3200.0, 3200.0))
+        with A.Context(s.scene_ptr) as ctx:
+            ctx.set_counting(True)
+            p = ctx.draw_map(d)
+            out.append((p, ctx.counters()["ncoll"], ctx.svo_info()[2]))
+        s.close()
+    (a, ca, la), (b, cb, lb) = out
+    assert lb == 1 and la > 100 and cb > 1.5 * ca
+    if config == 0:
+        E1, S1, E2, S2 = a["radiance_E"], a["radiance_SE"], b["radiance_E"], b["radiance_SE"]
+    else:
+        (E1, S1), (E2, S2) = A.estimate(a["flux_sum"], a["flux_sum2"], a["flux_n"]), A.estimate(
+            b["flux_sum"], b["flux_sum2"], b["flux_n"])
+    z = (E1 - E2) / np.sqrt(S1 ** 2 + S2 ** 2)
+    assert np.mean(np.abs(z) <= 3) >= 0.98 and abs(z.mean()) < 0.2

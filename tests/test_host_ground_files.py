@@ -143,7 +143,7 @@ def test_obj_variants(files):
     tmp, write, mats = files
     obj = write("var.obj", (
         "mtllib x.mtl\no ground\n"
-        "v 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0 1.0\n"
+        "v 0 0 0\nv 1 0 0 0.5 0.5 0.5\nv 1 1 0\nv 0 1 0 1.0\n"
         "vt 0 0\nvn 0 0 1\n"
         "f 1 2 3\n"                       # before any usemtl: not ground
         "usemtl air:brick\ns off\ng a\n"
