@@ -155,7 +155,9 @@ def bench_batch(args):
     from htrdr_kmin_amd import atmosphere as A
     from htrdr_kmin_amd import hip as H
     items = A.uneven_batch(seed=args.seed, n=64, spp_scale=args.batch_scale)
+    t_setup = time.perf_counter()
     ctxs = [A.Context(s.scene_ptr) for s, _ in items]
+    setup = {"ctx_create_s": round(time.perf_counter() - t_setup, 4), "scenes": len(ctxs)}
     bufs = [H.DeviceBuffer(d.width * d.height * A.ACCUM_DTYPE.itemsize) for _, d in items]
     stream = H.Stream()
     total = sum(d.width * d.height * d.spp for _, d in items)
@@ -255,6 +257,7 @@ def bench_batch(args):
         "back_to_back_launches_s": None if seq_s is None else round(seq_s, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None},
+        "setup": setup,
         "cpu_baseline": None,
     }
     if not args.no_cpu:
@@ -292,7 +295,13 @@ def main():
     desc.seed = args.seed
     npix = desc.width * desc.height
     S = args.spp_per_step
+    # scene set-up, outside the timed region: host octree (and BVH) build and
+    # the uploads to HBM
+    t_setup = time.perf_counter()
     ctx = A.Context(synth.scene_ptr)
+    setup = {"ctx_create_s": round(time.perf_counter() - t_setup, 4)}
+    nl, _, nf = ctx.svo_info()
+    setup.update({"octree_levels": nl, "octree_leaves": nf})
     acc = H.DeviceBuffer(npix * A.ACCUM_DTYPE.itemsize)
     stream = H.Stream()
 
@@ -434,6 +443,7 @@ def main():
                 "scatter_surface_events": round(counters["nevents"] / creal, 3),
             },
         },
+        "setup": setup,
         "cpu_baseline": None,
     }
     if args.fp64_steps > 0 and main_solver == F.SOLVER_WAVEFRONT_FP32:
