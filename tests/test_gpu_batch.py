@@ -125,3 +125,38 @@ def test_plan_bad_arguments(batch):
     assert lib.htgx_batch_plan_create(arr, 0, 1, C.byref(h)) == F.BAD_ARG
     assert lib.htgx_batch_plan_step(None, None) == F.BAD_ARG
     assert lib.htgx_batch_plan_destroy(None) == F.BAD_ARG
+
+
+def test_mixed_plane_and_mesh_batch():
+    """A batch of a plane-ground map, a mesh-ground map and a map with a rare
+    feature (an orthographic camera) runs in the kernel that carries all of
+    them; each map still equals its own launch (in its own, leaner kernel)
+    bit for bit."""
+    plane = A.Synth(0, seed=3)
+    mesh = A.Synth(0, seed=4, ground_type=F.GROUND_MESH, hf_res=33, hf_octaves=5, hf_relief=300.0,
+                   hf_hurst=0.7)
+    ortho = A.Synth(0, seed=5)
+    descs = [A.map_preset(0, width=24, height=16, spp=8, solver=F.SOLVER_WAVEFRONT_FP32),
+             A.map_preset(0, width=16, height=16, spp=8, solver=F.SOLVER_WAVEFRONT_FP32,
+                          pos=(3200.0, -1500.0, 1800.0), tgt=(3200.0, -1499.0, 1799.4)),
+             A.map_preset(0, width=16, height=8, spp=8, solver=F.SOLVER_WAVEFRONT_FP32,
+                          projection=F.PROJECTION_ORTHOGRAPHIC, ortho_height=3000.0)]
+    synths = [plane, mesh, ortho]
+    ctxs = [A.Context(s.scene_ptr) for s in synths]
+    bufs = [H.DeviceBuffer(d.width * d.height * A.ACCUM_DTYPE.itemsize) for d in descs]
+    for b in bufs:
+        b.zero()
+    A.render_batch([(c, d, b.address) for c, d, b in zip(ctxs, descs, bufs)])
+    got = [b.download(np.zeros(d.width * d.height, A.ACCUM_DTYPE)) for d, b in zip(descs, bufs)]
+    for c, d, b, g in zip(ctxs, descs, bufs, got):
+        b.zero()
+        c.render_accum(d, b.address)
+        H.synchronize()
+        ref = b.download(np.zeros(d.width * d.height, A.ACCUM_DTYPE))
+        assert np.all(ref["n"] == d.spp)
+        for k in EXACT:
+            assert np.array_equal(g[k], ref[k]), k
+    for c in ctxs:
+        c.close()
+    for s in synths:
+        s.close()
