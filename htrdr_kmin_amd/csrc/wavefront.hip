@@ -853,7 +853,7 @@ __device__ __forceinline__ void consume(const SceneF& S, Lane& L, const float* u
 
 /* Sensor ray, wavelength, band and k-point, then the first walk
  * (..._draw_map.c:348-400 and :233-302) */
-template<int SENSOR, int SPECTRAL, bool COUNT, bool FULL>
+template<int SENSOR, int SPECTRAL, bool COUNT, bool RARE>
 __device__ __forceinline__ void real_begin(const SceneF& S, const DevMap& M, Lane& L, const float* u)
 {
   const uint32_t px = L.pixel % M.width, py = L.pixel / M.width;
@@ -925,7 +925,7 @@ __device__ __forceinline__ void real_begin(const SceneF& S, const DevMap& M, Lan
   }
 
   double pdf;
-  const double wlen = sample_wavelength<FULL>(M, (double)u[4], (double)u[5], &pdf);
+  const double wlen = sample_wavelength<RARE>(M, (double)u[4], (double)u[5], &pdf);
   LC(pdf) = (float)(pdf * 1e9);
   LC(wlen) = (float)wlen;
   uint32_t band;
@@ -1025,13 +1025,13 @@ enum { kFeatRare = 1, kFeatMesh = 2 };
 template<int SENSOR, int SPECTRAL, bool COUNT, int FEAT>
 __device__ __forceinline__ void run_scene(const SceneF& S0, const DevMap& M0, const Launch& P)
 {
-  constexpr bool FULL = (FEAT & kFeatRare) != 0;
+  constexpr bool RARE = (FEAT & kFeatRare) != 0;
   SceneF S = S0;
   DevMap M = M0;
   /* no mesh: the ground type is NONE or PLANE (0 or 1), so that every test
    * for HTGX_GROUND_MESH (2) folds to false */
   if(!(FEAT & kFeatMesh)) S.ground_type &= 1;
-  if(!FULL) {
+  if(!RARE) {
     S.repeat = 0;
     S.ground_repeat = 0;
     S.gmat = 0;
@@ -1195,7 +1195,7 @@ __device__ __forceinline__ void run_scene(const SceneF& S0, const DevMap& M0, co
         for(int i = 0; i < NP; ++i) { const float2 v = draw2(L, key); u[2 * i] = v.x; u[2 * i + 1] = v.y; }
         WF_TICK(3)
         if(begin) {
-          real_begin<SENSOR, SPECTRAL, COUNT, FULL>(S, M, L, u);
+          real_begin<SENSOR, SPECTRAL, COUNT, RARE>(S, M, L, u);
         }
         WF_TICK(4)
         if(!begin && L.pending) {
