@@ -194,6 +194,8 @@ SIGNATURES = {
     "htgx_ground_files_destroy": (C.c_int, [C.c_void_p]),
     "htgx_files_last_error": (C.c_char_p, []),
     "htgx_dump_octrees_vtk": (C.c_int, [C.POINTER(SceneDesc), C.c_int]),
+    "htgx_debug_octree": (C.c_int, [C.POINTER(SceneDesc), C.c_int, c_uint32_p, c_float_p, c_uint64_p,
+                                    c_uint64_p, c_uint64_p, c_double_p]),
 }
 
 _lib = None

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -271,6 +272,16 @@ copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n)
 }
 
 template<class T>
+int device_alloc(htgx_ctx* ctx, size_t n, T** out)
+{
+  void* d = nullptr;
+  HIP_OK(hipMalloc(&d, std::max<size_t>(n, 1) * sizeof(T)));
+  ctx->bufs.push_back(d);
+  *out = (T*)d;
+  return HTGX_OK;
+}
+
+template<class T>
 int upload(htgx_ctx* ctx, const T* host, size_t n, const T** out)
 {
   void* d = nullptr;
@@ -371,6 +382,64 @@ check_scene(const htgx_scene_desc* s)
   }
   if(!(s->optical_thickness >= 0)) return HTGX_BAD_ARG;
   return HTGX_OK;
+}
+
+extern "C" int
+htgx_debug_octree(const htgx_scene_desc* scene, int on_device, uint32_t* lmap, float* bounds,
+                  uint64_t* nblocks, uint64_t* nnodes, uint64_t* nleaves, double* build_ms)
+{
+  int res;
+  if(!nblocks || !nnodes || !nleaves || !build_ms) return HTGX_BAD_ARG;
+  if((res = check_scene(scene)) != HTGX_OK) return res;
+  const htgx_cloud_desc& cl = scene->clouds;
+  if(!(cl.def[0] && cl.def[1] && cl.def[2])) return HTGX_BAD_OP;
+  SvoHost svo;
+  if((res = svo_layout(*scene, svo)) != HTGX_OK) return res;
+  const uint64_t nb = (uint64_t)svo.dims[0][0] * svo.dims[0][1] * svo.dims[0][2];
+  *nblocks = nb;
+  *nnodes = svo.nnodes;
+  *nleaves = 0;
+  *build_ms = 0;
+  if(!lmap || !bounds) return HTGX_OK;
+  if(!on_device) {
+    const auto t0 = std::chrono::steady_clock::now();
+    if((res = svo_build(*scene, svo)) != HTGX_OK) return res;
+    *build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    std::memcpy(lmap, svo.lmap.data(), nb * sizeof(uint32_t));
+    std::memcpy(bounds, svo.bounds.data(), svo.nnodes * sizeof(float2));
+    *nleaves = svo.nleaves;
+    return HTGX_OK;
+  }
+  const size_t ncells = (size_t)cl.def[0] * cl.def[1] * cl.def[2];
+  float* d_lwc = nullptr;
+  float2* d_bounds = nullptr;
+  uint32_t* d_lmap = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  res = HTGX_UNKNOWN_ERR;
+  if(hipMalloc(&d_lwc, ncells * sizeof(float)) == hipSuccess &&
+     hipMalloc(&d_bounds, svo.nnodes * sizeof(float2)) == hipSuccess &&
+     hipMalloc(&d_lmap, nb * sizeof(uint32_t)) == hipSuccess &&
+     hipMemcpy(d_lwc, cl.lwc, ncells * sizeof(float), hipMemcpyHostToDevice) == hipSuccess &&
+     hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess &&
+     hipEventRecord(e0, nullptr) == hipSuccess) {
+    uint64_t n = 0;
+    res = svo_build_device(svo, d_lwc, cl.def, scene->optical_thickness, d_bounds, d_lmap, &n);
+    float ms = 0.f;
+    if(res == HTGX_OK &&
+       (hipEventRecord(e1, nullptr) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess ||
+        hipMemcpy(lmap, d_lmap, nb * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(bounds, d_bounds, svo.nnodes * sizeof(float2), hipMemcpyDeviceToHost) != hipSuccess))
+      res = HTGX_UNKNOWN_ERR;
+    *nleaves = n;
+    *build_ms = ms;
+  }
+  if(e0) (void)hipEventDestroy(e0);
+  if(e1) (void)hipEventDestroy(e1);
+  if(d_lwc) (void)hipFree(d_lwc);
+  if(d_bounds) (void)hipFree(d_bounds);
+  if(d_lmap) (void)hipFree(d_lmap);
+  return res;
 }
 
 /* Option -d: the octree's leaves per band and k-point as legacy VTK */
@@ -547,8 +616,10 @@ htgx_ctx_init(htgx_ctx* ctx, const htgx_scene_desc& sc)
   /* Clouds + octree */
   const htgx_cloud_desc& cl = sc.clouds;
   if(cl.def[0] && cl.def[1] && cl.def[2]) {
+    /* the octree is built on the device from the uploaded LWC grid
+     * (svo_device.hip); only its shape is computed here */
     SvoHost svo;
-    if((res = svo_build(sc, svo)) != HTGX_OK) return res;
+    if((res = svo_layout(sc, svo)) != HTGX_OK) return res;
     S.has_clouds = 1;
     S.B = svo.B;
     for(int a = 0; a < 3; ++a) {
@@ -563,15 +634,25 @@ htgx_ctx_init(htgx_ctx* ctx, const htgx_scene_desc& sc)
     for(int a = 0; a < 2; ++a) S.period[a] = (double)svo.dims[0][a] * svo.bs[a];
     const size_t ncells = (size_t)cl.def[0] * cl.def[1] * cl.def[2];
     if((res = upload(ctx, cl.lwc, ncells, &S.lwc))) return res;
-    if((res = upload(ctx, svo.lmap.data(), svo.lmap.size(), &S.lmap))) return res;
-    if((res = upload(ctx, svo.bounds.data(), svo.bounds.size(), &S.bounds))) return res;
+    const uint64_t nblocks = (uint64_t)svo.dims[0][0] * svo.dims[0][1] * svo.dims[0][2];
+    {
+      float2* bounds = nullptr;
+      uint32_t* lmap = nullptr;
+      if((res = device_alloc(ctx, svo.nnodes, &bounds))) return res;
+      if((res = device_alloc(ctx, nblocks, &lmap))) return res;
+      if((res = svo_build_device(svo, S.lwc, cl.def, sc.optical_thickness, bounds, lmap,
+                                 &svo.nleaves)) != HTGX_OK)
+        return res;
+      S.bounds = bounds;
+      S.lmap = lmap;
+    }
     if((res = upload(ctx, cl.cext, nb, &S.cext))) return res;
     if((res = upload(ctx, cl.ssa, nb, &S.ssa))) return res;
     if((res = upload(ctx, cl.hg_g, nb, &S.hg_g))) return res;
     ctx->nlevels = svo.nlevels;
     ctx->nnodes = svo.nnodes;
     ctx->nleaves = svo.nleaves;
-    ctx->svo_bytes = svo.lmap.size() * sizeof(uint32_t) + svo.bounds.size() * sizeof(float2);
+    ctx->svo_bytes = nblocks * sizeof(uint32_t) + svo.nnodes * sizeof(float2);
   }
 
   /* Ground */

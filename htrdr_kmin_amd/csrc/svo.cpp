@@ -7,7 +7,7 @@
 
 namespace htgx {
 
-int svo_build(const htgx_scene_desc& scene, SvoHost& svo)
+int svo_layout(const htgx_scene_desc& scene, SvoHost& svo)
 {
   const htgx_cloud_desc& cl = scene.clouds;
   const uint32_t B = scene.leaf_block ? scene.leaf_block : 1;
@@ -40,7 +40,15 @@ int svo_build(const htgx_scene_desc& scene, SvoHost& svo)
   svo.cext_max = 0;
   for(uint32_t b = 0; b < scene.spectral.nbands; ++b)
     svo.cext_max = std::max(svo.cext_max, cl.cext[b]);
+  return HTGX_OK;
+}
 
+int svo_build(const htgx_scene_desc& scene, SvoHost& svo)
+{
+  int res = svo_layout(scene, svo);
+  if(res != HTGX_OK) return res;
+  const htgx_cloud_desc& cl = scene.clouds;
+  const uint32_t B = svo.B;
   svo.bounds.resize(svo.nnodes);
   svo.mergeable.resize(svo.nnodes);
 

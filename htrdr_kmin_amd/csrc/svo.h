@@ -49,6 +49,18 @@ struct SvoHost {
  * block or if the tree has more than 2^28 nodes. */
 int svo_build(const htgx_scene_desc& scene, SvoHost& svo);
 
+/* The tree's shape only (block and cell sizes, level dimensions and offsets,
+ * cext_max): what svo_build computes before filling the arrays, and all a
+ * device build needs besides the LWC grid */
+int svo_layout(const htgx_scene_desc& scene, SvoHost& svo);
+
+/* The same tree built on the current device from the LWC grid already in
+ * device memory (lwc, def[2] x def[1] x def[0] floats), into bounds
+ * [svo.nnodes] and lmap [blocks], with the same min/max and merge rule, bit
+ * for bit (svo holds the layout from svo_layout; its nleaves is set). */
+int svo_build_device(const SvoHost& layout, const float* lwc, const uint32_t def[3],
+                     double optical_thickness, float2* bounds, uint32_t* lmap, uint64_t* nleaves);
+
 } /* namespace htgx */
 
 #endif /* HTGX_SVO_H */

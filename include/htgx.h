@@ -550,6 +550,18 @@ int htgx_debug_realisations
    uint32_t n,
    double* weights);
 
+/* The k_min/k_max octree of a scene's clouds built on the host (svo_build)
+ * or on the current device (the build htgx_ctx_create uses), for comparing
+ * the two: *nnodes node bounds (LWC min, max as 2 floats, level by level)
+ * into bounds and *nblocks level-map words (level << 28 | leaf id, one per
+ * finest block) into lmap; *nleaves leaves; *build_ms the build's time
+ * (device: kernels only, the LWC grid already uploaded). With lmap or bounds
+ * NULL only the sizes are returned. HTGX_BAD_OP for a scene without clouds.
+ * HOST arrays. */
+int htgx_debug_octree
+  (const struct htgx_scene_desc* scene, int on_device, uint32_t* lmap, float* bounds,
+   uint64_t* nblocks, uint64_t* nnodes, uint64_t* nleaves, double* build_ms);
+
 /* Threefry-4x64-20 blocks computed on the device: out[i][0..3] =
  * threefry(ctr[i][0..3], key[i][0..3]). HOST arrays. */
 int htgx_debug_threefry
