@@ -501,6 +501,8 @@ struct onode {
   struct onode* child[8];
 };
 
+struct obvh;
+
 struct oracle_sky {
   struct htgx_scene_desc scene; /* shallow copy (arrays owned by caller) */
   uint32_t* quad_offset;       /* [nbands] first quadrature point of a band */
@@ -518,6 +520,11 @@ struct oracle_sky {
   double glo[3], ghi[3];       /* bounding box of the mesh ground (-R period) */
   struct onode* root;
   uint64_t nnodes_alloc, nleaves;
+  /* optional BVH over the mesh ground, for timing runs only (the casts it
+   * gives are those of the brute force, bit for bit) */
+  struct obvh* bvh;
+  uint32_t nbvh;
+  uint32_t* bvh_tris;
   /* gas */
   double inv_dz_layer;
   /* sun */
@@ -711,6 +718,8 @@ void
 oracle_sky_destroy(struct oracle_sky* s)
 {
   if(!s) return;
+  free(s->bvh);
+  free(s->bvh_tris);
   free_node(s->root);
   free(s->quad_offset);
   free(s->quad_cdf);
@@ -1443,6 +1452,191 @@ ground_trace(const struct oracle_sky* s, const double o[3], const double d[3],
   return h;
 }
 
+/* The best hit so far of a mesh cast: the smallest distance, the lowest
+ * triangle index among equal distances (the brute force's order) */
+struct cand {
+  float best, bu, bv;
+  int64_t bt;
+};
+
+/* Triangle t as a candidate of the cast (of, df) in FP32, with the range
+ * and the self-hit rules of src/core/htrdr_geometry.c:116-150 */
+static void
+tri_candidate(const struct htgx_ground_desc* g, uint32_t t, const float of[3], const float df[3],
+              float r0, float r1, int64_t pprim, int pedge, struct cand* c)
+{
+  const float* p0 = g->verts + 3*(size_t)g->tris[3*(size_t)t + 0];
+  const float* p1 = g->verts + 3*(size_t)g->tris[3*(size_t)t + 1];
+  const float* p2 = g->verts + 3*(size_t)g->tris[3*(size_t)t + 2];
+  float e1[3], e2[3], tt, u, v;
+  int a;
+  for(a = 0; a < 3; ++a) { e1[a] = p1[a] - p0[a]; e2[a] = p2[a] - p0[a]; }
+  if(!tri_hit_f(p0, e1, e2, of, df, &tt, &u, &v)) return;
+  if(tt > c->best || (tt == c->best && (c->bt < 0 || (int64_t)t > c->bt))) return;
+  if(tt <= r0 || tt >= r1) return;
+  if((int64_t)t == pprim) return;
+  if(pprim >= 0 && fabsf(tt) <= 1e-1f && pedge && on_edge_f(u, v)) return;
+  c->best = tt; c->bu = u; c->bv = v; c->bt = (int64_t)t;
+}
+
+/* A bounding volume hierarchy over the mesh, median split on the longest
+ * axis, leaves of up to 4 triangles, boxes in FP64 padded so that no FP32
+ * hit can fall outside them. Visiting its leaves instead of every triangle
+ * finds the same candidate (tri_candidate keeps the smallest distance, then
+ * the lowest index), so the casts equal the brute force's. */
+struct obvh {
+  double lo[3], hi[3];
+  uint32_t a, b;  /* inner: the two children; leaf: first triangle slot, count */
+  int leaf;
+};
+
+static void
+tri_box(const struct htgx_ground_desc* g, uint32_t t, double lo[3], double hi[3])
+{
+  int k, a;
+  for(a = 0; a < 3; ++a) { lo[a] = INFINITY; hi[a] = -INFINITY; }
+  for(k = 0; k < 3; ++k) {
+    const float* p = g->verts + 3*(size_t)g->tris[3*(size_t)t + k];
+    for(a = 0; a < 3; ++a) {
+      if(p[a] < lo[a]) lo[a] = p[a];
+      if(p[a] > hi[a]) hi[a] = p[a];
+    }
+  }
+}
+
+/* centroid keys of the triangles being sorted (qsort has no context) */
+static const double* g_sort_key;
+
+static int
+cmp_key(const void* pa, const void* pb)
+{
+  const uint32_t ta = *(const uint32_t*)pa, tb = *(const uint32_t*)pb;
+  const double ca = g_sort_key[ta], cb = g_sort_key[tb];
+  return ca < cb ? -1 : ca > cb ? 1 : (ta < tb ? -1 : ta > tb);
+}
+
+static uint32_t
+bvh_node(struct oracle_sky* s, double* key, uint32_t first, uint32_t n)
+{
+  const struct htgx_ground_desc* g = &s->scene.ground;
+  const uint32_t id = s->nbvh++;
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  double ext = -1;
+  uint32_t i;
+  int a, axis = 0;
+  for(i = first; i < first + n; ++i) {
+    double tl[3], th[3];
+    tri_box(g, s->bvh_tris[i], tl, th);
+    for(a = 0; a < 3; ++a) {
+      if(tl[a] < lo[a]) lo[a] = tl[a];
+      if(th[a] > hi[a]) hi[a] = th[a];
+    }
+  }
+  for(a = 0; a < 3; ++a) {
+    const double pad = 1e-4 * (fabs(lo[a]) + fabs(hi[a])) + 1e-3;
+    s->bvh[id].lo[a] = lo[a] - pad;
+    s->bvh[id].hi[a] = hi[a] + pad;
+    if(hi[a] - lo[a] > ext) { ext = hi[a] - lo[a]; axis = a; }
+  }
+  if(n <= 4) {
+    s->bvh[id].leaf = 1;
+    s->bvh[id].a = first;
+    s->bvh[id].b = n;
+    return id;
+  }
+  for(i = first; i < first + n; ++i) {
+    double tl[3], th[3];
+    tri_box(g, s->bvh_tris[i], tl, th);
+    key[s->bvh_tris[i]] = tl[axis] + th[axis];
+  }
+  g_sort_key = key;
+  qsort(s->bvh_tris + first, n, sizeof(uint32_t), cmp_key);
+  s->bvh[id].leaf = 0;
+  s->bvh[id].a = bvh_node(s, key, first, n / 2);
+  s->bvh[id].b = bvh_node(s, key, first + n / 2, n - n / 2);
+  return id;
+}
+
+int
+oracle_sky_use_bvh(struct oracle_sky* s, int on)
+{
+  const struct htgx_ground_desc* g;
+  double* key;
+  uint32_t t;
+  if(!s) return HTGX_BAD_ARG;
+  free(s->bvh);
+  free(s->bvh_tris);
+  s->bvh = NULL;
+  s->bvh_tris = NULL;
+  s->nbvh = 0;
+  g = &s->scene.ground;
+  if(!on || g->type != HTGX_GROUND_MESH || !g->ntris) return HTGX_OK;
+  s->bvh = malloc(2 * (size_t)g->ntris * sizeof(struct obvh));
+  s->bvh_tris = malloc((size_t)g->ntris * sizeof(uint32_t));
+  key = malloc((size_t)g->ntris * sizeof(double));
+  if(!s->bvh || !s->bvh_tris || !key) {
+    free(key);
+    oracle_sky_use_bvh(s, 0);
+    return HTGX_MEM_ERR;
+  }
+  for(t = 0; t < g->ntris; ++t) s->bvh_tris[t] = t;
+  bvh_node(s, key, 0, g->ntris);
+  free(key);
+  return HTGX_OK;
+}
+
+/* Entry distance of the ray into a box within [0, r1], or INFINITY */
+static double
+box_entry(const struct obvh* nd, const double o[3], const double inv[3], double r1)
+{
+  double t0 = 0, t1 = r1;
+  int a;
+  for(a = 0; a < 3; ++a) {
+    double ta = (nd->lo[a] - o[a]) * inv[a], tb = (nd->hi[a] - o[a]) * inv[a];
+    if(ta > tb) { const double x = ta; ta = tb; tb = x; }
+    if(ta != ta) ta = -INFINITY; /* 0 * inf: the ray runs in the slab's boundary */
+    if(tb != tb) tb = INFINITY;
+    if(ta > t0) t0 = ta;
+    if(tb < t1) t1 = tb;
+  }
+  return t0 <= t1 ? t0 : INFINITY;
+}
+
+static void
+bvh_trace(const struct oracle_sky* s, const double o[3], const double d[3], const float of[3],
+          const float df[3], float r0, float r1, int64_t pprim, int pedge, struct cand* c)
+{
+  uint32_t stack[256];
+  int top = 0, a;
+  double inv[3];
+  for(a = 0; a < 3; ++a) inv[a] = 1.0 / d[a];
+  stack[top++] = 0;
+  while(top > 0) {
+    const struct obvh* nd = s->bvh + stack[--top];
+    /* boxes beyond the best distance so far (with a margin for the FP32
+     * distance) hold no better candidate */
+    const double lim = c->best == INFINITY ? INFINITY : (double)c->best * (1 + 1e-5) + 1e-2;
+    if(box_entry(nd, o, inv, lim) == INFINITY) continue;
+    if(nd->leaf) {
+      uint32_t i;
+      for(i = nd->a; i < nd->a + nd->b; ++i)
+        tri_candidate(&s->scene.ground, s->bvh_tris[i], of, df, r0, r1, pprim, pedge, c);
+    } else if(top + 2 > 256) {
+      /* deeper than any median split of 2^32 triangles: cast by brute force */
+      uint32_t t;
+      for(t = 0; t < s->scene.ground.ntris; ++t)
+        tri_candidate(&s->scene.ground, t, of, df, r0, r1, pprim, pedge, c);
+      return;
+    } else {
+      /* the nearer child on top of the stack */
+      const double ea = box_entry(s->bvh + nd->a, o, inv, lim);
+      const double eb = box_entry(s->bvh + nd->b, o, inv, lim);
+      if(ea <= eb) { stack[top++] = nd->b; stack[top++] = nd->a; }
+      else { stack[top++] = nd->a; stack[top++] = nd->b; }
+    }
+  }
+}
+
 /* Closest mesh hit by brute force over every triangle, the ray in FP32 */
 static struct ghit
 mesh_trace_once(const struct oracle_sky* s, const double o[3], const double d[3],
@@ -1455,24 +1649,16 @@ mesh_trace_once(const struct oracle_sky* s, const double o[3], const double d[3]
     const float df[3] = {(float)d[0], (float)d[1], (float)d[2]};
     const float r0 = (float)range[0];
     const float r1 = range[1] > (double)FLT_MAX ? INFINITY : (float)range[1];
-    float best = r1, bu = 0, bv = 0;
-    int64_t bt = -1;
+    struct cand c;
     uint32_t t;
-    for(t = 0; t < g->ntris; ++t) {
-      const float* p0 = g->verts + 3*(size_t)g->tris[3*(size_t)t + 0];
-      const float* p1 = g->verts + 3*(size_t)g->tris[3*(size_t)t + 1];
-      const float* p2 = g->verts + 3*(size_t)g->tris[3*(size_t)t + 2];
-      float e1[3], e2[3], tt, u, v;
-      int a;
-      for(a = 0; a < 3; ++a) { e1[a] = p1[a] - p0[a]; e2[a] = p2[a] - p0[a]; }
-      if(!tri_hit_f(p0, e1, e2, of, df, &tt, &u, &v)) continue;
-      if(tt >= best) continue;
-      /* self hits, src/core/htrdr_geometry.c:116-150 */
-      if(tt <= r0 || tt >= r1) continue;
-      if((int64_t)t == pprim) continue;
-      if(pprim >= 0 && fabsf(tt) <= 1e-1f && pedge && on_edge_f(u, v)) continue;
-      best = tt; bu = u; bv = v; bt = (int64_t)t;
+    c.best = r1; c.bu = 0; c.bv = 0; c.bt = -1;
+    if(s->bvh) {
+      bvh_trace(s, o, d, of, df, r0, r1, pprim, pedge, &c);
+    } else {
+      for(t = 0; t < g->ntris; ++t) tri_candidate(g, t, of, df, r0, r1, pprim, pedge, &c);
     }
+    const float best = c.best, bu = c.bu, bv = c.bv;
+    const int64_t bt = c.bt;
     if(bt >= 0) {
       const float* p0 = g->verts + 3*(size_t)g->tris[3*(size_t)bt + 0];
       const float* p1 = g->verts + 3*(size_t)g->tris[3*(size_t)bt + 1];

@@ -28,9 +28,23 @@
  *   SW / LW walks .......... closed forms: vacuum + Lambertian ground, purely
  *                            absorbing slab, isothermal enclosure, optically
  *                            thin single scattering (test_oracle_walks.py)
- * No function of this oracle is "parity unpinned" except the bits of the
- * reference's own random streams, which counter-based Threefry replaces by
- * design (per-pixel statistical parity only).
+ *   mesh ground casts ...... flat mesh = plane, heightfield rejection area,
+ *                            no leaks (test_oracle_ground.py); the optional
+ *                            BVH = the brute force bit for bit
+ *   CIE 1931 sampler ....... CIE table, equal-energy integrals, exact in-band
+ *                            inversion, KS, vacuum closed form (test_oracle_cie.py)
+ *   cameras ................ ground-footprint closed forms (test_oracle_cameras.py)
+ *   cloud / ground copies .. explicit tiling, infinite-layer and tiled-plane
+ *                            closed forms (test_oracle_repeat.py)
+ *   ground materials ....... list selection, mean = linear interpolation,
+ *                            interface sides, mirror and two-sided closed forms
+ *                            (test_oracle_materials.py)
+ * "Parity unpinned": the bits of the reference's own random streams, which
+ * counter-based Threefry replaces by design (per-pixel statistical parity
+ * only); the thin-lens sample mapping of star-camera (not vendored; the
+ * expectation per pixel is pinned, not the mapping of the lens samples); and
+ * which of two mrumtl BRDF entries around a wavelength a canonical selects
+ * (mrumtl not vendored; the mean reflectivity is pinned, not the choice).
  */
 #ifndef HTGX_ORACLE_H
 #define HTGX_ORACLE_H
@@ -144,6 +158,11 @@ double oracle_phase_eval(int kind, double param, double cos_theta);
 int oracle_ground_trace
   (const struct oracle_sky* sky, const double ray[8], int64_t prev_prim,
    int prev_edge, double* t, int64_t* prim, int* edge);
+
+/* Cast mesh-ground rays through a BVH of the oracle's own (on) or against
+ * every triangle (off, the default). Both give the same casts bit for bit;
+ * the BVH only makes timing runs on large terrains affordable. */
+int oracle_sky_use_bvh(struct oracle_sky* sky, int on);
 
 /* Ground materials of a scene's ground description: the BRDF entry of
  * material m at wavelength wlen (nm) for the canonical r, returning its type

@@ -51,6 +51,7 @@ _SIG = {
     "oracle_cie_fit": (C.c_double, [C.c_int, C.c_double]),
     "oracle_ground_trace": (C.c_int, [C.c_void_p, F.c_double_p, C.c_int64, C.c_int, F.c_double_p,
                                       C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
+    "oracle_sky_use_bvh": (C.c_int, [C.c_void_p, C.c_int]),
     "oracle_brdf_fetch": (C.c_int, [C.POINTER(F.GroundDesc), C.c_int32, C.c_double, C.c_double,
                                     F.c_double_p]),
     "oracle_hit_material": (C.c_int32, [C.POINTER(F.GroundDesc), C.c_uint32, F.c_double_p,

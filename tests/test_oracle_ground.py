@@ -143,3 +143,43 @@ def test_mesh_scene_validation():
     assert lib.htgx_ctx_create(sc.ptr, C.byref(h)) == F.BAD_ARG
     sc = with_mesh(Scene(), v, t, [0, 3], [0.3], [0.0])
     assert lib.htgx_ctx_create(sc.ptr, C.byref(h)) == F.BAD_ARG
+
+
+def test_oracle_bvh_casts_equal_brute_force():
+    """The oracle's optional BVH (for timing runs) gives the brute force's
+    casts bit for bit: hit, distance, triangle and edge flag, on a terrain,
+    for fresh rays and for rays leaving a previous hit, tied distances
+    included (shared edges and vertices)"""
+    import ctypes as C
+    s, v, t, _ = terrain(res=33, relief=500.0, seed=8)
+    sky = O.Sky(s.scene_ptr)
+    rng = np.random.default_rng(3)
+    rays, prevs = [], []
+    for i in range(3000):
+        o = np.array([rng.uniform(-500, 6900), rng.uniform(-500, 6900), rng.uniform(-100, 1500)])
+        if i % 3 == 0:  # aim at vertices: ties between triangles sharing them
+            p = v[rng.integers(len(v))].astype(np.float64)
+            d = p - o
+        else:
+            d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        rays.append(np.concatenate([o, d, [0.0, 1e30]]))
+        prevs.append((int(rng.integers(len(t))) if i % 4 == 1 else -1, i % 8 == 1))
+
+    def casts():
+        out = []
+        for ray, (pp, pe) in zip(rays, prevs):
+            tt, prim, edge = C.c_double(), C.c_int64(), C.c_int()
+            hit = O.lib().oracle_ground_trace(sky.h, O.dptr(ray), pp, int(pe), C.byref(tt), C.byref(prim),
+                                              C.byref(edge))
+            out.append((hit, tt.value, prim.value, edge.value))
+        return out
+
+    brute = casts()
+    assert O.lib().oracle_sky_use_bvh(sky.h, 1) == 0
+    fast = casts()
+    assert O.lib().oracle_sky_use_bvh(sky.h, 0) == 0
+    assert sum(h for h, _, _, _ in brute) > 1000
+    assert fast == brute
+    sky.close()
+    s.close()
