@@ -240,3 +240,35 @@ def test_ground_files_parity(tmp_path):
     s.close()
     w, zm, zs = zstats(p32["radiance_E"], p32["radiance_SE"], p64["radiance_E"], p64["radiance_SE"])
     assert w >= 0.99 and abs(zm) < 0.1 and zs < 1.2
+
+
+def test_general_materials_cie_image():
+    """The hills' spectral BRDF lists seen in a CIE XYZ image: wavelengths
+    sampled over 380-780 nm per channel select different entries (bands and
+    interpolated wavelengths); FP64 device vs oracle realisations, FP32 vs
+    FP64 per channel"""
+    s = A.Synth(0, seed=7, ground_type=F.GROUND_MESH, hf_res=33, hf_octaves=5, hf_relief=400.0,
+                hf_hurst=0.7, nbands=4, wlen_lo=380.0, wlen_hi=780.0)
+    sc = with_materials(Copy(s), MTLS, IFCS, sky_mtl=4)
+    d = hills_camera(spectral_type=F.SPECTRAL_SW_CIE_XYZ, wlen_range=(380.0, 780.0))
+    with A.Context(sc.ptr) as ctx:
+        sky = O.Sky(sc.ptr)
+        rng = np.random.default_rng(5)
+        n = 600
+        pix = rng.integers(0, d.width * d.height, size=n).astype(np.uint64)
+        real = (rng.integers(0, 3, size=n).astype(np.uint64) << np.uint64(32)) | rng.integers(
+            0, 1 << 20, size=n).astype(np.uint64)
+        g = ctx.realisations(d, pix, real)
+        o = np.array([sky.realisation(d, int(p), int(r)) for p, r in zip(pix, real)])
+        sky.close()
+        same = np.isclose(g, o, rtol=1e-9, atol=0) | (np.isnan(g) & np.isnan(o))
+        assert same.mean() >= 0.97
+        p64 = ctx.draw_map(hills_camera(spp=32, spectral_type=F.SPECTRAL_SW_CIE_XYZ,
+                                        wlen_range=(380.0, 780.0)))
+        p32 = ctx.draw_map(hills_camera(spp=32, spectral_type=F.SPECTRAL_SW_CIE_XYZ,
+                                        wlen_range=(380.0, 780.0), solver=F.SOLVER_WAVEFRONT_FP32,
+                                        seed=1))
+    s.close()
+    for c in "XYZ":
+        w, zm, zs = zstats(p32[c + "_E"], p32[c + "_SE"], p64[c + "_E"], p64[c + "_SE"])
+        assert w >= 0.99 and abs(zm) < 0.1 and zs < 1.2, c
