@@ -275,11 +275,11 @@ def uneven_batch(seed=0, n=64, spp_scale=1.0):
     """The uneven batched workload: n seeded SW scenes and camera images.
 
     Per scene: cloud grid uniform in [32, 256]^2 x [32, 96] cells (even),
-    100 m cells with the cfg0 LWC law; image N^2 with N uniform in
+    100 m cells with the benchmark scenes' LWC law; image N^2 with N uniform in
     [64, 512] (multiple of 8); realisations per pixel log-uniform in
     [16, 1024] (times spp_scale, a multiple of 16: 4 steps of a multiple of
     4 each); 1-16 k-points. The camera
-    looks at the grid from its south edge like cfg0's.
+    looks at the grid from its south edge like the small scene's camera.
     Returns [(Synth, MapDesc)]."""
     rng = np.random.default_rng(0x0BA7C4 + seed)
     out = []

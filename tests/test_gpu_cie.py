@@ -3,7 +3,7 @@ src/atmosphere/htrdr_atmosphere_draw_map.c:110-207, with the wavelength
 sampler of src/core/htrdr_ran_wlen_cie_xyz.c) against the oracle and a
 closed form.
 
-Scene: cfg0's clouds and gas with 4 shortwave bands over 380-780 nm. Bars as
+Scene: the small seeded cloud field and its gas with 4 shortwave bands over 380-780 nm. Bars as
 in test_gpu_parity_maps.py: FP64 device vs oracle single realisations >= 97%
 identical to 1e-9; per pixel and channel >= 99.5% within 3 combined sigma;
 FP32 vs FP64 >= 99% within 3 sigma, |mean z| < 0.1, z std < 1.2; the vacuum

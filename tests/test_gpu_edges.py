@@ -25,7 +25,7 @@ SIZES = [(1, 1), (13, 7), (9, 17), (65, 3)]
 
 
 @pytest.fixture(scope="module")
-def cfg0():
+def small():
     s = A.Synth(0, seed=4)
     ctx = A.Context(s.scene_ptr)
     sky = O.Sky(s.scene_ptr)
@@ -40,8 +40,8 @@ def desc(w, h, spp, solver, **kw):
 
 
 @pytest.mark.parametrize("w,h", SIZES)
-def test_ragged_images(cfg0, w, h):
-    _, ctx, sky = cfg0
+def test_ragged_images(small, w, h):
+    _, ctx, sky = small
     for solver in (F.SOLVER_PLAIN_FP64, F.SOLVER_WAVEFRONT_FP32):
         for spp in (1, 7):
             p = ctx.draw_map(desc(w, h, spp, solver))
@@ -54,10 +54,10 @@ def test_ragged_images(cfg0, w, h):
 
 
 @pytest.mark.parametrize("chunk", [1, 2, 4, 16])
-def test_fp32_odd_slices_add_up(cfg0, chunk):
+def test_fp32_odd_slices_add_up(small, chunk):
     """spp 3 + 4 + 6 at offsets 0, 3, 7 into one accumulator = spp 13 in one
     launch, whatever the chunk size (13 is a multiple of none)"""
-    _, ctx, _ = cfg0
+    _, ctx, _ = small
     ctx.set_wavefront_params(24, chunk, 0)
     w, h = 13, 7
     npix = w * h
@@ -78,10 +78,10 @@ def test_fp32_odd_slices_add_up(cfg0, chunk):
     np.testing.assert_allclose(Ea, Eb, rtol=1e-12, atol=0)
 
 
-def test_row_pitch(cfg0):
+def test_row_pitch(small):
     """rows written `pitch' bytes apart; the padding between them keeps the
     caller's bytes"""
-    _, ctx, _ = cfg0
+    _, ctx, _ = small
     lib = F.load()
     w, h = 13, 7
     for solver in (F.SOLVER_PLAIN_FP64, F.SOLVER_WAVEFRONT_FP32):

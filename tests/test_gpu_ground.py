@@ -28,7 +28,7 @@ DET_RTOL = {F.SOLVER_PLAIN_FP64: 1e-9, F.SOLVER_WAVEFRONT_FP32: 2e-5}
 
 @pytest.fixture(scope="module")
 def hills():
-    """cfg0's clouds over a 33x33-vertex fBm terrain of 400 m relief"""
+    """The small seeded cloud field over a 33x33-vertex fBm terrain of 400 m relief"""
     s = A.Synth(0, seed=7, ground_type=F.GROUND_MESH, hf_res=33, hf_octaves=5, hf_relief=400.0,
                 hf_hurst=0.7)
     ctx = A.Context(s.scene_ptr)

@@ -3,7 +3,7 @@ specular BRDFs, two-sided interfaces and the sky material of the rectangle
 sensor; src/core/htrdr_materials.c:415-463, src/core/htrdr_interface.h:45-87,
 ..._draw_map.c:75-100) against the oracle and closed forms, both solvers.
 
-Scene: cfg0's clouds over a 33x33-vertex fBm terrain whose four height bands
+Scene: the small seeded cloud field (64x64x32 cells) over a 33x33-vertex fBm terrain whose four height bands
 get general materials: a wavelength list interpolated at 550 nm, a band list
 mixing a specular and a Lambertian entry around 550 nm, a mirror, a wide
 Lambertian band, and the air; one interface reversed (air in front), one
@@ -218,8 +218,8 @@ def test_grey_ground_in_the_general_model(solver):
 
 
 def test_ground_files_parity(tmp_path):
-    """A terrain read from htrdr-obj/htrdr-materials/mrumtl files under cfg0's
-    clouds: device FP64 vs oracle realisations, FP32 vs FP64 pixels"""
+    """A terrain read from htrdr-obj/htrdr-materials/mrumtl files under the small
+    seeded cloud field: device FP64 vs oracle realisations, FP32 vs FP64 pixels"""
     from test_cli import write_ground
     obj, mats = write_ground(tmp_path)
     s = A.Synth(0, seed=3, ground_type=F.GROUND_NONE)

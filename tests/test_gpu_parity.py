@@ -94,7 +94,7 @@ def test_realisations_match_oracle(ctx0, sky0):
     assert np.all(np.isfinite(g)) and np.all(g >= 0)
 
 
-def test_draw_map_cfg0_3sigma(ctx0, sky0):
+def test_draw_map_small_scene_3sigma(ctx0, sky0):
     desc = A.map_preset(0, spp=16)
     gpu = ctx0.draw_map(desc)
     acc = sky0.draw_map(desc)

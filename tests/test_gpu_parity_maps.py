@@ -5,7 +5,7 @@ FP64 one on them.
 
 Scenes: the LW configuration (4-20 um, 8 bands x 16 k-points, 16000 one-nm
 CDF bands, temperature lapse, log-uniform gas ka) on a 64x64x32 cloud grid so
-that the oracle stays fast, and the cfg0 SW scene. Bars:
+that the oracle stays fast, and the small shortwave scene (64x64x32 cloud cells). Bars:
 * FP64 device vs oracle, single realisations on the same keys: >= 97%
   identical to 1e-9 (the rest differ through last-ulp transcendental
   differences that flip a branch);

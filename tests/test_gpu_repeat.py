@@ -106,7 +106,7 @@ def test_repeated_mesh_tiles_the_plane(solver):
 
 
 def test_repeated_terrain_realisations():
-    """cfg0's clouds over a repeated 17x17-vertex fBm terrain (seams
+    """The small seeded cloud field over a repeated 17x17-vertex fBm terrain (seams
     included), looking across many copies of it"""
     from test_oracle_ground import terrain
     s, _, _, _ = terrain(res=17, relief=300.0, seed=4)

@@ -24,8 +24,8 @@ def zscores(a, b):
     return z, m
 
 
-def test_wavefront_vs_plain_cfg0(ctx0):
-    """cfg0 at 64 spp: FP32 wavefront vs FP64 plain. Bars: >= 99% of pixels
+def test_wavefront_vs_plain_small_scene(ctx0):
+    """The small shortwave scene (64x64x32 cloud cells over 6.4 km) at 64 spp: FP32 wavefront vs FP64 plain. Bars: >= 99% of pixels
     within 3 combined sigma (heavy-tailed per-pixel estimates at 64 spp), the
     z-scores have |mean| < 0.1 and std < 1.2, the image means agree within 3
     sigma of their difference."""
