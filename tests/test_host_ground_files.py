@@ -13,6 +13,7 @@ import math
 
 import numpy as np
 import pytest
+from hypothesis import HealthCheck, given, settings, strategies as st
 
 import oracle_ffi as O
 from htrdr_kmin_amd import _ffi as F
@@ -257,3 +258,36 @@ def test_loaded_ground_in_the_oracle(files):
         math.radians(50.0)) * (1 + math.cos(SUN_HALF_ANGLE)) / 2
     m, se = w.mean(), w.std(ddof=1) / math.sqrt(w.size)
     assert abs(m - expect) <= 4 * se and se < 0.02 * expect
+
+
+LINE = st.one_of(
+    st.text(alphabet=" \t#:/-.0123456789eabcdfilmnorstuvw\r", max_size=40),
+    st.sampled_from(["v 0 0 0", "v 1 0 0", "v 1 1 0", "f 1 2 3", "f -1 -2 -3", "f 1/1/1 2//2 3",
+                     "usemtl air:ground", "usemtl ground:air", "usemtl air:ground:air", "usemtl :",
+                     "f 1 2", "f 0 1 2", "v nan 0 0", "v 1e999 0 0", "f 99999999999999999999 1 2"]))
+
+
+@settings(max_examples=300, deadline=None, suppress_health_check=[HealthCheck.function_scoped_fixture])
+@given(obj=st.tuples(st.lists(LINE, max_size=6), st.lists(LINE, max_size=6)).map(
+           lambda a: a[0] + ["v 0 0 0", "v 1 0 0", "v 0 1 0", "usemtl air:ground"] + a[1] + ["f 1 3 2"]),
+       mtl=st.sampled_from(["wavelengths 2\n400 lambertian 0.5\n600 specular 1\n",
+                            "bands 1\n0 1e9 lambertian 0\n", "wavelengths 0\n", "bands 2\n1 2 lambertian 0.5\n",
+                            "wavelengths 1\n500 lambertian nan\n", "\x00\x01garbage\n", ""]))
+def test_malformed_files_fail_cleanly(files, obj, mtl):
+    """Random obj lines and mrumtl texts: the reader returns a description or
+    an error carrying a message, and never crashes; a description it returns
+    has triangles whose vertex indices all lie in the vertex array"""
+    tmp, write, _ = files
+    write("f.mrumtl", mtl)
+    mats = write("m_fuzz", f"ground {tmp / 'f.mrumtl'} 300\nair none\n")
+    o = write("fuzz.obj", "\n".join(obj) + "\n")
+    try:
+        g = GroundFiles(o, mats)
+    except F.HtgxError as e:
+        assert e.code in (F.BAD_ARG, F.IO_ERR) and "(" in str(e)
+        return
+    with g:
+        d = g.desc
+        assert d.ntris > 0 and d.ninterfaces > 0
+        t = np.ctypeslib.as_array(d.tris, shape=(d.ntris, 3))
+        assert t.max() < d.nverts
