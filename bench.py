@@ -39,9 +39,10 @@ sys.path.insert(0, ROOT)
 METRIC = "Mrealisations/s & time-to-0.5%-std-error, FP32 batched vs FP64 plain"
 SOLVER_NAMES = {0: "FP64 plain", 1: "FP32 wavefront"}
 HBM_PEAK_GBS = 8000.0
-# (solver, config) -> rocprofv3 summary of the kernel on that workload, for
-# its 16-realisation launches
-PROFILED = {("wavefront", 1): "traffic_wavefront_sw_image_1024.json"}
+# (solver, config) -> rocprofv3 summary of the kernel on that workload and the
+# map width it was taken at, for its 16-realisation launches
+PROFILED = {("wavefront", 1): ("traffic_wavefront_sw_image_1024.json", 1024),
+            ("wavefront", 4): ("traffic_wavefront_mesh_flux_512.json", 512)}
 
 
 def parse():
@@ -394,9 +395,9 @@ def main():
     # rocprofv3 counters of the same launches, summarised under profiles/
     # (named for the kernel and workload they cover)
     traffic, counters_prof = None, {}
-    tname = PROFILED.get((args.solver, args.config))
+    tname, twidth = PROFILED.get((args.solver, args.config), (None, 0))
     tfile = os.path.join(ROOT, "profiles", tname) if tname else None
-    if tfile and os.path.exists(tfile) and S == 16 and desc.width == 1024:
+    if tfile and os.path.exists(tfile) and S == 16 and desc.width == twidth:
         with open(tfile) as f:
             counters_prof = json.load(f)
         traffic = counters_prof.get("bytes_per_launch")
