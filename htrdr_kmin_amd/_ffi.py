@@ -149,6 +149,8 @@ class SynthParams(C.Structure):
 # Entry points: name -> (restype, argtypes)
 SIGNATURES = {
     "htgx_ctx_create": (C.c_int, [C.POINTER(SceneDesc), C.POINTER(C.c_void_p)]),
+    "htgx_ctx_create_cached": (C.c_int, [C.POINTER(SceneDesc), C.c_char_p, C.POINTER(C.c_int),
+                                         C.c_char_p, C.c_size_t, C.POINTER(C.c_void_p)]),
     "htgx_ctx_destroy": (C.c_int, [C.c_void_p]),
     "htgx_ctx_get_svo_info": (C.c_int, [C.c_void_p, c_uint32_p, c_uint64_p, c_uint64_p]),
     "htgx_draw_map": (C.c_int, [C.c_void_p, C.POINTER(MapDesc), C.c_void_p, C.c_size_t]),

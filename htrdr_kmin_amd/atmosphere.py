@@ -9,6 +9,7 @@ per-path time and, for flux maps, the overall flux. Errors are the library's
 rsys-compatible result codes, raised as ``HtgxError``.
 """
 import ctypes as C
+import os
 import math
 
 import numpy as np
@@ -168,12 +169,26 @@ def map_preset(config, **overrides):
 
 
 class Context:
-    """A libhtgx context: the scene resident on the current HIP device."""
+    """A libhtgx context: the scene resident on the current HIP device.
 
-    def __init__(self, scene_ptr):
+    cache: path of the octree cache (option -O of htrdr-atmosphere,
+    htgx_ctx_create_cached): read when it holds the octree of the same inputs,
+    written when it does not exist, refused (HtgxError with the message)
+    when it was built from other inputs. `cache_loaded' says which."""
+
+    def __init__(self, scene_ptr, cache=None):
         self._lib = F.load()
         self._h = C.c_void_p()
-        F.check("htgx_ctx_create", self._lib.htgx_ctx_create(scene_ptr, C.byref(self._h)))
+        self.cache_loaded = False
+        if cache is None:
+            F.check("htgx_ctx_create", self._lib.htgx_ctx_create(scene_ptr, C.byref(self._h)))
+            return
+        loaded, err = C.c_int(0), C.create_string_buffer(512)
+        res = self._lib.htgx_ctx_create_cached(scene_ptr, os.fsencode(cache), C.byref(loaded),
+                                               err, len(err), C.byref(self._h))
+        if res != F.OK:
+            raise F.HtgxError(f"htgx_ctx_create_cached ({err.value.decode()})", res)
+        self.cache_loaded = bool(loaded.value)
 
     @property
     def handle(self):

@@ -19,8 +19,11 @@
  *   plane | synth                an infinite plane (the quad repeated with -R)
  *                                or the configuration's fBm heightfield.
  * -d writes the octrees (htgx_dump_octrees_vtk) instead of rendering.
+ * -O reads the octree from the cache file, or writes it there when the file
+ * does not exist (htgx_ctx_create_cached); a cache built from other inputs
+ * aborts the run.
  * Options without a counterpart on the GPU path are rejected with an error
- * (-C focal-length) or accepted and ignored (-t, -V, -O).
+ * (-C focal-length) or accepted and ignored (-t, -V).
  * The solver is the FP32 wavefront one; HTGX_SOLVER=plain selects the FP64
  * plain one.
  */
@@ -58,6 +61,7 @@ struct args {
   double wlen[2], tref;
   double sun_az, sun_el;
   double thickness;
+  const char* cache; /* -O: octree cache file */
 };
 
 static void
@@ -242,7 +246,7 @@ parse_args(int argc, char** argv, struct args* a)
       case 'M': a->mtl = optarg; break;
       case 'm': a->mie = optarg; break;
       case 'n': a->sky_mtl = optarg; break;
-      case 'O': break; /* no octree cache: the octree is rebuilt in a fraction of a second */
+      case 'O': a->cache = optarg; break;
       case 'o': a->output = optarg; break;
       case 'P': a->output_type = 0; a->ortho = 1; bad = parse_list(optarg, ortho_param, a); break;
       case 'p': a->output_type = 1; bad = parse_list(optarg, rect_param, a); break;
@@ -465,7 +469,15 @@ main(int argc, char** argv)
     else if(res != HTGX_OK) fprintf(stderr, "%s: could not write the octrees (%d)\n", argv[0], res);
     goto error;
   }
-  if((res = htgx_ctx_create(&scene, &ctx)) != HTGX_OK) {
+  if(a.cache) { /* htsky's cache_filename, src/atmosphere/htrdr_atmosphere.c:369 */
+    char msg[512];
+    int loaded = 0;
+    if((res = htgx_ctx_create_cached(&scene, a.cache, &loaded, msg, sizeof(msg), &ctx)) != HTGX_OK) {
+      fprintf(stderr, "%s: could not set up the scene on the GPU (%d)%s%s\n", argv[0], res,
+              msg[0] ? ": " : "", msg);
+      goto error;
+    }
+  } else if((res = htgx_ctx_create(&scene, &ctx)) != HTGX_OK) {
     fprintf(stderr, "%s: could not set up the scene on the GPU (%d)\n", argv[0], res);
     goto error;
   }

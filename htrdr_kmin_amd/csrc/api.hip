@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -359,10 +360,35 @@ check_scene(const htgx_scene_desc* s)
   /* a scene may have no gas layers (nlayers = 0: tests, cloud-only media) */
   if(s->gas.nlayers && (!(s->gas.layer_thickness > 0) || !s->gas.ka || !s->gas.ks))
     return HTGX_BAD_ARG;
+  /* coefficients finite and in range: NaN or infinite extinctions would
+   * leave the tracking loops with no progress to make */
+  auto nonneg = [](const double* v, size_t n) {
+    for(size_t i = 0; i < n; ++i) if(!(v[i] >= 0 && v[i] <= DBL_MAX)) return false;
+    return true;
+  };
+  uint64_t nq = 0;
+  for(uint32_t b = 0; b < sp.nbands; ++b) nq += sp.nquads[b];
+  if(s->gas.nlayers) {
+    if(!nonneg(s->gas.ka, nq * s->gas.nlayers) || !nonneg(s->gas.ks, (size_t)sp.nbands * s->gas.nlayers))
+      return HTGX_BAD_ARG;
+    if(s->gas.temperature && !nonneg(s->gas.temperature, s->gas.nlayers)) return HTGX_BAD_ARG;
+  }
   const htgx_cloud_desc& c = s->clouds;
   const bool clouds = c.def[0] || c.def[1] || c.def[2];
   if(clouds && (!c.def[0] || !c.def[1] || !c.def[2] || !c.lwc || !c.cext || !c.ssa || !c.hg_g))
     return HTGX_BAD_ARG;
+  if(clouds) {
+    for(int a = 0; a < 3; ++a)
+      if(!(c.upper[a] > c.lower[a]) || !std::isfinite(c.lower[a]) || !std::isfinite(c.upper[a]))
+        return HTGX_BAD_ARG;
+    if(!nonneg(c.cext, sp.nbands)) return HTGX_BAD_ARG;
+    for(uint32_t b = 0; b < sp.nbands; ++b)
+      if(!(c.ssa[b] >= 0 && c.ssa[b] <= 1) || !(c.hg_g[b] > -1 && c.hg_g[b] < 1)) return HTGX_BAD_ARG;
+    const size_t ncells = (size_t)c.def[0] * c.def[1] * c.def[2];
+    bool ok = true;
+    for(size_t i = 0; i < ncells; ++i) ok &= c.lwc[i] >= 0.f && c.lwc[i] <= FLT_MAX;
+    if(!ok) return HTGX_BAD_ARG;
+  }
   if(s->ground.type == HTGX_GROUND_PLANE || s->ground.type == HTGX_GROUND_MESH) {
     const htgx_ground_desc& g = s->ground;
     int res = check_materials(g);
@@ -565,6 +591,27 @@ htgx_ctx_create(const htgx_scene_desc* scene, htgx_ctx** out)
   return HTGX_OK;
 }
 
+extern "C" int
+htgx_ctx_create_cached(const htgx_scene_desc* scene, const char* cache_path, int* loaded,
+                       char* err, size_t err_len, htgx_ctx** out)
+{
+  int res;
+  if(!out || !cache_path || !*cache_path) return HTGX_BAD_ARG;
+  if(err && err_len) err[0] = 0;
+  if((res = check_scene(scene)) != HTGX_OK) return res;
+  htgx_ctx* ctx = new(std::nothrow) htgx_ctx();
+  if(!ctx) return HTGX_MEM_ERR;
+  *out = nullptr;
+  ctx->cache_path = cache_path;
+  res = htgx_ctx_init(ctx, *scene);
+  ctx->cache_path = nullptr;
+  if(loaded) *loaded = ctx->cache_loaded ? 1 : 0;
+  if(err && err_len) std::snprintf(err, err_len, "%s", ctx->cache_err.c_str());
+  if(res != HTGX_OK) { htgx_ctx_destroy(ctx); return res; }
+  *out = ctx;
+  return HTGX_OK;
+}
+
 int
 htgx_ctx_init(htgx_ctx* ctx, const htgx_scene_desc& sc)
 {
@@ -619,7 +666,21 @@ htgx_ctx_init(htgx_ctx* ctx, const htgx_scene_desc& sc)
     /* the octree is built on the device from the uploaded LWC grid
      * (svo_device.hip); only its shape is computed here */
     SvoHost svo;
-    if((res = svo_layout(sc, svo)) != HTGX_OK) return res;
+    SvoCache cache;
+    uint64_t digest = 0;
+    htgx_scene_desc lsc = sc;
+    if(ctx->cache_path) {
+      digest = svo_cache_digest(sc);
+      bool found = false;
+      if((res = svo_cache_read(ctx->cache_path, digest, cache, &found, ctx->cache_err)))
+        return res;
+      if(found) { /* the cache's own threshold and leaf block (-T, -V ignored) */
+        lsc.leaf_block = cache.leaf_block;
+        lsc.optical_thickness = cache.optical_thickness;
+        ctx->cache_loaded = true;
+      }
+    }
+    if((res = svo_layout(lsc, svo)) != HTGX_OK) return res;
     S.has_clouds = 1;
     S.B = svo.B;
     for(int a = 0; a < 3; ++a) {
@@ -640,9 +701,33 @@ htgx_ctx_init(htgx_ctx* ctx, const htgx_scene_desc& sc)
       uint32_t* lmap = nullptr;
       if((res = device_alloc(ctx, svo.nnodes, &bounds))) return res;
       if((res = device_alloc(ctx, nblocks, &lmap))) return res;
-      if((res = svo_build_device(svo, S.lwc, cl.def, sc.optical_thickness, bounds, lmap,
-                                 &svo.nleaves)) != HTGX_OK)
-        return res;
+      if(ctx->cache_loaded) {
+        if(cache.lmap.size() != nblocks || cache.bounds.size() != svo.nnodes
+        || cache.nlevels != svo.nlevels) {
+          ctx->cache_err = std::string("the cache `") + ctx->cache_path + "' does not fit the cloud grid";
+          return HTGX_BAD_ARG;
+        }
+        HIP_OK(hipMemcpy(lmap, cache.lmap.data(), nblocks * sizeof(uint32_t), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(bounds, cache.bounds.data(), svo.nnodes * sizeof(float2),
+                         hipMemcpyHostToDevice));
+        svo.nleaves = cache.nleaves;
+      } else {
+        if((res = svo_build_device(svo, S.lwc, cl.def, lsc.optical_thickness, bounds, lmap,
+                                   &svo.nleaves)) != HTGX_OK)
+          return res;
+        if(ctx->cache_path) {
+          cache.leaf_block = svo.B;
+          cache.nlevels = svo.nlevels;
+          cache.optical_thickness = lsc.optical_thickness;
+          cache.nleaves = svo.nleaves;
+          cache.lmap.resize(nblocks);
+          cache.bounds.resize(svo.nnodes);
+          HIP_OK(hipMemcpy(cache.lmap.data(), lmap, nblocks * sizeof(uint32_t), hipMemcpyDeviceToHost));
+          HIP_OK(hipMemcpy(cache.bounds.data(), bounds, svo.nnodes * sizeof(float2),
+                           hipMemcpyDeviceToHost));
+          if((res = svo_cache_write(ctx->cache_path, digest, cache, ctx->cache_err))) return res;
+        }
+      }
       S.bounds = bounds;
       S.lmap = lmap;
     }

@@ -6,6 +6,7 @@
 #include "htgx_device.h"
 
 #include <hip/hip_runtime.h>
+#include <string>
 #include <vector>
 
 struct htgx_wavefront_state;
@@ -25,6 +26,9 @@ struct htgx_ctx {
   uint32_t nmtls = 0;     /* ground materials */
   uint32_t bvh_depth = 0; /* mesh ground BVH depth */
   size_t svo_bytes = 0;
+  const char* cache_path = nullptr; /* option -O, during set-up only */
+  bool cache_loaded = false;
+  std::string cache_err;
   double us_per_tick = 0.01;
   /* per-slice partial accumulators of the plain solver */
   void* d_partials = nullptr;

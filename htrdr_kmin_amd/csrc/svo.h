@@ -24,6 +24,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <string>
 #include <vector>
 
 namespace htgx {
@@ -60,6 +61,24 @@ int svo_layout(const htgx_scene_desc& scene, SvoHost& svo);
  * for bit (svo holds the layout from svo_layout; its nleaves is set). */
 int svo_build_device(const SvoHost& layout, const float* lwc, const uint32_t def[3],
                      double optical_thickness, float2* bounds, uint32_t* lmap, uint64_t* nleaves);
+
+/* Octree cache of option -O (svo_cache.cpp): the tree and the build
+ * parameters it was made with */
+struct SvoCache {
+  uint32_t leaf_block = 0, nlevels = 0;
+  double optical_thickness = 0;
+  uint64_t nleaves = 0;
+  std::vector<uint32_t> lmap;
+  std::vector<float2> bounds;
+};
+
+/* Digest of the inputs a cache depends on (bands, gas, clouds, cloud optics) */
+uint64_t svo_cache_digest(const htgx_scene_desc& scene);
+/* *found = false and HTGX_OK if the file does not exist; HTGX_BAD_ARG (err
+ * says why) if it is not a cache, was built from other inputs or is
+ * truncated; HTGX_IO_ERR if it cannot be opened */
+int svo_cache_read(const char* path, uint64_t digest, SvoCache& out, bool* found, std::string& err);
+int svo_cache_write(const char* path, uint64_t digest, const SvoCache& cache, std::string& err);
 
 } /* namespace htgx */
 

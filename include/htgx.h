@@ -316,6 +316,22 @@ struct htgx_ctx;
  * Replaces the htsky_create + ground/sun setup of htrdr_atmosphere_create
  * (src/atmosphere/htrdr_atmosphere.c:348-382). */
 int htgx_ctx_create(const struct htgx_scene_desc* scene, struct htgx_ctx** ctx);
+
+/* htgx_ctx_create with the octree cache of option -O (htsky_args
+ * cache_filename, src/atmosphere/htrdr_atmosphere.c:369;
+ * doc/htrdr-atmosphere.1.in:283-309). If cache_path does not exist, the
+ * octree is built and written there. If it exists and was built from the
+ * same atmosphere, cloud and Mie data (spectral bands and quadrature, gas
+ * layers, cloud grid and LWC, cloud optics), the octree is read from it
+ * instead, with the merge threshold and leaf block it was built with
+ * (scene->optical_thickness and leaf_block are then ignored, as the reference
+ * ignores -T and -V). A cache built from other data, or not a cache, is
+ * refused with HTGX_BAD_ARG. *loaded (may be NULL) says whether the cache was
+ * read; err (may be NULL) receives a message on failure. A scene without
+ * clouds neither reads nor writes the file. */
+int htgx_ctx_create_cached
+  (const struct htgx_scene_desc* scene, const char* cache_path, int* loaded,
+   char* err, size_t err_len, struct htgx_ctx** ctx);
 int htgx_ctx_destroy(struct htgx_ctx* ctx);
 
 /* Number of nodes, leaves and octree levels (diagnostics). */

@@ -200,3 +200,29 @@ def test_cli_ground_files(tmp_path, kind):
     assert np.array_equal(cli[..., :6], lib[..., :6])
     if kind == "flux":  # the rectangle cuts through the terrain
         assert 0 < px["flux_n"].sum() < 24 * 16 * 8
+
+
+def columns(path, n=6):
+    return [l.split()[:n] for l in path.read_text().splitlines() if l.strip()]
+
+
+@pytest.mark.gpu
+def test_cli_octree_cache(tmp_path):
+    """-O: the first run writes the cache, the second reads it and prints the
+    same estimates; a run with other cloud data and the same cache aborts
+    (doc/htrdr-atmosphere.1.in:283-309)"""
+    cache = tmp_path / "octree.cache"
+    args = ["-a", "synth:0,3", "-c", "synth", "-g", "plane", "-D", "0,60", "-f", "-s", "sw=550,550",
+            "-i", "def=16x8:spp=8", "-O", str(cache),
+            "-C", "pos=3200,-2000,5000:tgt=3200,-1999.5,4999.133974596:up=0,0,1:fov=60"]
+    r1 = run(*args, "-o", str(tmp_path / "a.txt"))
+    assert r1.returncode == 0, r1.stderr
+    assert cache.exists()
+    data = cache.read_bytes()
+    r2 = run(*args, "-T", "5", "-o", str(tmp_path / "b.txt"))
+    assert r2.returncode == 0, r2.stderr
+    assert columns(tmp_path / "a.txt") == columns(tmp_path / "b.txt")
+    other = [a if a != "synth:0,3" else "synth:0,4" for a in args]
+    r3 = run(*other, "-o", str(tmp_path / "c.txt"))
+    assert r3.returncode != 0 and "other atmosphere, cloud or Mie inputs" in r3.stderr
+    assert cache.read_bytes() == data

@@ -1,0 +1,105 @@
+"""The octree cache of option -O (htgx_ctx_create_cached; htsky's
+cache_filename, src/atmosphere/htrdr_atmosphere.c:369, as
+doc/htrdr-atmosphere.1.in:283-309 specifies it): the first run writes the
+tree, the next reads it and renders the same map bit for bit; a cache in use
+keeps the threshold and leaf block it was built with (-T and -V ignored); a
+cache built from other cloud, gas or Mie data, a file that is not a cache and
+a truncated cache are refused with a message, and the file is left as it
+was."""
+import numpy as np
+import pytest
+
+from htrdr_kmin_amd import _ffi as F
+from htrdr_kmin_amd import atmosphere as A
+from scenes import Scene
+
+pytestmark = pytest.mark.gpu
+
+
+def scene(seed=0, T=1.0, block=2, cext=0.1, ks=1e-6):
+    rng = np.random.default_rng(seed)
+    shape = (8, 12, 16)
+    lwc = np.exp(rng.normal(-1.0, 1.0, shape)).astype(np.float32)
+    lwc[rng.random(shape) < 0.3] = 0.0
+    return Scene(lwc=lwc, lower=(0.0, 0.0, 1000.0), upper=(1600.0, 1200.0, 1800.0), cext=cext,
+                 nlayers=2, dz=1000.0, ks=ks, optical_thickness=T, leaf_block=block)
+
+
+def render(ctx):
+    """a small FP64 image looking down through the clouds"""
+    d = A.map_preset(0, width=8, height=8, spp=16, solver=F.SOLVER_PLAIN_FP64,
+                     pos=(800.0, 600.0, 3000.0), tgt=(800.0, 600.0, 0.0), up=(0.0, 1.0, 0.0), fov_y=40.0)
+    px = ctx.draw_map(d)
+    assert np.all(px["time_n"] == 16) and np.any(px["radiance_E"] > 0)
+    return px
+
+
+def same(a, b):
+    """same estimates bit for bit (the time columns are wall-clock figures)"""
+    return all(np.array_equal(a[k], b[k]) for k in ("radiance_E", "radiance_SE"))
+
+
+def test_written_then_read(tmp_path):
+    path = tmp_path / "octree.cache"
+    sc = scene()
+    with A.Context(sc.ptr) as ref:
+        info, px = ref.svo_info(), render(ref)
+    with A.Context(sc.ptr, cache=path) as c1:
+        assert not c1.cache_loaded and path.exists()
+        assert c1.svo_info() == info
+        assert same(render(c1), px)
+    data = path.read_bytes()
+    assert data[:8] == b"HTGXOCT1"
+    with A.Context(sc.ptr, cache=path) as c2:
+        assert c2.cache_loaded
+        assert c2.svo_info() == info
+        assert same(render(c2), px)
+    assert path.read_bytes() == data  # reading leaves the file alone
+
+
+def test_cache_keeps_its_threshold_and_block(tmp_path):
+    """-T and the leaf block of a later run are ignored while the cache is used"""
+    path = tmp_path / "octree.cache"
+    first = scene(T=1.0, block=2)  # the descriptor lives as long as its Scene
+    with A.Context(first.ptr, cache=path) as c1:
+        info, px = c1.svo_info(), render(c1)
+    other = scene(T=1e9, block=4)
+    with A.Context(other.ptr) as fresh:
+        assert fresh.svo_info() != info  # the options alone would give another tree
+    with A.Context(other.ptr, cache=path) as c2:
+        assert c2.cache_loaded and c2.svo_info() == info
+        assert same(render(c2), px)
+
+
+@pytest.mark.parametrize("change", ["lwc", "cext", "gas"])
+def test_other_inputs_are_refused(tmp_path, change):
+    path = tmp_path / "octree.cache"
+    first = scene()
+    with A.Context(first.ptr, cache=path):
+        pass
+    data = path.read_bytes()
+    other = {"lwc": scene(seed=1), "cext": scene(cext=0.2), "gas": scene(ks=1e-5)}[change]
+    with pytest.raises(F.HtgxError) as e:
+        A.Context(other.ptr, cache=path)
+    assert e.value.code == F.BAD_ARG and "other atmosphere, cloud or Mie inputs" in str(e.value)
+    assert path.read_bytes() == data
+
+
+def test_bad_files_are_refused(tmp_path):
+    sc = scene()
+    junk = tmp_path / "junk"
+    junk.write_bytes(b"not an octree cache at all" * 4)
+    with pytest.raises(F.HtgxError) as e:
+        A.Context(sc.ptr, cache=junk)
+    assert e.value.code == F.BAD_ARG and "is not an octree cache" in str(e.value)
+    path = tmp_path / "octree.cache"
+    with A.Context(sc.ptr, cache=path):
+        pass
+    cut = tmp_path / "cut"
+    cut.write_bytes(path.read_bytes()[:-5])
+    with pytest.raises(F.HtgxError) as e:
+        A.Context(sc.ptr, cache=cut)
+    assert e.value.code == F.BAD_ARG and "truncated" in str(e.value)
+    with pytest.raises(F.HtgxError) as e:
+        A.Context(sc.ptr, cache=tmp_path / "no_such_dir" / "x")  # cannot be created
+    assert e.value.code == F.IO_ERR
