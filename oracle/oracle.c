@@ -2122,6 +2122,22 @@ oracle_radiance
     : radiance_sw(s, &rng, cpnt_mask, pos, dir, wlen, iband, iquad);
 }
 
+void
+oracle_radiance_many
+  (const struct oracle_sky* s, uint64_t seed, uint64_t pixel, uint64_t realisation0,
+   uint64_t n, int spectral_type, int cpnt_mask, const double pos[3],
+   const double dir[3], double wlen, uint32_t iband, uint32_t iquad, double* out,
+   int nthreads)
+{
+  int64_t i;
+  if(nthreads > 0) omp_set_num_threads(nthreads);
+  #pragma omp parallel for schedule(static, 4096)
+  for(i = 0; i < (int64_t)n; ++i) {
+    out[i] = oracle_radiance(s, seed, pixel, realisation0 + (uint64_t)i, spectral_type,
+                             cpnt_mask, pos, dir, wlen, iband, iquad);
+  }
+}
+
 /*******************************************************************************
  * Per-pixel realisations: ..._draw_map.c:52-107, 210-322, 325-436
  ******************************************************************************/

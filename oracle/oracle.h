@@ -108,6 +108,14 @@ double oracle_radiance
    const double pos[3], const double dir[3], double wlen,
    uint32_t iband, uint32_t iquad);
 
+/* oracle_radiance of realisations realisation0 .. realisation0 + n - 1 of one
+ * ray into out[n], nthreads OpenMP threads (0 = all) */
+void oracle_radiance_many
+  (const struct oracle_sky* sky, uint64_t seed, uint64_t pixel, uint64_t realisation0,
+   uint64_t n, int spectral_type, int cpnt_mask, const double pos[3],
+   const double dir[3], double wlen, uint32_t iband, uint32_t iquad, double* out,
+   int nthreads);
+
 /* The whole map into per-pixel accumulators (row-major; for a CIE XYZ
  * image 3 planes, X, Y, Z), nthreads OpenMP threads (0 = all). CIE channel c
  * draws from the streams keyed by seed ^ (c+1) << 56; oracle_realisation

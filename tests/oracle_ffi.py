@@ -30,6 +30,9 @@ _SIG = {
     "oracle_realisation": (C.c_double, [C.c_void_p, C.POINTER(F.MapDesc), C.c_uint64, C.c_uint64]),
     "oracle_radiance": (C.c_double, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_int, C.c_int,
                                      F.c_double_p, F.c_double_p, C.c_double, C.c_uint32, C.c_uint32]),
+    "oracle_radiance_many": (None, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_int,
+                                    C.c_int, F.c_double_p, F.c_double_p, C.c_double, C.c_uint32,
+                                    C.c_uint32, F.c_double_p, C.c_int]),
     "oracle_draw_map": (C.c_int, [C.c_void_p, C.POINTER(F.MapDesc), C.c_void_p, C.c_int]),
     "oracle_draw_pixels": (C.c_int, [C.c_void_p, C.POINTER(F.MapDesc), F.c_uint64_p, C.c_uint64,
                                      C.c_void_p, C.c_int]),

@@ -11,6 +11,7 @@ import math
 import numpy as np
 import pytest
 
+import closed_forms as CF
 from htrdr_kmin_amd import _ffi as F
 from htrdr_kmin_amd import atmosphere as A
 from scenes import Scene, camera, cells_along, planck, rectangle, sun_dir, sun_solid_angle, SUN_HALF_ANGLE
@@ -197,3 +198,59 @@ def test_looser_majorants_leave_the_estimate_unchanged(solver, config):
             b["flux_sum"], b["flux_sum2"], b["flux_n"])
     z = (E1 - E2) / np.sqrt(S1 ** 2 + S2 ** 2)
     assert np.mean(np.abs(z) <= 3) >= 0.98 and abs(z.mean()) < 0.2
+
+
+def pooled(E, SE):
+    """Mean of equally sampled pixel estimates and its standard error"""
+    return float(E.mean()), float(math.sqrt((SE ** 2).sum()) / E.size)
+
+
+def narrow_camera(pos, d, n, spp, solver, **kw):
+    """n x n pixels all looking along d (fov 1e-4 deg); straight-down views
+    take +y as the camera's up"""
+    d = np.asarray(d, np.float64) / np.linalg.norm(d)
+    up = (0, 1, 0) if abs(d[2]) > 0.99 else (0, 0, 1)
+    return camera(n, n, spp, pos=tuple(pos), tgt=tuple(np.asarray(pos) + d), up=up, fov=1e-4,
+                  solver=solver, **kw)
+
+
+@pytest.mark.parametrize("solver", SOLVERS)
+@pytest.mark.parametrize("view", sorted(CF.THIN_VIEWS))
+@pytest.mark.parametrize("medium", sorted(CF.THIN_MEDIA))
+def test_single_scattering_thin_slab(medium, view, solver):
+    """SW volume scattering on the device: the first-order closed form of the
+    thin slab (HG particles, Rayleigh gas, 3:1 mix; 20 and 150 degrees from
+    the sun), 4.2 M walks. Tolerance: 4 pooled standard errors + 0.5 % for
+    second-order scattering; a flipped scattering angle or swapped
+    gas/particle odds lie more than 5 tolerances away."""
+    ks_gas, ks_par = CF.THIN_MEDIA[medium]
+    pos, d, _ = CF.THIN_VIEWS[view]
+    p = render(CF.thin_slab_scene(ks_gas, ks_par), narrow_camera(pos, d, 16, 16384, solver))
+    assert np.all(p["time_n"] == 16384)
+    m, se = pooled(p["radiance_E"] * 1e9, p["radiance_SE"] * 1e9)  # per nm -> per m (pdf 1e9 m^-1)
+    expect, alternatives = CF.thin_slab_expectations(L_SUN_550, ks_gas, ks_par, view)
+    tol = 4 * se + CF.THIN_MS_RTOL * expect
+    assert abs(m - expect) <= tol, (m, expect, se)
+    for name, alt in alternatives.items():
+        if abs(alt - expect) > 0.2 * expect:
+            assert abs(m - alt) > 5 * tol, name
+
+
+@pytest.mark.parametrize("solver", SOLVERS)
+@pytest.mark.parametrize("view", sorted(CF.LW_VIEWS))
+@pytest.mark.parametrize("medium", ["gas", "cloud"])
+def test_lw_emission_under_a_temperature_gradient(medium, view, solver):
+    """LW emission at the absorption point's layer temperature on the device:
+    ten layers 9 K apart, absorbing gas or an absorbing cloud through six of
+    them, seen up and down from three heights, 262 k walks. Tolerance: 4
+    pooled standard errors + 2e-5 (FP32 Planck); a layer index off by one or
+    the starting layer's temperature lie more than 10 standard errors away."""
+    pos, d, _ = CF.LW_VIEWS[view]
+    desc = narrow_camera(pos, d, 8, 4096, solver, wlen=(CF.LW_WLEN_NM, CF.LW_WLEN_NM),
+                         spectral=F.SPECTRAL_LW, Tref=280.0)
+    p = render(CF.lw_layers_scene(medium), desc)
+    m, se = pooled(p["radiance_E"] * 1e9, p["radiance_SE"] * 1e9)
+    expect = CF.lw_layers_expectations(medium, view)
+    assert abs(m - expect) <= 4 * se + 2e-5 * expect, (m, expect, se)
+    for name, alt in CF.lw_layers_alternatives(medium, view).items():
+        assert abs(m - alt) > 10 * se, name

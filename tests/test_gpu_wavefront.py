@@ -5,6 +5,8 @@ Threefry word), so it is compared statistically: per-pixel differences in
 units of the combined Monte Carlo standard error, their mean and spread, and
 the image mean. Tolerances are stated in each test.
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -63,3 +65,37 @@ def test_wavefront_counters(ctx0):
     ctx0.set_counting(False)
     assert c["nrealisations"] == 32 * 32 * 8
     assert c["nleaf"] > 0 and c["ncoll"] > 0 and c["nevents"] > 0
+
+
+@pytest.fixture(scope="module")
+def ctx1():
+    s = A.Synth(1, seed=0)
+    with A.Context(s.scene_ptr) as ctx:
+        yield ctx
+    s.close()
+
+
+def test_precision_workload_window_vs_fp64(ctx1):
+    """The precision workload's transport: the full 256x256x64 cloud grid
+    over 25.6 km with 2^3 leaf blocks, seen by the central 256x256 window of
+    its 1024x1024 camera (same position and tilt, tan(fov/2) / 4), 16
+    realisations per pixel for each solver. The north-star bar on the
+    production solver: FP32 within 3 combined sigma of FP64 on >= 99.7 % of
+    the 65536 pixels; z mean within +-0.03, z std in [0.9, 1.15]; image
+    means within 3 sigma of their difference."""
+    fov = 2 * math.degrees(math.atan(math.tan(math.radians(30.0)) / 4))
+    d64 = A.map_preset(1, width=256, height=256, spp=16, fov_y=fov, solver=F.SOLVER_PLAIN_FP64)
+    d32 = A.map_preset(1, width=256, height=256, spp=16, fov_y=fov, solver=F.SOLVER_WAVEFRONT_FP32)
+    p64 = ctx1.draw_map(d64)
+    p32 = ctx1.draw_map(d32)
+    assert np.all(p32["time_n"] == 16) and np.all(p64["time_n"] == 16)
+    z, m = zscores((p32["radiance_E"], p32["radiance_SE"]), (p64["radiance_E"], p64["radiance_SE"]))
+    frac = ((np.abs(z) <= 3) | ~m).mean()
+    print(f"FP32 vs FP64 window: {frac:.5f} within 3 sigma, z mean {z[m].mean():+.4f}, "
+          f"z std {z[m].std():.4f}, {m.mean():.4f} of pixels with a non-zero SE")
+    assert frac >= 0.997
+    assert abs(z[m].mean()) < 0.03
+    assert 0.9 < z[m].std() < 1.15
+    d = p32["radiance_E"].mean() - p64["radiance_E"].mean()
+    sd = np.sqrt((p32["radiance_SE"] ** 2).sum() + (p64["radiance_SE"] ** 2).sum()) / p32.size
+    assert abs(d) <= 3 * sd

@@ -13,6 +13,7 @@ import math
 import numpy as np
 import pytest
 
+import closed_forms as CF
 import oracle_ffi as O
 from htrdr_kmin_amd import _ffi as F
 from scenes import (Scene, camera, cells_along, planck, rectangle, sun_dir, sun_solid_angle,
@@ -229,3 +230,64 @@ def test_flux_map_lw_thick_isothermal():
     E, SE = accum_estimate(acc)
     assert np.allclose(E, math.pi * planck(LW_WLEN * 1e-9, T) / 1e9, rtol=1e-12, atol=0)
     assert np.all(SE <= 1e-12 * E)
+
+
+def walk_many(sky, pos, d, n, wlen=550.0, spectral=F.SPECTRAL_SW, mask=F.RADIANCE_ALL):
+    pos = np.asarray(pos, np.float64)
+    d = np.asarray(d, np.float64)
+    d = d / np.linalg.norm(d)
+    out = np.zeros(n)
+    O.lib().oracle_radiance_many(sky.h, 0, 0, 0, n, spectral, mask, O.dptr(pos), O.dptr(d), wlen, 0, 0,
+                                 O.dptr(out), 0)
+    return out
+
+
+@pytest.mark.parametrize("view", sorted(CF.THIN_VIEWS))
+@pytest.mark.parametrize("medium", sorted(CF.THIN_MEDIA))
+def test_single_scattering_thin_slab(medium, view):
+    """SW volume scattering (..._sw.c:422-466) in an optically thin slab:
+    the mean radiance is the first-order closed form L_sun Omega
+    sum_c ks_c <p_c(dir . sun)> * path integral, for HG particles, Rayleigh
+    gas and a 3:1 gas/particle mix, 20 and 150 degrees from the sun. The
+    same data reject what a flipped scattering-angle convention (100x off
+    for HG) or swapped gas/particle odds would give by many standard errors.
+    Tolerance: 4 standard errors + 0.5 % for second-order scattering."""
+    ks_gas, ks_par = CF.THIN_MEDIA[medium]
+    sky = O.Sky(CF.thin_slab_scene(ks_gas, ks_par).ptr)
+    pos, d, _ = CF.THIN_VIEWS[view]
+    w = walk_many(sky, pos, d, 4_000_000)
+    m, se = mean_se(w)
+    expect, alternatives = CF.thin_slab_expectations(L_SUN_550, ks_gas, ks_par, view)
+    tol = 4 * se + CF.THIN_MS_RTOL * expect
+    assert abs(m - expect) <= tol, (m, expect, se)
+    assert se < 0.03 * expect
+    for name, alt in alternatives.items():
+        if abs(alt - expect) > 0.2 * expect:  # Rayleigh alone is even in mu
+            assert abs(m - alt) > 5 * tol, name
+    # a walk scores at most once here (single scattering dominates): every
+    # non-zero weight is one next-event term L_sun Omega p Tr_abs Tr
+    assert np.mean(w > 0) < 3e-3
+
+
+@pytest.mark.parametrize("view", sorted(CF.LW_VIEWS))
+@pytest.mark.parametrize("medium", ["gas", "cloud"])
+def test_lw_emission_under_a_temperature_gradient(medium, view):
+    """LW absorption emits B_lambda(T) at the temperature of the layer
+    holding the absorption point (..._lw.c:221-227): ten layers 9 K apart,
+    absorbing gas or an absorbing cloud across six of them, seen up and
+    down, from the ground, the top and mid-layer, over a black ground at
+    305 K. Expected: sum_i B(T_i)(1 - exp(-tau_i/mu)) times the
+    transmissivity of the layers before it. The same data reject a layer
+    index off by one either way and the starting layer's temperature.
+    Tolerance: 4 standard errors."""
+    sky = O.Sky(CF.lw_layers_scene(medium).ptr)
+    pos, d, _ = CF.LW_VIEWS[view]
+    w = walk_many(sky, pos, d, 200_000, wlen=CF.LW_WLEN_NM, spectral=F.SPECTRAL_LW)
+    m, se = mean_se(w)
+    expect = CF.lw_layers_expectations(medium, view)
+    assert abs(m - expect) <= 4 * se, (m, expect, se)
+    for name, alt in CF.lw_layers_alternatives(medium, view).items():
+        assert abs(m - alt) > 10 * se, name
+    # each realisation is one emission: B of a layer's or the ground's temperature, or 0
+    Bs = np.r_[planck(CF.LW_WLEN_NM * 1e-9, CF.LW_T), planck(CF.LW_WLEN_NM * 1e-9, CF.LW_TG), 0.0]
+    assert np.all(np.min(np.abs(w[:, None] - Bs[None, :]) / Bs.max(), axis=1) < 1e-13)
