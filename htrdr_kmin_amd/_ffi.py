@@ -126,7 +126,8 @@ class Segment(C.Structure):
 class Counters(C.Structure):
     _fields_ = [("nrealisations", C.c_uint64), ("nleaf", C.c_uint64), ("ncoll", C.c_uint64),
                 ("nbvh", C.c_uint64), ("ntri", C.c_uint64), ("nevents", C.c_uint64),
-                ("nseg", C.c_uint64), ("nlwc", C.c_uint64), ("diag", C.c_uint64 * 16)]
+                ("nseg", C.c_uint64), ("nlwc", C.c_uint64), ("diag", C.c_uint64 * 16),
+                ("ntrunc_coll", C.c_uint64), ("ntrunc_iter", C.c_uint64)]
 
 
 class SynthParams(C.Structure):
@@ -159,6 +160,9 @@ SIGNATURES = {
     "htgx_set_counting": (C.c_int, [C.c_void_p, C.c_int]),
     "htgx_get_counters": (C.c_int, [C.c_void_p, C.POINTER(Counters)]),
     "htgx_debug_trace_segments": (C.c_int, [C.c_void_p, c_double_p, C.c_uint32, C.c_int,
+                                            C.c_uint32, C.c_uint32, C.POINTER(Segment),
+                                            C.c_uint32, c_uint32_p]),
+    "htgx_debug_trace_segments_fp32": (C.c_int, [C.c_void_p, c_double_p, C.c_uint32, C.c_int,
                                             C.c_uint32, C.c_uint32, C.POINTER(Segment),
                                             C.c_uint32, c_uint32_p]),
     "htgx_debug_realisations": (C.c_int, [C.c_void_p, C.POINTER(MapDesc), c_uint64_p,
@@ -198,6 +202,9 @@ SIGNATURES = {
     "htgx_dump_octrees_vtk": (C.c_int, [C.POINTER(SceneDesc), C.c_int]),
     "htgx_debug_octree": (C.c_int, [C.POINTER(SceneDesc), C.c_int, c_uint32_p, c_float_p, c_uint64_p,
                                     c_uint64_p, c_uint64_p, c_double_p]),
+    "htgx_octree_cache_prepare": (C.c_int, [C.POINTER(SceneDesc), C.c_char_p, C.POINTER(C.c_int),
+                                            C.c_char_p, C.c_size_t]),
+    "htgx_leaf_block_for_definition": (C.c_int, [c_uint32_p, c_uint32_p, c_uint32_p]),
 }
 
 _lib = None

@@ -62,6 +62,30 @@ def dump_octrees_vtk(scene_ptr, out):
         F.check("htgx_dump_octrees_vtk", lib.htgx_dump_octrees_vtk(scene_ptr, f.fileno()))
 
 
+def octree_cache_prepare(scene_ptr, path):
+    """Option -O's cache prepared on the host (htgx_octree_cache_prepare):
+    checks the cache at ``path`` against the scene and returns True, or
+    builds the octree and writes it there and returns False; HtgxError
+    with the message for a cache that is refused."""
+    lib = F.load()
+    loaded, err = C.c_int(0), C.create_string_buffer(512)
+    res = lib.htgx_octree_cache_prepare(scene_ptr, os.fsencode(path), C.byref(loaded), err, len(err))
+    if res != F.OK:
+        raise F.HtgxError(f"htgx_octree_cache_prepare ({err.value.decode()})", res)
+    return bool(loaded.value)
+
+
+def leaf_block_for_definition(definition, max_definition):
+    """Leaf block for option -V (htgx_leaf_block_for_definition)"""
+    lib = F.load()
+    d = np.ascontiguousarray(definition, np.uint32)
+    m = np.ascontiguousarray(max_definition, np.uint32)
+    B = C.c_uint32(0)
+    F.check("htgx_leaf_block_for_definition", lib.htgx_leaf_block_for_definition(
+        d.ctypes.data_as(F.c_uint32_p), m.ctypes.data_as(F.c_uint32_p), C.byref(B)))
+    return B.value
+
+
 class GroundFiles:
     """The ground of option -g (an htrdr-obj file) with the materials of
     option -M (an htrdr-materials file and its mrumtl files) and the sky
@@ -239,12 +263,16 @@ class Context:
         out["diag"] = list(c.diag)
         return out
 
-    def trace_segments(self, rays, prop, iband=0, iquad=0, max_segs=256):
+    def trace_segments(self, rays, prop, iband=0, iquad=0, max_segs=256, fp32=False):
+        """Segments of rays [org, dir, range] as one solver's iterator sees
+        them: the FP64 solver's (htgx_debug_trace_segments) or, with fp32,
+        the FP32 wavefront solver's (htgx_debug_trace_segments_fp32)"""
         rays = np.ascontiguousarray(rays, np.float64).reshape(-1, 8)
         n = rays.shape[0]
         segs = (F.Segment * (n * max_segs))()
         nsegs = np.zeros(n, np.uint32)
-        F.check("htgx_debug_trace_segments", self._lib.htgx_debug_trace_segments(
+        name = "htgx_debug_trace_segments_fp32" if fp32 else "htgx_debug_trace_segments"
+        F.check(name, getattr(self._lib, name)(
             self._h, rays.ctypes.data_as(F.c_double_p), n, prop, iband, iquad, segs, max_segs,
             nsegs.ctypes.data_as(F.c_uint32_p)))
         arr = np.ctypeslib.as_array(segs).reshape(n, max_segs)

@@ -22,8 +22,12 @@
  * -O reads the octree from the cache file, or writes it there when the file
  * does not exist (htgx_ctx_create_cached); a cache built from other inputs
  * aborts the run.
+ * -V x,y,z bounds the definition of the octree's finest level: the leaf
+ * block becomes the smallest cube of cloud cells that divides the cloud grid
+ * and keeps at most x,y,z blocks along each axis
+ * (htgx_leaf_block_for_definition); ignored when a cache is read (-O).
  * Options without a counterpart on the GPU path are rejected with an error
- * (-C focal-length) or accepted and ignored (-t, -V).
+ * (-C focal-length) or accepted and ignored (-t).
  * The solver is the FP32 wavefront one; HTGX_SOLVER=plain selects the FP64
  * plain one.
  */
@@ -61,6 +65,7 @@ struct args {
   double wlen[2], tref;
   double sun_az, sun_el;
   double thickness;
+  unsigned vdef[3];  /* -V: maximum definition of the octree (0: the clouds') */
   const char* cache; /* -O: octree cache file */
 };
 
@@ -94,6 +99,35 @@ parse_doubles(const char* s, double* out, int n)
     }
   }
   return *s ? -1 : 0;
+}
+
+/* -V x,y,z (parse_grid_definition, src/atmosphere/htrdr_atmosphere_args.c:48-78) */
+static int
+parse_grid_definition(const char* s, unsigned def[3])
+{
+  char* end;
+  int i;
+  for(i = 0; i < 3; ++i) {
+    unsigned long v;
+    errno = 0;
+    v = strtoul(s, &end, 10);
+    if(end == s || errno || v > 0xFFFFFFFFul || *s == '-') {
+      fprintf(stderr, "Invalid grid definition `%s'.\n", s);
+      return -1;
+    }
+    def[i] = (unsigned)v;
+    s = end;
+    if(i < 2) {
+      if(*s != ',') { fprintf(stderr, "Invalid grid definition `%s'.\n", s); return -1; }
+      ++s;
+    }
+  }
+  if(*s) { fprintf(stderr, "Invalid grid definition `%s'.\n", s); return -1; }
+  if(!def[0] || !def[1] || !def[2]) {
+    fprintf(stderr, "Invalid null grid definition {%u, %u, %u}.\n", def[0], def[1], def[2]);
+    return -1;
+  }
+  return 0;
 }
 
 /* "key=value:key=value..." with the handler of one parameter */
@@ -255,7 +289,7 @@ parse_args(int argc, char** argv, struct args* a)
       case 's': bad = parse_list(optarg, spectral_param, a); break;
       case 'T': bad = parse_doubles(optarg, &a->thickness, 1) || a->thickness < 0; break;
       case 't': break; /* thread hint: the GPU path has no host threads */
-      case 'V': break; /* grid definition hint */
+      case 'V': bad = parse_grid_definition(optarg, a->vdef); break;
       case 'v': a->verbose = 1; break;
       default: usage(); return HTGX_BAD_ARG;
     }
@@ -462,6 +496,18 @@ main(int argc, char** argv)
   if(ground) scene.ground = *htgx_ground_files_get_desc(ground);
   scene.repeat_clouds = a.repeat_clouds;
   scene.repeat_ground = a.repeat_ground;
+  if(a.vdef[0] && scene.clouds.def[0]) { /* -V: the octree's finest definition */
+    uint32_t B = 0;
+    const uint32_t vdef[3] = {a.vdef[0], a.vdef[1], a.vdef[2]};
+    if(htgx_leaf_block_for_definition(scene.clouds.def, vdef, &B) != HTGX_OK) {
+      fprintf(stderr, "%s: no block of cells divides the cloud grid %ux%ux%u into at most "
+              "%u,%u,%u blocks -- option '-V'\n", argv[0], scene.clouds.def[0], scene.clouds.def[1],
+              scene.clouds.def[2], vdef[0], vdef[1], vdef[2]);
+      res = HTGX_BAD_ARG;
+      goto error;
+    }
+    scene.leaf_block = B;
+  }
   if(a.dump) { /* dump_volumetric_acceleration_structure, ..._atmosphere.c:228-255 */
     fflush(out);
     res = htgx_dump_octrees_vtk(&scene, fileno(out));

@@ -411,6 +411,60 @@ check_scene(const htgx_scene_desc* s)
 }
 
 extern "C" int
+htgx_octree_cache_prepare(const htgx_scene_desc* scene, const char* path, int* loaded, char* err,
+                          size_t errlen)
+{
+  if(loaded) *loaded = 0;
+  if(err && errlen) err[0] = '\0';
+  if(!path) return HTGX_BAD_ARG;
+  int res;
+  if((res = check_scene(scene)) != HTGX_OK) return res;
+  const htgx_cloud_desc& cl = scene->clouds;
+  if(!(cl.def[0] && cl.def[1] && cl.def[2])) return HTGX_OK; /* no octree, no cache */
+  std::string msg;
+  const uint64_t digest = svo_cache_digest(*scene);
+  SvoCache cache;
+  bool found = false;
+  res = svo_cache_read(path, *scene, digest, cache, &found, msg);
+  if(res == HTGX_OK && !found) {
+    SvoHost svo;
+    if((res = svo_build(*scene, svo)) == HTGX_OK) {
+      cache.leaf_block = svo.B;
+      cache.nlevels = svo.nlevels;
+      cache.optical_thickness = scene->optical_thickness;
+      cache.nleaves = svo.nleaves;
+      cache.lmap = std::move(svo.lmap);
+      cache.bounds = std::move(svo.bounds);
+      res = svo_cache_write(path, digest, cache, msg);
+    } else {
+      msg = "the octree cannot be built for this cloud grid and leaf block";
+    }
+  }
+  if(loaded) *loaded = found ? 1 : 0;
+  if(res != HTGX_OK && err && errlen) {
+    std::strncpy(err, msg.c_str(), errlen - 1);
+    err[errlen - 1] = '\0';
+  }
+  return res;
+}
+
+extern "C" int
+htgx_leaf_block_for_definition(const uint32_t def[3], const uint32_t max_def[3],
+                               uint32_t* leaf_block)
+{
+  if(!def || !max_def || !leaf_block) return HTGX_BAD_ARG;
+  uint32_t B = 1, dmin = UINT32_MAX;
+  for(int a = 0; a < 3; ++a) {
+    if(!def[a] || !max_def[a]) return HTGX_BAD_ARG;
+    B = std::max(B, (def[a] + max_def[a] - 1) / max_def[a]);
+    dmin = std::min(dmin, def[a]);
+  }
+  for(; B <= dmin; ++B)
+    if(def[0] % B == 0 && def[1] % B == 0 && def[2] % B == 0) { *leaf_block = B; return HTGX_OK; }
+  return HTGX_BAD_ARG;
+}
+
+extern "C" int
 htgx_debug_octree(const htgx_scene_desc* scene, int on_device, uint32_t* lmap, float* bounds,
                   uint64_t* nblocks, uint64_t* nnodes, uint64_t* nleaves, double* build_ms)
 {
@@ -642,6 +696,7 @@ htgx_ctx_init(htgx_ctx* ctx, const htgx_scene_desc& sc)
   }
   ctx->band_lo.assign(sc.spectral.band_lo, sc.spectral.band_lo + nb);
   ctx->band_hi.assign(sc.spectral.band_hi, sc.spectral.band_hi + nb);
+  ctx->nquads.assign(sc.spectral.nquads, sc.spectral.nquads + nb);
   S.nbands = nb;
   if((res = upload(ctx, sc.spectral.band_hi, nb, &S.band_hi))) return res;
   if((res = upload(ctx, qoff.data(), qoff.size(), &S.quad_off))) return res;
@@ -672,7 +727,7 @@ htgx_ctx_init(htgx_ctx* ctx, const htgx_scene_desc& sc)
     if(ctx->cache_path) {
       digest = svo_cache_digest(sc);
       bool found = false;
-      if((res = svo_cache_read(ctx->cache_path, digest, cache, &found, ctx->cache_err)))
+      if((res = svo_cache_read(ctx->cache_path, sc, digest, cache, &found, ctx->cache_err)))
         return res;
       if(found) { /* the cache's own threshold and leaf block (-T, -V ignored) */
         lsc.leaf_block = cache.leaf_block;
@@ -680,7 +735,12 @@ htgx_ctx_init(htgx_ctx* ctx, const htgx_scene_desc& sc)
         ctx->cache_loaded = true;
       }
     }
-    if((res = svo_layout(lsc, svo)) != HTGX_OK) return res;
+    if((res = svo_layout(lsc, svo)) != HTGX_OK) {
+      if(ctx->cache_loaded)
+        ctx->cache_err = std::string("the leaf block of the cache `") + ctx->cache_path
+                       + "' does not fit the cloud grid";
+      return res;
+    }
     S.has_clouds = 1;
     S.B = svo.B;
     for(int a = 0; a < 3; ++a) {
@@ -1260,6 +1320,11 @@ htgx_get_counters(htgx_ctx* ctx, htgx_counters* c)
   c->nseg = h[2];
   c->nlwc = h[4];
   for(int i = 0; i < 16; ++i) c->diag[i] = h[8 + i];
+  uint64_t tr[2];
+  int res;
+  if((res = htgx_wavefront_truncations(ctx, tr)) != HTGX_OK) return res;
+  c->ntrunc_coll = tr[0];
+  c->ntrunc_iter = tr[1];
   return HTGX_OK;
 }
 
@@ -1435,6 +1500,7 @@ htgx_debug_trace_segments(htgx_ctx* ctx, const double* rays, uint32_t nrays, int
 {
   if(!ctx || !rays || !segs || !nsegs || !nrays || !max_segs) return HTGX_BAD_ARG;
   if(prop < HTGX_Ka || prop > HTGX_Kext || iband >= ctx->S.nbands) return HTGX_BAD_ARG;
+  if(iquad >= ctx->nquads[iband]) return HTGX_BAD_ARG;
   DevArray<double> d_rays;
   DevArray<htgx_segment> d_segs;
   DevArray<uint32_t> d_n;
@@ -1447,6 +1513,18 @@ htgx_debug_trace_segments(htgx_ctx* ctx, const double* rays, uint32_t nrays, int
   HIP_OK(hipGetLastError());
   if((res = d_segs.download(segs, (size_t)nrays * max_segs))) return res;
   return d_n.download(nsegs, nrays);
+}
+
+extern "C" int
+htgx_debug_trace_segments_fp32(htgx_ctx* ctx, const double* rays, uint32_t nrays, int prop,
+                               uint32_t iband, uint32_t iquad, htgx_segment* segs,
+                               uint32_t max_segs, uint32_t* nsegs)
+{
+  if(!ctx || !rays || !segs || !nsegs || !nrays || !max_segs) return HTGX_BAD_ARG;
+  if(prop < HTGX_Ka || prop > HTGX_Kext || iband >= ctx->S.nbands) return HTGX_BAD_ARG;
+  if(iquad >= ctx->nquads[iband]) return HTGX_BAD_ARG;
+  return htgx_debug_trace_segments_fp32_impl(ctx, rays, nrays, prop, iband, iquad, segs, max_segs,
+                                             nsegs);
 }
 
 extern "C" int

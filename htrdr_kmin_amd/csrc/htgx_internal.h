@@ -20,6 +20,7 @@ struct htgx_ctx {
   htgx::DevScene S;
   std::vector<void*> bufs;          /* scene buffers owned by the context */
   std::vector<double> band_lo, band_hi;
+  std::vector<uint32_t> nquads;     /* k-points of each band */
   bool has_temperature = false;
   uint32_t nlevels = 0;
   uint64_t nnodes = 0, nleaves = 0;
@@ -85,6 +86,13 @@ void wf_plan_free(htgx_batch_plan* p);
 int htgx_debug_realisations_fp32(htgx_ctx* ctx, const htgx_map_desc* desc,
                                  const uint64_t* pix, const uint64_t* real, uint32_t n,
                                  double* weights);
+int htgx_debug_trace_segments_fp32_impl(htgx_ctx* ctx, const double* rays, uint32_t nrays, int prop,
+                                        uint32_t iband, uint32_t iquad, htgx_segment* segs,
+                                        uint32_t max_segs, uint32_t* nsegs);
+/* Queries of the FP32 solver cut short since the context was created:
+ * [0] by the tentative-collision budget, [1] by the segment iterator's bound
+ * on passes without a segment */
+int htgx_wavefront_truncations(htgx_ctx* ctx, uint64_t out[2]);
 void htgx_wavefront_release(htgx_ctx* ctx);
 
 #endif /* HTGX_INTERNAL_H */

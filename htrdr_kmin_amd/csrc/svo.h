@@ -75,9 +75,12 @@ struct SvoCache {
 /* Digest of the inputs a cache depends on (bands, gas, clouds, cloud optics) */
 uint64_t svo_cache_digest(const htgx_scene_desc& scene);
 /* *found = false and HTGX_OK if the file does not exist; HTGX_BAD_ARG (err
- * says why) if it is not a cache, was built from other inputs or is
- * truncated; HTGX_IO_ERR if it cannot be opened */
-int svo_cache_read(const char* path, uint64_t digest, SvoCache& out, bool* found, std::string& err);
+ * says why) if it is not a cache, was built from other inputs, is truncated,
+ * does not fit the scene's cloud grid or holds an invalid tree (leaf words,
+ * tiling, bounds checked against the scene's LWC); HTGX_IO_ERR if it cannot
+ * be opened */
+int svo_cache_read(const char* path, const htgx_scene_desc& scene, uint64_t digest, SvoCache& out,
+                   bool* found, std::string& err);
 int svo_cache_write(const char* path, uint64_t digest, const SvoCache& cache, std::string& err);
 
 } /* namespace htgx */

@@ -4,10 +4,15 @@
  * instead of building. htsky's cache layout is not part of the reference, so
  * the layout is this library's own:
  *
- *   header (56 bytes): magic "HTGXOCT1", digest of the inputs (u64), leaf
- *   block edge and level count (u32 each), merge threshold -T (f64), blocks,
- *   nodes and leaves (u64 each); then the level map (u32 per finest block)
- *   and the node bounds (2 x f32 per node), native byte order.
+ *   header (64 bytes): magic "HTGXOCT2", digest of the inputs (u64),
+ *   digest of the payload (u64), leaf block edge and level count (u32 each),
+ *   merge threshold -T (f64), blocks, nodes and leaves (u64 each); then the
+ *   level map (u32 per finest block) and the node bounds (2 x f32 per node),
+ *   native byte order.
+ *
+ * A cache is accepted only if the payload digest matches (a damaged file)
+ * and the tree it holds is valid for the scene (a hand-made one): see
+ * cache_payload_ok.
  *
  * The digest covers what the reference's cache depends on, the contents of
  * the atmosphere (-a), clouds (-c) and Mie (-m) inputs: the spectral bands
@@ -17,15 +22,18 @@
  * threshold and the leaf block are not in the digest: a cache in use
  * imposes its own, as the reference then ignores -T and -V. */
 #include "svo.h"
+#include "htgx_device.h"
 
+#include <algorithm>
 #include <cerrno>
+#include <cfloat>
 #include <cstdio>
 #include <cstring>
 
 namespace htgx {
 namespace {
 
-constexpr char kMagic[8] = {'H', 'T', 'G', 'X', 'O', 'C', 'T', '1'};
+constexpr char kMagic[8] = {'H', 'T', 'G', 'X', 'O', 'C', 'T', '2'};
 
 /* 64-bit word-wise multiply-xorshift hash; detects any change to the inputs
  * with overwhelming probability (not a cryptographic digest) */
@@ -61,11 +69,21 @@ struct Digest {
 struct Header {
   char magic[8];
   uint64_t digest;
+  uint64_t payload_digest;
   uint32_t leaf_block, nlevels;
   double optical_thickness;
   uint64_t nblocks, nnodes, nleaves;
 };
-static_assert(sizeof(Header) == 56, "cache header layout");
+static_assert(sizeof(Header) == 64, "cache header layout");
+
+uint64_t
+payload_digest(const SvoCache& c)
+{
+  Digest d;
+  d.bytes(c.lmap.data(), c.lmap.size() * sizeof(uint32_t));
+  d.bytes(c.bounds.data(), c.bounds.size() * sizeof(float2));
+  return d.h;
+}
 
 } /* namespace */
 
@@ -100,8 +118,72 @@ svo_cache_digest(const htgx_scene_desc& sc)
   return d.h;
 }
 
+/* The payload of a cache against the layout of its own leaf block and the
+ * scene's LWC grid: every leaf word names an existing node that holds the
+ * block (its level's node over the block's coordinates), the leaves tile the
+ * blocks exactly once, every node bound is finite with 0 <= min <= max, each
+ * leaf's bounds contain the LWC of all its cells and the root's contain the
+ * whole grid's (the FP32 leaf records are scaled by the root's maximum), and
+ * the leaf count is the header's. A cache that passes gives valid majorants
+ * everywhere, so the estimators stay unbiased; one that does not is refused
+ * before anything reaches the device. */
+static bool
+cache_payload_ok(const SvoHost& lay, const htgx_cloud_desc& cl, const SvoCache& c, std::string& why)
+{
+  for(uint64_t i = 0; i < lay.nnodes; ++i) {
+    const float2 b = c.bounds[i];
+    if(!(b.x >= 0.f && b.x <= b.y && b.y <= FLT_MAX)) { why = "a node bound is invalid"; return false; }
+  }
+  const uint32_t* d0 = lay.dims[0];
+  const uint64_t nblocks = (uint64_t)d0[0] * d0[1] * d0[2];
+  std::vector<uint32_t> owner(nblocks, ~0u);
+  uint64_t nleaves = 0;
+  for(uint64_t i = 0; i < nblocks; ++i) {
+    const uint32_t x = (uint32_t)(i % d0[0]), y = (uint32_t)((i / d0[0]) % d0[1]);
+    const uint32_t z = (uint32_t)(i / ((uint64_t)d0[0] * d0[1]));
+    const uint32_t w = c.lmap[i], L = w >> kLeafIdBits;
+    if(L >= lay.nlevels) { why = "a leaf level is out of range"; return false; }
+    const uint32_t* dl = lay.dims[L];
+    const uint64_t id = lay.offset[L] + ((uint64_t)(z >> L) * dl[1] + (y >> L)) * dl[0] + (x >> L);
+    if((w & kLeafIdMask) != id) { why = "a leaf word does not name the node over its block"; return false; }
+    if(((x | y | z) & ((1u << L) - 1u)) != 0) continue; /* not the leaf's first block */
+    /* the leaf's first block paints the leaf's blocks: each exactly once */
+    ++nleaves;
+    const uint32_t hi[3] = {std::min(x + (1u << L), d0[0]), std::min(y + (1u << L), d0[1]),
+                            std::min(z + (1u << L), d0[2])};
+    for(uint32_t k = z; k < hi[2]; ++k)
+    for(uint32_t j = y; j < hi[1]; ++j)
+    for(uint32_t l = x; l < hi[0]; ++l) {
+      uint32_t& o = owner[((uint64_t)k * d0[1] + j) * d0[0] + l];
+      if(o != ~0u) { why = "two leaves overlap"; return false; }
+      o = w;
+    }
+  }
+  for(uint64_t i = 0; i < nblocks; ++i)
+    if(owner[i] != c.lmap[i]) { why = "the leaves do not tile the grid"; return false; }
+  if(nleaves != c.nleaves) { why = "the leaf count is not the header's"; return false; }
+  const uint32_t B = lay.B;
+  const float2 root = c.bounds[lay.nnodes - 1];
+  for(uint64_t i = 0; i < nblocks; ++i) {
+    const uint32_t x = (uint32_t)(i % d0[0]), y = (uint32_t)((i / d0[0]) % d0[1]);
+    const uint32_t z = (uint32_t)(i / ((uint64_t)d0[0] * d0[1]));
+    const float2 b = c.bounds[c.lmap[i] & kLeafIdMask];
+    for(uint32_t k = z * B; k < (z + 1) * B; ++k)
+    for(uint32_t j = y * B; j < (y + 1) * B; ++j) {
+      const float* row = cl.lwc + ((size_t)k * cl.def[1] + j) * cl.def[0];
+      for(uint32_t l = x * B; l < (x + 1) * B; ++l) {
+        const float v = row[l];
+        if(!(v >= b.x && v <= b.y)) { why = "a leaf's bounds do not hold its cells"; return false; }
+        if(!(v >= root.x && v <= root.y)) { why = "the root's bounds do not hold the grid"; return false; }
+      }
+    }
+  }
+  return true;
+}
+
 int
-svo_cache_read(const char* path, uint64_t digest, SvoCache& out, bool* found, std::string& err)
+svo_cache_read(const char* path, const htgx_scene_desc& scene, uint64_t digest, SvoCache& out,
+               bool* found, std::string& err)
 {
   *found = false;
   FILE* f = std::fopen(path, "rb");
@@ -112,6 +194,9 @@ svo_cache_read(const char* path, uint64_t digest, SvoCache& out, bool* found, st
   }
   int res = HTGX_OK;
   Header h;
+  SvoHost lay;
+  htgx_scene_desc lsc = scene;
+  lsc.leaf_block = 0;
   if(std::fread(&h, sizeof(h), 1, f) != 1 || std::memcmp(h.magic, kMagic, sizeof(kMagic))) {
     err = std::string("`") + path + "' is not an octree cache";
     res = HTGX_BAD_ARG;
@@ -119,8 +204,14 @@ svo_cache_read(const char* path, uint64_t digest, SvoCache& out, bool* found, st
     err = std::string("the cache `") + path
         + "' was built from other atmosphere, cloud or Mie inputs than those submitted";
     res = HTGX_BAD_ARG;
-  } else if(!h.nblocks || !h.nnodes || h.nblocks > h.nnodes || h.nnodes > (1ull << 28)) {
-    err = std::string("the cache `") + path + "' is corrupted";
+  } else if(!h.leaf_block || (lsc.leaf_block = h.leaf_block, svo_layout(lsc, lay)) != HTGX_OK) {
+    err = std::string("the leaf block of the cache `") + path + "' does not fit the cloud grid";
+    res = HTGX_BAD_ARG;
+  } else if(h.nblocks != (uint64_t)lay.dims[0][0] * lay.dims[0][1] * lay.dims[0][2]
+         || h.nnodes != lay.nnodes || h.nlevels != lay.nlevels
+         || !(h.optical_thickness >= 0) || !h.nleaves || h.nleaves > h.nblocks) {
+    /* sizes checked against the grid before anything is allocated */
+    err = std::string("the cache `") + path + "' is corrupted (its sizes do not fit the cloud grid)";
     res = HTGX_BAD_ARG;
   } else {
     out.leaf_block = h.leaf_block;
@@ -130,16 +221,27 @@ svo_cache_read(const char* path, uint64_t digest, SvoCache& out, bool* found, st
     out.lmap.resize(h.nblocks);
     out.bounds.resize(h.nnodes);
     char extra;
+    std::string why;
     if(std::fread(out.lmap.data(), sizeof(uint32_t), h.nblocks, f) != h.nblocks
     || std::fread(out.bounds.data(), sizeof(float2), h.nnodes, f) != h.nnodes
     || std::fread(&extra, 1, 1, f) != 0) {
       err = std::string("the cache `") + path + "' is truncated or has trailing data";
+      res = HTGX_BAD_ARG;
+    } else if(payload_digest(out) != h.payload_digest) {
+      err = std::string("the cache `") + path + "' is corrupted (its content does not match its digest)";
+      res = HTGX_BAD_ARG;
+    } else if(!cache_payload_ok(lay, scene.clouds, out, why)) {
+      err = std::string("the cache `") + path + "' is corrupted (" + why + ")";
       res = HTGX_BAD_ARG;
     } else {
       *found = true;
     }
   }
   std::fclose(f);
+  if(!*found) {
+    out.lmap.clear();
+    out.bounds.clear();
+  }
   return res;
 }
 
@@ -149,6 +251,7 @@ svo_cache_write(const char* path, uint64_t digest, const SvoCache& c, std::strin
   Header h;
   std::memcpy(h.magic, kMagic, sizeof(kMagic));
   h.digest = digest;
+  h.payload_digest = payload_digest(c);
   h.leaf_block = c.leaf_block;
   h.nlevels = c.nlevels;
   h.optical_thickness = c.optical_thickness;

@@ -133,6 +133,13 @@ struct Launch {
   unsigned int* done_count;
   unsigned long long* done_tick;
   uint32_t nvalid;
+  /* queries cut short, counted whether or not counting is on (null: not
+   * counted): [0] by the tentative-collision budget, [1] by the segment
+   * iterator's bound on passes without a segment */
+  unsigned long long* trunc;
+  /* list launches only (null otherwise): nvalid (pixel, realisation) pairs,
+   * one realisation each, whose accumulators go to partials[0..nvalid) */
+  const uint32_t* list;
 };
 
 enum { LS_SEG = 0, LS_COLL = 1, LS_EVENT = 2, LS_DONE = 3 };
@@ -171,6 +178,9 @@ struct Lane {
   float tc, Ts, maj, inv_maj;
   double Tmin;
   int ev;
+  /* two counts of the current query: tentative collisions in the low
+   * kCollBits bits, and above them the passes of the segment iterator that
+   * emitted nothing since its last segment */
   uint32_t nsteps;
   /* segment iterator */
   int cstate;          /* 0 before grid, 1 in leaf, 2 outside, 3 locate at tit, 4 step out of leaf */
@@ -208,10 +218,19 @@ struct LaneCold {
   float tsum[kBlock], tsum2[kBlock];
   uint32_t n[kBlock], nfail[kBlock], chunk[kBlock], r_end[kBlock];
   float shift[2][kBlock]; /* -r: world X/Y of the origin of the lane's frame */
+  uint32_t ntiles[kBlock]; /* -r: copies of the grid the query has crossed */
 };
 __shared__ LaneCold g_cold;
 /* Cap on the copies of a repeated cloud grid one query crosses */
 constexpr uint32_t kMaxTilesF = 1u << 16;
+/* Lane::nsteps: tentative collisions of the query (at most the budget
+ * Launch::max_steps = 2^20, then one more) in the low bits; passes of the
+ * segment iterator without a segment above them, bounded by kMaxEmptyPasses
+ * (a pass either emits a segment or crosses a boundary, so the bound only
+ * stops an iterator that makes no progress) */
+constexpr uint32_t kCollBits = 21;
+constexpr uint32_t kCollMask = (1u << kCollBits) - 1u;
+constexpr uint32_t kMaxEmptyPasses = 2000;
 #define LC(f) (g_cold.f[threadIdx.x])
 #define LCA(f, a) (g_cold.f[a][threadIdx.x])
 
@@ -452,7 +471,7 @@ __device__ __forceinline__ bool next_cell(const SceneF& S, Lane& L, uint32_t c[3
       if(out) {
         /* -r: into the next copy of the grid along X or Y; the lane's frame
          * moves with it (a bounded number of copies per query) */
-        if(a == 2 || !S.repeat || ++L.nsteps > kMaxTilesF) return false;
+        if(a == 2 || !S.repeat || ++LC(ntiles) > kMaxTilesF) return false;
         const float dp = L.dir[a] > 0.f ? S.period[a] : -S.period[a];
         L.pos[a] -= dp;
         LCA(shift, a) += dp;
@@ -479,6 +498,7 @@ __device__ __forceinline__ void iter_init(const SceneF& S, Lane& L)
   L.adv = false;
   L.cstate = 2;
   L.nsteps = 0;
+  if(S.repeat) LC(ntiles) = 0;
 #pragma unroll
   for(int a = 0; a < 3; ++a) L.inv_dir[a] = 1.f / L.dir[a];
   if(S.has_clouds) {
@@ -588,11 +608,13 @@ __device__ __forceinline__ bool iter_next(const SceneF& S, Lane& L, int prop, fl
       L.adv_g = tg == t1;
       L.adv_c = tc == t1;
       L.adv_end = at_end;
+      L.nsteps &= kCollMask;
       if(COUNT) L.c_seg++;
       return true;
     }
     iter_advance(L, t1, tg == t1, tc == t1, at_end);
-    if(++L.nsteps > 4096u) { L.itdone = true; return false; } /* safety bound */
+    if((L.nsteps >> kCollBits) >= kMaxEmptyPasses) { L.itdone = true; return false; }
+    L.nsteps += 1u << kCollBits;
   }
 }
 
@@ -1022,7 +1044,7 @@ __device__ __forceinline__ uint64_t wave_sum64(uint64_t v)
 enum { kFeatRare = 1, kFeatMesh = 2 };
 
 /* The persistent loop of one wavefront over the chunks of one map */
-template<int SENSOR, int SPECTRAL, bool COUNT, int FEAT>
+template<int SENSOR, int SPECTRAL, bool COUNT, int FEAT, bool LIST = false>
 __device__ __forceinline__ void run_scene(const SceneF& S0, const DevMap& M0, const Launch& P)
 {
   constexpr bool RARE = (FEAT & kFeatRare) != 0;
@@ -1097,6 +1119,7 @@ __device__ __forceinline__ void run_scene(const SceneF& S0, const DevMap& M0, co
         L.nq_ground = 2;
         L.lstate = LS_EVENT;
       } else {
+        if(P.trunc && (L.nsteps >> kCollBits) >= kMaxEmptyPasses) atomicAdd(P.trunc + 1, 1ull);
         L.ev = EV_NONE;
         L.pending = true;
         L.lstate = LS_EVENT;
@@ -1138,8 +1161,10 @@ __device__ __forceinline__ void run_scene(const SceneF& S0, const DevMap& M0, co
           LC(ev_layer) = L.s_layer;
           L.pending = true;
           L.lstate = LS_EVENT;
-        } else if(++L.nsteps > P.max_steps) {
-          /* micro-step budget of a query exhausted: end it without event */
+        } else if((++L.nsteps & kCollMask) > P.max_steps) {
+          /* tentative-collision budget of a query exhausted: end it without
+           * event, and count it */
+          if(P.trunc) atomicAdd(P.trunc, 1ull);
           L.pending = true;
           L.lstate = LS_EVENT;
         } else {
@@ -1158,7 +1183,22 @@ __device__ __forceinline__ void run_scene(const SceneF& S0, const DevMap& M0, co
     if(want && (__popcll(want) >= P.ev_thresh || !other)) {
       if(COUNT) { d_ee++; d_el += __popcll(want); }
       if(L.lstate == LS_EVENT) {
-        if(LC(stage) == ST_FETCH) {
+        if(LIST && LC(stage) == ST_FETCH) {
+          /* the next (pixel, realisation) pair of the list, one realisation */
+          const uint32_t c = atomicAdd(P.counters, 1u);
+          if(c >= P.nvalid) {
+            L.lstate = LS_DONE;
+          } else {
+            LC(chunk) = c;
+            L.pixel = P.list[2 * c];
+            L.r = P.list[2 * c + 1];
+            LC(r_end) = L.r + 1u;
+            LC(n) = LC(nfail) = 0;
+            LC(sum) = LC(sum2) = 0.0;
+            LC(tsum) = LC(tsum2) = 0.f;
+            LC(stage) = ST_BEGIN;
+          }
+        } else if(LC(stage) == ST_FETCH) {
           uint32_t c;
           if(!fetch_chunk(P, xcd, &c)) {
             L.lstate = LS_DONE;
@@ -1217,7 +1257,7 @@ __device__ __forceinline__ void run_scene(const SceneF& S0, const DevMap& M0, co
             LC(n)++;
           }
           L.r++;
-          if(L.r < LC(r_end)) {
+          if(L.r != LC(r_end)) {
             LC(stage) = ST_BEGIN;
           } else {
             htgx_mc_accum a;
@@ -1307,6 +1347,59 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 wavefront_kernel(SceneF S, DevMap M, Launch P)
 {
   run_scene<SENSOR, SPECTRAL, COUNT, FEAT>(S, M, P);
+}
+
+/* Given (pixel, realisation) pairs of one map, one realisation each (the
+ * debug entry of single FP32 realisations): the loop of the production
+ * kernels with the rare and mesh features compiled in, which give the same
+ * arithmetic as the leaner instances */
+template<int SENSOR, int SPECTRAL>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+wavefront_list_kernel(SceneF S, DevMap M, Launch P)
+{
+  run_scene<SENSOR, SPECTRAL, false, kFeatRare | kFeatMesh, true>(S, M, P);
+}
+
+/* The FP32 segment iterator on given rays over [0, range[1]] (a debug
+ * entry): the segments and their property bounds as the production kernels
+ * see them, with the leaf id read back from the leaf record */
+__global__ void __launch_bounds__(256)
+wf_trace_segments_kernel(SceneF S, const double* rays, uint32_t nrays, int prop, uint32_t iband,
+                         uint32_t iquad, htgx_segment* segs, uint32_t max_segs, uint32_t* nsegs)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= nrays) return;
+  const double* r = rays + (size_t)i * 8;
+  Lane L;
+#pragma unroll
+  for(int a = 0; a < 3; ++a) { L.pos[a] = (float)r[a]; L.dir[a] = (float)r[3 + a]; }
+  L.qr = (float)r[7];
+  L.gka_off = (S.quad_off[iband] + iquad) * S.nlayers;
+  L.gks_off = iband * S.nlayers;
+  L.cka = S.has_clouds ? S.cka[iband] : 0.f;
+  L.cks = S.has_clouds ? S.cks[iband] : 0.f;
+  LCA(shift, 0) = LCA(shift, 1) = 0.f;
+  iter_init(S, L);
+  uint32_t n = 0;
+  float t0;
+  while(iter_next<false>(S, L, prop, &t0)) {
+    if(n < max_segs) {
+      htgx_segment& o = segs[(size_t)i * max_segs + n];
+      o.t0 = t0;
+      o.t1 = L.s_t1;
+      o.kmin = L.s_kmin;
+      o.kmax = L.s_kmax;
+      o.leaf = -1;
+      if(L.s_cloud) {
+        const uint32_t c[3] = {L.lohi[0] & 0xFFFFu, L.lohi[1] & 0xFFFFu, L.lohi[2] & 0xFFFFu};
+        o.leaf = S.leafrec[(c[2] * S.dims0[1] + c[1]) * S.dims0[0] + c[0]].x & kLeafIdMask;
+      }
+      o.layer = L.s_layer;
+      o.pad = 0;
+    }
+    n++;
+  }
+  nsegs[i] = n;
 }
 
 /* One map of a batched launch */
@@ -1492,6 +1585,7 @@ struct htgx_wavefront_state {
   SceneF S;
   std::vector<void*> bufs;
   unsigned int* d_counters = nullptr;
+  unsigned long long* d_trunc = nullptr; /* [2] queries cut short, Launch::trunc */
   void* d_partials = nullptr;
   size_t partials_cap = 0;
   uint32_t grid = 0;
@@ -1543,6 +1637,7 @@ wavefront_free(htgx_wavefront_state* st)
 {
   for(void* p : st->bufs) (void)hipFree(p);
   if(st->d_counters) (void)hipFree(st->d_counters);
+  if(st->d_trunc) (void)hipFree(st->d_trunc);
   if(st->d_partials) (void)hipFree(st->d_partials);
   delete st;
 }
@@ -1653,6 +1748,8 @@ wavefront_build(htgx_ctx* ctx, htgx_wavefront_state* st)
   S.sun_solid_angle = (float)D.sun_solid_angle;
 
   HIP_OK(hipMalloc(&st->d_counters, kXcds * sizeof(unsigned int)));
+  HIP_OK(hipMalloc(&st->d_trunc, 2 * sizeof(unsigned long long)));
+  HIP_OK(hipMemset(st->d_trunc, 0, 2 * sizeof(unsigned long long)));
   /* persistent grid: as many resident blocks as the device holds */
   int dev = 0, ncu = 0, per_cu = 0;
   HIP_OK(hipGetDevice(&dev));
@@ -1697,6 +1794,18 @@ setup_launch(htgx_ctx* ctx, const DevMap& M, Launch* P)
   P->counters = st->d_counters;
   P->partials = (htgx_mc_accum*)st->d_partials;
   P->stats = ctx->d_counters;
+  P->trunc = st->d_trunc;
+  return HTGX_OK;
+}
+
+int htgx_wavefront_truncations(htgx_ctx* ctx, uint64_t out[2])
+{
+  out[0] = out[1] = 0;
+  if(!ctx->wf) return HTGX_OK;
+  unsigned long long h[2];
+  HIP_OK(hipMemcpy(h, ctx->wf->d_trunc, sizeof(h), hipMemcpyDeviceToHost));
+  out[0] = h[0];
+  out[1] = h[1];
   return HTGX_OK;
 }
 
@@ -2119,10 +2228,95 @@ int htgx_wavefront_batch_autotune(htgx_ctx* const* ctxs, const DevMap* Ms, uint3
   return HTGX_OK;
 }
 
-int htgx_debug_realisations_fp32(htgx_ctx*, const htgx_map_desc*, const uint64_t*,
-                                 const uint64_t*, uint32_t, double*)
+int htgx_debug_realisations_fp32(htgx_ctx* ctx, const htgx_map_desc* desc, const uint64_t* pix,
+                                 const uint64_t* real, uint32_t n, double* weights)
 {
-  return HTGX_BAD_OP;
+  int res;
+  if((res = wavefront_prepare(ctx)) != HTGX_OK) return res;
+  htgx_wavefront_state* st = ctx->wf;
+  DevMap M;
+  if((res = htgx_map_setup(ctx, desc, &M)) != HTGX_OK) return res;
+  const bool cie = desc->spectral_type == HTGX_SPECTRAL_SW_CIE_XYZ;
+  for(uint32_t i = 0; i < n; ++i)
+    if((real[i] >> 32) > (cie ? 2u : 0u)) return HTGX_BAD_ARG;
+  /* one list launch per CIE channel over the realisations of that channel */
+  for(int c = 0; c < (cie ? 3 : 1); ++c) {
+    std::vector<uint32_t> list;
+    std::vector<uint32_t> idx;
+    for(uint32_t i = 0; i < n; ++i)
+      if((int)(real[i] >> 32) == c) {
+        idx.push_back(i);
+        list.push_back((uint32_t)pix[i]);
+        list.push_back((uint32_t)(real[i] & 0xFFFFFFFFull));
+      }
+    if(idx.empty()) continue;
+    const uint32_t m = (uint32_t)idx.size();
+    const DevMap Mc = cie ? htgx_cie_channel_map(ctx, M, c) : M;
+    DevArrayLite<uint32_t> d_list;
+    DevArrayLite<htgx_mc_accum> d_acc;
+    if((res = d_list.alloc(list.size())) || (res = d_acc.alloc(m))) return res;
+    HIP_OK(hipMemcpy(d_list.p, list.data(), list.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    Launch P;
+    std::memset(&P, 0, sizeof(P));
+    P.width = Mc.width;
+    P.height = Mc.height;
+    P.K = 1;
+    P.nsub = 1;
+    P.ev_thresh = st->ev_thresh;
+    P.max_steps = 1u << 20;
+    P.counters = st->d_counters;
+    P.partials = d_acc.p;
+    P.nvalid = m;
+    P.trunc = st->d_trunc;
+    P.list = d_list.p;
+    zero_u32_kernel<<<1, 64>>>(st->d_counters, kXcds);
+    const uint32_t grid = std::min(st->grid_max, (m + 255) / 256);
+    if(Mc.sensor == HTGX_SENSOR_CAMERA) {
+      if(Mc.spectral_type == HTGX_SPECTRAL_LW)
+        wavefront_list_kernel<HTGX_SENSOR_CAMERA, HTGX_SPECTRAL_LW><<<grid, 256>>>(st->S, Mc, P);
+      else
+        wavefront_list_kernel<HTGX_SENSOR_CAMERA, HTGX_SPECTRAL_SW><<<grid, 256>>>(st->S, Mc, P);
+    } else {
+      if(Mc.spectral_type == HTGX_SPECTRAL_LW)
+        wavefront_list_kernel<HTGX_SENSOR_RECTANGLE, HTGX_SPECTRAL_LW><<<grid, 256>>>(st->S, Mc, P);
+      else
+        wavefront_list_kernel<HTGX_SENSOR_RECTANGLE, HTGX_SPECTRAL_SW><<<grid, 256>>>(st->S, Mc, P);
+    }
+    HIP_OK(hipGetLastError());
+    std::vector<htgx_mc_accum> acc(m);
+    HIP_OK(hipMemcpy(acc.data(), d_acc.p, m * sizeof(htgx_mc_accum), hipMemcpyDeviceToHost));
+    /* a rejected realisation (a flux sample under the ground) is NaN, as in
+     * the plain solver */
+    for(uint32_t k = 0; k < m; ++k)
+      weights[idx[k]] = acc[k].n ? acc[k].sum : std::nan("");
+  }
+  return HTGX_OK;
+}
+
+int htgx_debug_trace_segments_fp32_impl(htgx_ctx* ctx, const double* rays, uint32_t nrays, int prop,
+                                        uint32_t iband, uint32_t iquad, htgx_segment* segs,
+                                        uint32_t max_segs, uint32_t* nsegs)
+{
+  int res;
+  if((res = wavefront_prepare(ctx)) != HTGX_OK) return res;
+  for(uint32_t i = 0; i < nrays; ++i) {
+    const double* r = rays + (size_t)i * 8;
+    if(r[6] != 0.0 || !(r[7] >= 0.0)) return HTGX_BAD_ARG; /* queries start at their origin */
+  }
+  DevArrayLite<double> d_rays;
+  DevArrayLite<htgx_segment> d_segs;
+  DevArrayLite<uint32_t> d_n;
+  if((res = d_rays.alloc((size_t)nrays * 8)) || (res = d_segs.alloc((size_t)nrays * max_segs)) ||
+     (res = d_n.alloc(nrays)))
+    return res;
+  HIP_OK(hipMemcpy(d_rays.p, rays, (size_t)nrays * 8 * sizeof(double), hipMemcpyHostToDevice));
+  wf_trace_segments_kernel<<<(nrays + 255) / 256, 256>>>(ctx->wf->S, d_rays.p, nrays, prop, iband,
+                                                         iquad, d_segs.p, max_segs, d_n.p);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpy(segs, d_segs.p, (size_t)nrays * max_segs * sizeof(htgx_segment),
+                   hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(nsegs, d_n.p, (size_t)nrays * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  return HTGX_OK;
 }
 
 void htgx_wavefront_release(htgx_ctx* ctx)

@@ -312,7 +312,7 @@ struct htgx_map_desc {
  ******************************************************************************/
 struct htgx_ctx;
 
-/* Build the octree on the host, upload everything to the current device.
+/* Upload the scene to the current device and build the octree there.
  * Replaces the htsky_create + ground/sun setup of htrdr_atmosphere_create
  * (src/atmosphere/htrdr_atmosphere.c:348-382). */
 int htgx_ctx_create(const struct htgx_scene_desc* scene, struct htgx_ctx** ctx);
@@ -521,6 +521,15 @@ struct htgx_counters {
    * collision step, chunk fetch, event draws, realisation start, event
    * consumption, end of walk, end of realisation, query set-up. */
   uint64_t diag[16];
+  /* FP32 solver, maintained whether or not counting is enabled and summed
+   * over every map of the context since its creation: queries cut short by
+   * the budget of 2^20 tentative collisions (a majorant optical depth above
+   * about 10^6), and by the segment iterator's bound on consecutive passes
+   * that emit no segment (an iterator making no progress). Either ends the
+   * query as if it had left the medium, which would bias the estimate: a
+   * non-zero count says so. */
+  uint64_t ntrunc_coll;
+  uint64_t ntrunc_iter;
 };
 int htgx_set_counting(struct htgx_ctx* ctx, int enable);
 int htgx_get_counters(struct htgx_ctx* ctx, struct htgx_counters* counters);
@@ -554,10 +563,29 @@ int htgx_debug_trace_segments
    uint32_t max_segs,
    uint32_t* nsegs);
 
+/* The same segments as the FP32 wavefront solver's iterator sees them:
+ * FP32 origin, direction and distances, the leaf bounds of its fused leaf
+ * records (FP16 multiples of the grid's largest LWC, rounded outwards, so
+ * that kmin never exceeds and kmax never falls below the exact bounds up to
+ * the FP32 rounding of the gas term). Queries start at their origin:
+ * range[0] must be 0. Same arguments as htgx_debug_trace_segments. */
+int htgx_debug_trace_segments_fp32
+  (struct htgx_ctx* ctx,
+   const double* rays,
+   uint32_t nrays,
+   int prop, uint32_t iband, uint32_t iquad,
+   struct htgx_segment* segs,
+   uint32_t max_segs,
+   uint32_t* nsegs);
+
 /* Weight of individual realisations (pixel index y*W+x, realisation index),
- * computed by the solver desc->solver. For the plain FP64 solver these use the
- * same random streams as htgx_draw_map. For a CIE XYZ image, bits 32-33 of
- * the realisation index select the channel (0 X, 1 Y, 2 Z). HOST arrays. */
+ * computed by the solver desc->solver with the random streams of
+ * htgx_draw_map: the plain FP64 solver in one thread per realisation, the
+ * FP32 wavefront solver in its persistent loop fed from the list (for the
+ * FP32 solver the realisation index is taken modulo 2^32, as its streams are
+ * keyed). For a CIE XYZ image, bits 32-33 of the realisation index select
+ * the channel (0 X, 1 Y, 2 Z). A rejected realisation (a flux sample under
+ * the ground) is NaN. HOST arrays. */
 int htgx_debug_realisations
   (struct htgx_ctx* ctx,
    const struct htgx_map_desc* desc,
@@ -565,6 +593,33 @@ int htgx_debug_realisations
    const uint64_t* realisation_ids,
    uint32_t n,
    double* weights);
+
+/* The octree cache of option -O prepared on the host, without a device:
+ * if path holds a cache it is checked against the scene as
+ * htgx_ctx_create_cached checks it (same inputs; then its structure and its
+ * bounds against the scene's LWC) and *loaded = 1; if path does not exist
+ * the octree is built on the host (the same tree as the device build, bit
+ * for bit) and written there, *loaded = 0. HTGX_BAD_ARG with a message in
+ * err (errlen bytes, may be NULL) for a cache of other inputs, a damaged or
+ * hand-made one whose tree is invalid, or a cache that does not fit the
+ * cloud grid. A scene without clouds neither reads nor writes the file.
+ * No counterpart in the reference, where htsky reads its cache inside
+ * htsky_create (src/atmosphere/htrdr_atmosphere.c:369): it lets a cache be
+ * made or checked on a machine without the GPU. */
+int htgx_octree_cache_prepare
+  (const struct htgx_scene_desc* scene, const char* path, int* loaded, char* err, size_t errlen);
+
+/* Leaf block for option -V x,y,z, the maximum definition of the
+ * acceleration structure (src/atmosphere/htrdr_atmosphere_args.c:49-78,
+ * passed to htsky at src/atmosphere/htrdr_atmosphere.c:370-372; "by default
+ * the finest definition is that of clouds", doc/htrdr-atmosphere.1.in:
+ * 414-416). The octree's finest level is the grid of B x B x B blocks of
+ * cloud cells, so the finest definition along an axis is def / B; this
+ * returns the smallest B that divides the cloud definition def along every
+ * axis and keeps def / B <= max_def. HTGX_BAD_ARG if a definition is 0 or
+ * if no such block exists. */
+int htgx_leaf_block_for_definition
+  (const uint32_t def[3], const uint32_t max_def[3], uint32_t* leaf_block);
 
 /* The k_min/k_max octree of a scene's clouds built on the host (svo_build)
  * or on the current device (the build htgx_ctx_create uses), for comparing

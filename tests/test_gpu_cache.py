@@ -49,7 +49,7 @@ def test_written_then_read(tmp_path):
         assert c1.svo_info() == info
         assert same(render(c1), px)
     data = path.read_bytes()
-    assert data[:8] == b"HTGXOCT1"
+    assert data[:8] == b"HTGXOCT2"
     with A.Context(sc.ptr, cache=path) as c2:
         assert c2.cache_loaded
         assert c2.svo_info() == info
