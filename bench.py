@@ -100,8 +100,7 @@ def cpu_baseline(synth, desc, target_s):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_ffi as O
     from htrdr_kmin_amd import atmosphere as A
-    threads = min(16, os.cpu_count() or 1)
-    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    threads = host_threads()
     sky = O.Sky(synth.scene_ptr)
     # large terrains: the oracle's own BVH (the same casts as its brute force)
     O.lib().oracle_sky_use_bvh(sky.h, 1)
@@ -123,9 +122,28 @@ def cpu_baseline(synth, desc, target_s):
     dt = time.perf_counter() - t0
     sky.close()
     return {"value": n * d.spp / dt / 1e6, "unit": "Mrealisations/s", "cores": threads,
-            "kind": "port",
+            "kind": "port", "label": "restatement",
+            "threads_from": THREADS_FROM,
             "sample": f"{n} pixels evenly strided over the {desc.width}x{desc.height} image x "
-                      f"{d.spp} realisations, FP64 C/OpenMP oracle, {dt:.1f} s"}
+                      f"{d.spp} realisations, FP64 C/OpenMP oracle (the reference's algorithm "
+                      f"restated; the reference itself cannot be built), {threads} threads, "
+                      f"{dt:.1f} s"}
+
+
+THREADS_FROM = ""
+
+
+def host_threads():
+    """Threads of the CPU baseline: every core this process may run on
+    (its affinity mask), or OMP_NUM_THREADS when the host sets it (the CPU
+    share of one GPU on a shared GPU node)"""
+    global THREADS_FROM
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        THREADS_FROM = "OMP_NUM_THREADS"
+        return int(env)
+    THREADS_FROM = "sched_getaffinity"
+    return len(os.sched_getaffinity(0))
 
 
 def workload_name(synth, desc, S, steps):
@@ -380,6 +398,7 @@ def main():
         print(f"[bench] autotune: {params} after {ntrials} trials ({rate / 1e6:.2f} Mreal/s)",
               file=sys.stderr, flush=True)
     total_s, wall, h, counters = run(main_solver, args.steps, acc, True)
+    trunc = ctx.counters()
     real = npix * S * args.steps
     value = real / total_s / 1e6
     t05_main, spp_req, relvar_main, E_main, SE_main = t05(h, total_s, real)
@@ -433,7 +452,9 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 6),
             "traffic": traffic,
             "bytes_per_realisation": round(bytes_real, 2),
-            # the kernel's real bound (rocprofv3 counters, same launches)
+            # what the counters of the same launches show limits the kernel
+            # (the roofline above prices its algorithmic bytes against HBM)
+            "measured_limiter": counters_prof.get("limiter"),
             "valu_busy": counters_prof.get("valu_busy"),
             "valu_lane_utilisation": counters_prof.get("valu_lane_utilisation"),
             "l2_hit_rate": counters_prof.get("l2_hit_rate"),
@@ -447,6 +468,11 @@ def main():
             },
         },
         "setup": setup,
+        # queries of the FP32 solver cut short by its tentative-collision
+        # budget or its iterator's no-progress bound (each would bias the
+        # estimate; counted on every launch of the run)
+        "truncated_queries": {"collision_budget": trunc["ntrunc_coll"],
+                              "iterator_bound": trunc["ntrunc_iter"]},
         "cpu_baseline": None,
     }
     if args.fp64_steps > 0 and main_solver == F.SOLVER_WAVEFRONT_FP32:

@@ -74,6 +74,10 @@ struct SceneF {
   float zb, dzl, inv_dzl;
   const float* gas_ka; /* [#quads][nlayers] */
   const float* gas_ks; /* [nbands][nlayers] */
+  /* vertical optical depth of gas_ka / gas_ks from the bottom of the gas to
+   * the bottom of each layer, same layout (FP64 sums of the FP32 values) */
+  const double* cum_ka;
+  const double* cum_ks;
   const float* gas_T;
   int has_clouds;
   uint32_t B;
@@ -146,6 +150,7 @@ enum { LS_SEG = 0, LS_COLL = 1, LS_EVENT = 2, LS_DONE = 3 };
 enum { MODE_TR = 0, MODE_SC = 1, MODE_LW = 2 };
 enum { EV_NONE = 0, EV_HIT = 1, EV_ABS = 2, EV_SCA = 3 };
 enum {
+  /* ST_ABS: the scattering event proper, entered from ST_SCATTER */
   ST_DIRECT = 0, ST_SCATTER = 1, ST_ABS = 2, ST_SUN = 3, ST_LW = 4,
   ST_WALK_DONE = 5, ST_REAL_DONE = 6, ST_BEGIN = 7, ST_FETCH = 8
 };
@@ -175,7 +180,13 @@ struct Lane {
   bool lazy;            /* the range ends at the mesh's top, not yet cast */
   float qr;             /* end of the query range */
   float inv_dir[3];
-  float tc, Ts, maj, inv_maj;
+  float tc, Ts, maj, inv_maj;   /* next tentative collision (INFINITY: none in the segment) */
+  /* MODE_SC: the absorption transmissivity of the same path (k_min/k_max on
+   * Ka, ..._sw.c:396-400) tracked alongside the scattering query: Tmin holds
+   * its k_min optical depth, Ts_a the residual optical thickness left to its
+   * next tentative collision, tc_a that collision in the current segment
+   * (INFINITY: none), maj_a the segment's residual majorant */
+  float Ts_a, tc_a, maj_a;
   double Tmin;
   int ev;
   /* two counts of the current query: tentative collisions in the low
@@ -554,12 +565,34 @@ __device__ __forceinline__ void iter_advance(Lane& L, float t1, bool adv_g, bool
   if(at_end) L.itdone = true;
 }
 
+/* Vertical optical depth of the gas from its bottom up to height z (clamped
+ * into the gas), for the property prop of the lane's band and k-point: the
+ * cumulative sum at the layer's bottom plus the layer's coefficient times the
+ * height into it */
+__device__ __forceinline__ double gas_column(const SceneF& S, const Lane& L, int prop, float z)
+{
+  const float fj = floorf((z - S.zb) * S.inv_dzl);
+  const uint32_t j = fj < 0.f ? 0u : fj > (float)(S.nlayers - 1) ? S.nlayers - 1 : (uint32_t)fj;
+  const double h = fmin(fmax((double)z - ((double)S.zb + (double)j * (double)S.dzl), 0.0), (double)S.dzl);
+  double c = 0.0;
+  if(prop != HTGX_Ks) c += S.cum_ka[L.gka_off + j] + (double)S.gas_ka[L.gka_off + j] * h;
+  if(prop != HTGX_Ka) c += S.cum_ks[L.gks_off + j] + (double)S.gas_ks[L.gks_off + j] * h;
+  return c;
+}
+
 /* Next segment holding some medium into L.s_*; false at the end of the range.
+ * With `runs' (transmissivity queries, whose gas segments have k_min = k_max
+ * and so no collisions), a stretch of the ray outside the cloud grid is one
+ * segment over all the gas layers it crosses, up to the grid's entry, the
+ * range's end or the edge of the gas, whose k_min = k_max is its mean: its
+ * length times that is the stretch's optical depth, from the cumulative
+ * column of the gas (gas_column) instead of a sum over its layers.
  * The iterator only moves past an emitted segment at the next call, so that
  * the leaf range used to clamp the cells of its collisions stays the one of
  * the segment. Each pass of the loop loads at most one leaf, at one site. */
 template<bool COUNT>
-__device__ __forceinline__ bool iter_next(const SceneF& S, Lane& L, int prop, float* t0)
+__device__ __forceinline__ bool iter_next(const SceneF& S, Lane& L, int prop, float* t0,
+                                          bool runs = false)
 {
   if(L.adv) {
     L.adv = false;
@@ -575,6 +608,38 @@ __device__ __forceinline__ bool iter_next(const SceneF& S, Lane& L, int prop, fl
     }
     const float t = L.tit;
     const float tc = L.cstate == 0 ? L.tnext : L.cstate == 1 ? fmaxf(L.tnext, t) : INFINITY;
+    if(runs && L.cstate != 1 && L.j >= 0 && L.j < (int32_t)S.nlayers && L.dir[2] != 0.f) {
+      /* a gas run: up to the grid's entry, the range's end or the edge of
+       * the gas, whichever comes first */
+      const float ze = L.dir[2] > 0.f ? S.zb + (float)S.nlayers * S.dzl : S.zb;
+      const float tx = fmaxf((ze - L.pos[2]) * L.inv_dir[2], t);
+      const float t1 = fminf(fminf(tc, tx), tmax);
+      if(t1 > t) {
+        const double tau = fabs(gas_column(S, L, prop, L.pos[2] + t1 * L.dir[2])
+                              - gas_column(S, L, prop, L.pos[2] + t * L.dir[2]))
+                         * fabs((double)L.inv_dir[2]);
+        const float k = (float)(tau / (double)(t1 - t));
+        *t0 = t;
+        L.s_t1 = t1;
+        L.s_kmin = L.s_kmax = k;
+        L.s_gka = L.s_gks = 0.f;
+        L.s_cloud = false;
+        L.s_layer = L.j;
+        /* past the run: the layer the ray is in just beyond t1 (or out of
+         * the gas), the grid entered if the run ended there */
+        const float zf = (L.pos[2] + t1 * L.dir[2] - S.zb) * S.inv_dzl;
+        int32_t j1 = L.dir[2] > 0.f ? (int32_t)floorf(zf) : (int32_t)ceilf(zf) - 1;
+        if(t1 == tx) j1 = L.dir[2] > 0.f ? (int32_t)S.nlayers : -1;
+        L.j = j1 < -1 ? -1 : j1 > (int32_t)S.nlayers ? (int32_t)S.nlayers : j1;
+        L.adv = true;
+        L.adv_g = false;
+        L.adv_c = tc == t1;
+        L.adv_end = tc == INFINITY && t1 == tx;
+        L.nsteps &= kCollMask;
+        if(COUNT) L.c_seg++;
+        return true;
+      }
+    }
     float tg;
     if(L.dir[2] > 0.f && L.j < (int32_t)S.nlayers) tg = (S.zb + (float)(L.j + 1) * S.dzl - L.pos[2]) * L.inv_dir[2];
     else if(L.dir[2] < 0.f && L.j >= 0) tg = (S.zb + (float)L.j * S.dzl - L.pos[2]) * L.inv_dir[2];
@@ -725,22 +790,23 @@ __device__ __forceinline__ void consume(const SceneF& S, Lane& L, const float* u
       break;
 
     case ST_SCATTER: {
+      /* the scattering query (..._sw.c:355-375) and, tracked alongside it,
+       * the absorption transmissivity up to its event (:396-400): a real
+       * absorption collision on the way makes the latter 0 and ends the
+       * walk, as an escape does */
       const bool hit = L.ev == EV_HIT;
-      if(!L.gh_hit && !hit) { LC(stage) = ST_WALK_DONE; break; }
+      if(L.ev == EV_ABS || (!L.gh_hit && !hit)) { LC(stage) = ST_WALK_DONE; break; }
       if(COUNT) L.c_ev++;
       L.surface = !hit;
       const float dst = L.surface ? L.qr : LC(ev_dst);
       /* ..._sw.c:392: the next traces leave this surface hit, if any */
       LC(prev_prim) = L.surface ? LC(gh_prim) : -1;
       L.prev_edge = L.surface && L.gh_edge;
-      LC(stage) = ST_ABS;
       LC(ev_dst) = dst;
-      /* the absorption query runs from pos along dir up to the event */
-      request(L, MODE_TR, HTGX_Ka, dst);
-    } break;
-
+      LC(Tr_abs) = L.Tmin != 0.0 ? (float)exp(-L.Tmin) : 1.f;
+    }
+      [[fallthrough]];
     case ST_ABS: {
-      LC(Tr_abs) = query_tr(L);
       if(LC(Tr_abs) <= 0.f) { LC(stage) = ST_WALK_DONE; break; }
       const float dst = LC(ev_dst);
       float N[3] = {0.f, 0.f, 1.f};
@@ -1099,10 +1165,27 @@ __device__ __forceinline__ void run_scene(const SceneF& S0, const DevMap& M0, co
     if(L.lstate == LS_SEG) {
       float t0;
       const int sprop = L.mode == MODE_SC ? HTGX_Ks : L.mode == MODE_LW ? HTGX_Kext : L.prop;
-      if(iter_next<COUNT>(S, L, sprop, &t0)) {
-        if(L.mode == MODE_TR) L.Tmin += (double)(L.s_t1 - t0) * (double)L.s_kmin;
+      if(iter_next<COUNT>(S, L, sprop, &t0, L.mode == MODE_TR)) {
+        const float len = L.s_t1 - t0;
+        if(L.mode == MODE_TR) L.Tmin += (double)len * (double)L.s_kmin;
+        if(L.mode == MODE_SC) {
+          /* the absorption transmissivity over the segment: its k_min part
+           * (taken back beyond a scattering event) and the first tentative
+           * collision of its residual tracking */
+          const float amin = L.s_gka + (L.s_cloud ? L.cka * L.lmn : 0.f);
+          const float maj_a = L.s_cloud ? L.cka * (L.lmx - L.lmn) : 0.f;
+          L.Tmin += (double)len * (double)amin;
+          const float Ta = len * maj_a;
+          if(L.Ts_a > Ta || !(maj_a > 0.f)) {
+            L.Ts_a -= Ta;
+          } else {
+            L.maj_a = maj_a;
+            L.tc_a = t0 + L.Ts_a / maj_a;
+            L.lstate = LS_COLL;
+          }
+        }
         const float maj = L.mode == MODE_TR ? L.s_kmax - L.s_kmin : L.s_kmax;
-        const float T = (L.s_t1 - t0) * maj;
+        const float T = len * maj;
         if(L.Ts > T || !(maj > 0.f)) {
           L.Ts -= T; /* no collision in this segment */
         } else {
@@ -1134,30 +1217,45 @@ __device__ __forceinline__ void run_scene(const SceneF& S0, const DevMap& M0, co
       d_cl += __popcll(bc);
     }
     if(L.lstate == LS_COLL) {
-      if(L.mode == MODE_SC && L.tc > 1e9f) {
-        L.lstate = LS_SEG; /* ..._sw.c:123 */
-      } else {
+      /* ..._sw.c:123: a scattering candidate beyond 1e9 m is dropped, the
+       * query moving on to the next segment with the same optical thickness */
+      if(L.mode == MODE_SC && L.tc > 1e9f) L.tc = INFINITY;
+      /* the nearer of the scattering and the absorption candidates */
+      const bool is_a = L.tc_a < L.tc;
+      const float tx = is_a ? L.tc_a : L.tc;
+      if(tx < INFINITY) {
         if(COUNT) L.c_coll++;
         float x[3];
 #pragma unroll
-        for(int a = 0; a < 3; ++a) x[a] = L.pos[a] + L.tc * L.dir[a];
+        for(int a = 0; a < 3; ++a) x[a] = L.pos[a] + tx * L.dir[a];
         const float v = L.s_cloud ? seg_lwcf<COUNT>(S, L, x) : 0.f;
         const float gka = L.s_gka, gks = L.s_gks;
         const float ka = gka + L.cka * v;
         const float ks = gks + L.cks * v;
         const float2 uu = draw2(L, key);
-        if(L.mode == MODE_LW) {
+        if(is_a) {
+          /* residual tracking of the absorption transmissivity: a real
+           * collision makes it 0 and ends the walk (..._sw.c:396-400,
+           * 206-209) */
+          const float amin = gka + (L.s_cloud ? L.cka * L.lmn : 0.f);
+          L.ev = uu.x < (ka - amin) / L.maj_a ? EV_ABS : EV_NONE;
+        } else if(L.mode == MODE_LW) {
           const float pa = ka * L.inv_maj, ps = ks * L.inv_maj;
           L.ev = uu.x < pa ? EV_ABS : uu.x < pa + ps ? EV_SCA : EV_NONE;
         } else {
           const float k = L.prop == HTGX_Ka ? ka : L.prop == HTGX_Ks ? ks : ka + ks;
           const float proba = (L.mode == MODE_TR ? k - L.s_kmin : k) * L.inv_maj;
           L.ev = uu.x < proba ? EV_HIT : EV_NONE;
-          /* gas vs particle scattering at the event, ..._sw.c:427-438 */
-          if(L.mode == MODE_SC && L.ev != EV_NONE) LC(gas_ratio) = gks / ks;
+          if(L.mode == MODE_SC && L.ev != EV_NONE) {
+            /* gas vs particle scattering at the event, ..._sw.c:427-438 */
+            LC(gas_ratio) = gks / ks;
+            /* the absorption optical depth stops at the event */
+            const float amin = gka + (L.s_cloud ? L.cka * L.lmn : 0.f);
+            L.Tmin -= (double)(L.s_t1 - tx) * (double)amin;
+          }
         }
         if(L.ev != EV_NONE) {
-          LC(ev_dst) = L.tc;
+          LC(ev_dst) = tx;
           LC(ev_layer) = L.s_layer;
           L.pending = true;
           L.lstate = LS_EVENT;
@@ -1167,13 +1265,20 @@ __device__ __forceinline__ void run_scene(const SceneF& S0, const DevMap& M0, co
           if(P.trunc) atomicAdd(P.trunc, 1ull);
           L.pending = true;
           L.lstate = LS_EVENT;
+        } else if(is_a) {
+          L.Ts_a = exp1f(uu.y);
+          const float Ta = (L.s_t1 - L.tc_a) * L.maj_a;
+          if(L.Ts_a > Ta) { L.Ts_a -= Ta; L.tc_a = INFINITY; }
+          else L.tc_a += L.Ts_a / L.maj_a;
         } else {
           L.Ts = exp1f(uu.y);
           const float T = (L.s_t1 - L.tc) * L.maj;
-          if(L.Ts > T) { L.Ts -= T; L.lstate = LS_SEG; }
+          if(L.Ts > T) { L.Ts -= T; L.tc = INFINITY; }
           else L.tc += L.Ts * L.inv_maj;
         }
       }
+      /* no candidate left in the segment: on to the next one */
+      if(L.lstate == LS_COLL && L.tc == INFINITY && L.tc_a == INFINITY) L.lstate = LS_SEG;
     }
     WF_TICK(1)
 
@@ -1305,8 +1410,10 @@ __device__ __forceinline__ void run_scene(const SceneF& S0, const DevMap& M0, co
             L.Ts = exp1f(u[2 * NP - 1]);
             L.Tmin = 0.0;
             L.pending = false;
+            if(L.mode == MODE_SC) L.Ts_a = exp1f(draw2(L, key).x);
             iter_init(S, L);
           }
+          L.tc = L.tc_a = INFINITY;
           L.lstate = LS_SEG;
         }
         WF_TICK(8)
@@ -1679,6 +1786,31 @@ wavefront_build(htgx_ctx* ctx, htgx_wavefront_state* st)
   const uint32_t nq = qoff[D.nbands];
   if((res = upload_as_float(st, D.gas_ka, (size_t)nq * D.nlayers, &S.gas_ka))) return res;
   if((res = upload_as_float(st, D.gas_ks, (size_t)D.nbands * D.nlayers, &S.gas_ks))) return res;
+  {
+    /* cumulative vertical optical depths of the FP32 coefficients, for the
+     * gas runs of transmissivity queries */
+    const uint32_t nl = D.nlayers;
+    std::vector<double> ka((size_t)nq * nl), ks((size_t)D.nbands * nl);
+    if(nl) {
+      HIP_OK(hipMemcpy(ka.data(), D.gas_ka, ka.size() * sizeof(double), hipMemcpyDeviceToHost));
+      HIP_OK(hipMemcpy(ks.data(), D.gas_ks, ks.size() * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    const double dz = (double)S.dzl;
+    auto cumulate = [&](std::vector<double>& v, size_t nrows) {
+      for(size_t r = 0; r < nrows; ++r) {
+        double acc = 0.0;
+        for(uint32_t j = 0; j < nl; ++j) {
+          const double k = (double)(float)v[r * nl + j];
+          v[r * nl + j] = acc;
+          acc += k * dz;
+        }
+      }
+    };
+    cumulate(ka, nq);
+    cumulate(ks, D.nbands);
+    if((res = upload_f(st, ka, &S.cum_ka))) return res;
+    if((res = upload_f(st, ks, &S.cum_ks))) return res;
+  }
   if(D.gas_T && (res = upload_as_float(st, D.gas_T, D.nlayers, &S.gas_T))) return res;
   S.has_clouds = D.has_clouds;
   if(D.has_clouds) {

@@ -13,21 +13,37 @@
  * phase functions, star-camera pinhole); see oracle.c for each function.
  *
  * Parity pins (what checks each function, tests/ file):
- *   threefry4x64_20 ........ Random123 known-answer vectors + rocRAND's
- *                            independent implementation (test_oracle_rng.py)
+ *   threefry4x64_20 ........ Random123's published known answers for the zero
+ *                            and the all-ones counter and key (the latter
+ *                            runs the key schedule's parity word through every
+ *                            injection); every counter and key word changes
+ *                            every output word (test_oracle_rng.py)
  *   octree / segments ...... brute-force DDA over the finest cells on tiny
  *                            grids, exact FP32 min/max (test_oracle_svo.py)
  *   planck / wiebelt ....... Stefan-Boltzmann integral, Wien's law, numerical
  *                            quadrature of Planck (test_oracle_spectral.py)
  *   brightness temperature . closed-form monochromatic inverse, round trips
  *   wavelength sampler ..... brute-force CDF on small band counts, mono case
- *   phase / cone / lambert . chi-square against analytic pdfs, moments
+ *   phase / cone / lambert . Kolmogorov-Smirnov against analytic CDFs, moments
  *                            (test_oracle_samplers.py)
  *   transmissivity ......... k_min = k_max gives exp(-tau) with zero variance;
- *                            majorant inflation leaves the mean unchanged
- *   SW / LW walks .......... closed forms: vacuum + Lambertian ground, purely
- *                            absorbing slab, isothermal enclosure, optically
- *                            thin single scattering (test_oracle_walks.py)
+ *                            absorbing cloud against an exact cell-by-cell
+ *                            integral at three octree thresholds
+ *                            (test_oracle_walks.py)
+ *   SW walk ................ vacuum + Lambertian ground; direct sun through
+ *                            Kext; volume scattering by the first-order closed
+ *                            form of an optically thin slab (HG particles,
+ *                            Rayleigh gas, a 3:1 mix; 20 and 150 degrees from
+ *                            the sun), which a flipped scattering angle or
+ *                            swapped gas/particle odds fail by many standard
+ *                            errors (test_oracle_walks.py, closed_forms.py)
+ *   LW walk ................ isothermal enclosures, absorbing slab; emission at
+ *                            the layer temperature under a 9 K-per-layer
+ *                            gradient (gas and cloud absorption, looking up
+ *                            and down from three heights), which a layer index
+ *                            off by one or the start layer's temperature fail
+ *                            (test_oracle_walks.py, closed_forms.py)
+ *   flux maps .............. SW vacuum cos Omega L_sun; LW thick isothermal pi B
  *   mesh ground casts ...... flat mesh = plane, heightfield rejection area,
  *                            no leaks (test_oracle_ground.py); the optional
  *                            BVH = the brute force bit for bit

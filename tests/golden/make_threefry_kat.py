@@ -2,8 +2,9 @@
 
 * "published": the Random123 known-answer vectors (kat_vectors of the
   Random123 1.x distribution, Salmon et al., SC'11) for Threefry-4x64-20
-  (zero counter and key) and Threefry-2x32-20 (zero, all-ones and the pi
-  digits), typed in as data.
+  (zero and all-ones counter and key; the all-ones key exercises the key
+  schedule's parity word in every injection) and Threefry-2x32-20 (zero,
+  all-ones and the pi digits), typed in as data.
 * "rocrand_2x32": 64 seeded random (counter, key) pairs through rocRAND's
   Threefry-2x32-20 (oracle/_build/kat_rocrand, built by `make -C oracle`),
   an implementation written independently of this repository's.
@@ -28,6 +29,8 @@ PUBLISHED = {
     "threefry4x64_20": [
         {"ctr": ["0000000000000000"] * 4, "key": ["0000000000000000"] * 4,
          "out": ["09218ebde6c85537", "55941f5266d86105", "4bd25e16282434dc", "ee29ec846bd2e40b"]},
+        {"ctr": ["ffffffffffffffff"] * 4, "key": ["ffffffffffffffff"] * 4,
+         "out": ["29c24097942bba1b", "0371bbfb0f6f4e11", "3c231ffa33f83a1c", "cd29113fde32d168"]},
     ],
     "threefry2x32_20": [
         {"ctr": ["00000000", "00000000"], "key": ["00000000", "00000000"], "out": ["6b200159", "99ba4efe"]},

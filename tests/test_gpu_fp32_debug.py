@@ -75,7 +75,7 @@ def test_fp32_segments_match_oracle(tree, prop, request):
     o, on = sky.trace_segments(rays, prop, max_segs=512)
     assert on.max() < 512 and gn.max() < 512
     c = cloud_factor(synth.scene, prop)
-    scale = float(synth.lwc.max())
+    scale = float(synth.lwc().max())
     checked = nseg = 0
     for i in range(len(rays)):
         a, b = g[i, : gn[i]], o[i, : on[i]]
