@@ -108,15 +108,23 @@ def test_general_maps_vs_oracle(gm):
 
 
 def test_general_fp32_vs_fp64(gm):
+    """FP32 vs FP64 per pixel. On the flux map the rectangle cuts through
+    the terrain, so a pixel keeps only its samples above the ground: pixels
+    are compared where both solvers kept at least 8 of their 32 (a mean of 1
+    or 2 samples has no usable standard error: one side at n = 0 against
+    n = 2 on the other gives any z)"""
     _, ctx, _ = gm
     for mk in (hills_camera, hills_flux):
         p64 = ctx.draw_map(mk(spp=32, solver=F.SOLVER_PLAIN_FP64))
         p32 = ctx.draw_map(mk(spp=32, solver=F.SOLVER_WAVEFRONT_FP32, seed=1))
         if mk is hills_camera:
             a, b = (p32["radiance_E"], p32["radiance_SE"]), (p64["radiance_E"], p64["radiance_SE"])
+            keep = np.ones(a[0].shape, bool)
         else:
             a, b = flux_estimate(p32), flux_estimate(p64)
-        w, zm, zs = zstats(a[0], a[1], b[0], b[1])
+            keep = (p32["flux_n"] >= 8) & (p64["flux_n"] >= 8)
+            assert keep.mean() > 0.5
+        w, zm, zs = zstats(a[0][keep], a[1][keep], b[0][keep], b[1][keep])
         assert w >= 0.99 and abs(zm) < 0.1 and zs < 1.2, mk.__name__
 
 
