@@ -156,6 +156,7 @@ SIGNATURES = {
     "htgx_ctx_get_svo_info": (C.c_int, [C.c_void_p, c_uint32_p, c_uint64_p, c_uint64_p]),
     "htgx_draw_map": (C.c_int, [C.c_void_p, C.POINTER(MapDesc), C.c_void_p, C.c_size_t]),
     "htgx_render_accum": (C.c_int, [C.c_void_p, C.POINTER(MapDesc), C.c_void_p, C.c_void_p]),
+    "htgx_render_accum_noalloc": (C.c_int, [C.c_void_p, C.POINTER(MapDesc), C.c_void_p, C.c_void_p]),
     "htgx_finalize_pixels": (C.c_int, [C.POINTER(MapDesc), C.c_void_p, C.c_void_p, C.c_size_t]),
     "htgx_set_counting": (C.c_int, [C.c_void_p, C.c_int]),
     "htgx_get_counters": (C.c_int, [C.c_void_p, C.POINTER(Counters)]),

@@ -928,7 +928,7 @@ static double normalize(double r[3], const double a[3])
 }
 
 int
-htgx_map_setup(htgx_ctx* ctx, const htgx_map_desc* d, DevMap* M)
+htgx_map_setup(htgx_ctx* ctx, const htgx_map_desc* d, DevMap* M, bool may_allocate)
 {
   if(!ctx || !d || !M) return HTGX_BAD_ARG;
   /* check_draw_map_args, src/core/htrdr_draw_map.c:66-79 */
@@ -1015,6 +1015,7 @@ htgx_map_setup(htgx_ctx* ctx, const htgx_map_desc* d, DevMap* M)
     const bool cached = ctx->d_cie_cdf && ctx->cie_nbands == nwb
       && ctx->cie_range[0] == range[0] && ctx->cie_range[1] == range[1];
     if(!cached) {
+      if(!may_allocate) return HTGX_BAD_OP;
       /* setup_cie_xyz, :183-270: per band the trapezoidal integral of each
        * fit over 2 sub-intervals per nm (trapezoidal_integration, :55-78) */
       std::vector<double> cdf(3 * nwb);
@@ -1060,6 +1061,7 @@ htgx_map_setup(htgx_ctx* ctx, const htgx_map_desc* d, DevMap* M)
       && ctx->w_range[1] == range[1] && ctx->w_Tref == Tref;
     M->band_len = (range[1] - range[0]) / (double)nwb;
     if(!cached) {
+      if(!may_allocate) return HTGX_BAD_OP;
       /* setup_wlen_planck_ran_cdf, src/core/htrdr_ran_wlen_planck.c:55-134 */
       std::vector<double> pdf(nwb), cdf(nwb);
       double sum = 0;
@@ -1168,6 +1170,29 @@ htgx_render_accum(htgx_ctx* ctx, const htgx_map_desc* desc, htgx_mc_accum* accum
       ? htgx_render_wavefront(ctx, Mc, a, (hipStream_t)stream)
       : render_plain(ctx, Mc, a, (hipStream_t)stream);
     if(res != HTGX_OK) return res;
+    nreal += ctx->last_nrealisations;
+  }
+  ctx->last_nrealisations = nreal;
+  return HTGX_OK;
+}
+
+extern "C" int
+htgx_render_accum_noalloc(htgx_ctx* ctx, const htgx_map_desc* desc, htgx_mc_accum* accum,
+                          void* stream)
+{
+  DevMap M;
+  int res;
+  if(!ctx || !desc || !accum) return HTGX_BAD_ARG;
+  if(desc->solver != HTGX_SOLVER_WAVEFRONT_FP32) return HTGX_BAD_ARG;
+  if((res = htgx_map_setup(ctx, desc, &M, false)) != HTGX_OK) return res;
+  const int nch = desc->spectral_type == HTGX_SPECTRAL_SW_CIE_XYZ ? 3 : 1;
+  const size_t npix = (size_t)desc->width * desc->height;
+  uint64_t nreal = 0;
+  for(int c = 0; c < nch; ++c) {
+    const DevMap Mc = nch == 3 ? htgx_cie_channel_map(ctx, M, c) : M;
+    if((res = htgx_render_wavefront_noalloc(ctx, Mc, accum + (size_t)c * npix,
+                                            (hipStream_t)stream)) != HTGX_OK)
+      return res;
     nreal += ctx->last_nrealisations;
   }
   ctx->last_nrealisations = nreal;

@@ -54,7 +54,11 @@ struct htgx_ctx {
 };
 
 int htgx_ctx_init(htgx_ctx* ctx, const htgx_scene_desc& scene);
-int htgx_map_setup(htgx_ctx* ctx, const htgx_map_desc* desc, htgx::DevMap* map);
+/* may_allocate false: a map whose wavelength tables are not those the
+ * context holds from its last map returns HTGX_BAD_OP instead of building
+ * and uploading them */
+int htgx_map_setup(htgx_ctx* ctx, const htgx_map_desc* desc, htgx::DevMap* map,
+                   bool may_allocate = true);
 /* The map of channel c (0 X, 1 Y, 2 Z) of a CIE XYZ image set up by
  * htgx_map_setup: the channel's wavelength CDF and its own random streams */
 htgx::DevMap htgx_cie_channel_map(const htgx_ctx* ctx, const htgx::DevMap& M, int c);
@@ -93,6 +97,10 @@ int htgx_debug_trace_segments_fp32_impl(htgx_ctx* ctx, const double* rays, uint3
  * [0] by the tentative-collision budget, [1] by the segment iterator's bound
  * on passes without a segment */
 int htgx_wavefront_truncations(htgx_ctx* ctx, uint64_t out[2]);
+/* htgx_render_accum_noalloc of one map (channel) on the FP32 solver: the
+ * chunk counters and partials in the library's static scratch array */
+int htgx_render_wavefront_noalloc(htgx_ctx* ctx, const htgx::DevMap& map, htgx_mc_accum* accum,
+                                  hipStream_t stream);
 void htgx_wavefront_release(htgx_ctx* ctx);
 
 #endif /* HTGX_INTERNAL_H */

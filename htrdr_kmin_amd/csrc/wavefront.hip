@@ -44,6 +44,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -1686,6 +1687,47 @@ int launch(const SceneF& S, const DevMap& M, const Launch& P, uint32_t grid, hip
 } /* namespace */
 
 /*******************************************************************************
+ * Scratch of htgx_render_accum_noalloc
+ ******************************************************************************/
+/* The chunk queue counters (the first kScratchHead bytes) and the chunk
+ * partial sums of one FP32 map launched by htgx_render_accum_noalloc, in
+ * device memory reserved with the code object on each GPU the library runs
+ * on and held until the process ends, instead of memory the call allocates.
+ * Every call on a GPU shares that GPU's copy: calls that overlap on one GPU
+ * race. */
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+constexpr size_t kScratchHead = 256;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+/* Device address of each GPU's copy, looked up once per device (at the FP32
+ * set-up of its first context, with that device current). The array's name
+ * in host code is a stand-in, not a device address: only these pointers
+ * reach kernels. */
+constexpr int kMaxScratchDevices = 64;
+static void* g_scratch[kMaxScratchDevices];
+static std::mutex g_scratch_lock;
+
+static int scratch_lookup()
+{
+  int dev = 0;
+  HIP_OK(hipGetDevice(&dev));
+  if(dev < 0 || dev >= kMaxScratchDevices) return HTGX_BAD_OP;
+  std::lock_guard<std::mutex> guard(g_scratch_lock);
+  if(g_scratch[dev]) return HTGX_OK;
+  void* p = nullptr;
+  HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_scratch[dev] = p;
+  return HTGX_OK;
+}
+
+static void* scratch_of(int dev)
+{
+  if(dev < 0 || dev >= kMaxScratchDevices) return nullptr;
+  std::lock_guard<std::mutex> guard(g_scratch_lock);
+  return g_scratch[dev];
+}
+
+/*******************************************************************************
  * Host side
  ******************************************************************************/
 struct htgx_wavefront_state {
@@ -1881,6 +1923,7 @@ wavefront_build(htgx_ctx* ctx, htgx_wavefront_state* st)
 
   HIP_OK(hipMalloc(&st->d_counters, kXcds * sizeof(unsigned int)));
   HIP_OK(hipMalloc(&st->d_trunc, 2 * sizeof(unsigned long long)));
+  if((res = scratch_lookup()) != HTGX_OK) return res;
   HIP_OK(hipMemset(st->d_trunc, 0, 2 * sizeof(unsigned long long)));
   /* persistent grid: as many resident blocks as the device holds */
   int dev = 0, ncu = 0, per_cu = 0;
@@ -1895,9 +1938,9 @@ wavefront_build(htgx_ctx* ctx, htgx_wavefront_state* st)
   return HTGX_OK;
 }
 
-/* Launch parameters of one map; sizes the chunk partials of the context */
+/* Launch parameters of one map, its partials and counters aside */
 static int
-setup_launch(htgx_ctx* ctx, const DevMap& M, Launch* P)
+fill_launch(htgx_ctx* ctx, const DevMap& M, Launch* P)
 {
   htgx_wavefront_state* st = ctx->wf;
   std::memset(P, 0, sizeof(*P));
@@ -1917,7 +1960,19 @@ setup_launch(htgx_ctx* ctx, const DevMap& M, Launch* P)
   const uint64_t nchunks = (uint64_t)P->ntiles * P->nsub * 64u;
   if(nchunks > 0xFFFFFFFFull) return HTGX_BAD_ARG;
   P->nvalid = (uint32_t)((uint64_t)M.width * M.height * P->nsub);
-  const size_t need = nchunks * sizeof(htgx_mc_accum);
+  P->stats = ctx->d_counters;
+  P->trunc = st->d_trunc;
+  return HTGX_OK;
+}
+
+/* Launch parameters of one map; sizes the chunk partials of the context */
+static int
+setup_launch(htgx_ctx* ctx, const DevMap& M, Launch* P)
+{
+  htgx_wavefront_state* st = ctx->wf;
+  int res;
+  if((res = fill_launch(ctx, M, P)) != HTGX_OK) return res;
+  const size_t need = (size_t)P->ntiles * P->nsub * 64u * sizeof(htgx_mc_accum);
   if(need > st->partials_cap) {
     if(st->d_partials) { HIP_OK(hipFree(st->d_partials)); st->d_partials = nullptr; }
     HIP_OK(hipMalloc(&st->d_partials, need));
@@ -1925,8 +1980,6 @@ setup_launch(htgx_ctx* ctx, const DevMap& M, Launch* P)
   }
   P->counters = st->d_counters;
   P->partials = (htgx_mc_accum*)st->d_partials;
-  P->stats = ctx->d_counters;
-  P->trunc = st->d_trunc;
   return HTGX_OK;
 }
 
@@ -1960,6 +2013,46 @@ int htgx_render_wavefront(htgx_ctx* ctx, const DevMap& M, htgx_mc_accum* accum, 
   const uint32_t npix = M.width * M.height;
   wf_merge_kernel<<<(npix + 255) / 256, 256, 0, stream>>>(
     (const htgx_mc_accum*)st->d_partials, M.width, M.height, P.ntx, P.nsub, accum);
+  HIP_OK(hipGetLastError());
+  ctx->last_nrealisations = (uint64_t)npix * M.spp;
+  return HTGX_OK;
+}
+
+int htgx_render_wavefront_noalloc(htgx_ctx* ctx, const DevMap& M, htgx_mc_accum* accum,
+                                  hipStream_t stream)
+{
+  htgx_wavefront_state* st = ctx->wf;
+  if(!st) return HTGX_BAD_OP; /* the FP32 tables would have to be built */
+  int dev = 0, res;
+  HIP_OK(hipGetDevice(&dev));
+  if(dev != ctx->device) return HTGX_BAD_ARG;
+  unsigned char* base = (unsigned char*)scratch_of(dev);
+  if(!base) return HTGX_BAD_OP;
+  Launch P;
+  if((res = fill_launch(ctx, M, &P)) != HTGX_OK) return res;
+  const uint64_t need = (uint64_t)P.ntiles * P.nsub * 64u * sizeof(htgx_mc_accum);
+  if(need > kScratchBytes - kScratchHead) {
+    fprintf(stderr, "htgx: htgx_render_accum_noalloc: %llu chunk partials of %zu B exceed the "
+            "%zu MiB scratch (fewer pixels, fewer realisations per call, or a larger chunk)\n",
+            (unsigned long long)(need / sizeof(htgx_mc_accum)), sizeof(htgx_mc_accum),
+            kScratchBytes >> 20);
+    return HTGX_BAD_ARG;
+  }
+  P.counters = (unsigned int*)base;
+  P.partials = (htgx_mc_accum*)(base + kScratchHead);
+  /* fill kernels, no memset: the call may be captured in a graph */
+  zero_u32_kernel<<<1, 64, 0, stream>>>(P.counters, kXcds);
+  HIP_OK(hipGetLastError());
+  if(ctx->counting) {
+    zero_u64_kernel<<<1, 64, 0, stream>>>(ctx->d_counters, HTGX_NSTATS);
+    HIP_OK(hipGetLastError());
+    if((res = launch<true>(st->S, M, P, st->grid, stream))) return res;
+  } else {
+    if((res = launch<false>(st->S, M, P, st->grid, stream))) return res;
+  }
+  const uint32_t npix = M.width * M.height;
+  wf_merge_kernel<<<(npix + 255) / 256, 256, 0, stream>>>(P.partials, M.width, M.height, P.ntx,
+                                                           P.nsub, accum);
   HIP_OK(hipGetLastError());
   ctx->last_nrealisations = (uint64_t)npix * M.spp;
   return HTGX_OK;

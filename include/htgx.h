@@ -384,6 +384,28 @@ int htgx_render_accum
    struct htgx_mc_accum* accum,
    void* stream);
 
+/* htgx_render_accum on the FP32 wavefront solver without any allocation.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Its chunk counters and partial sums live
+ * in a static device array of 64 MiB that each GPU holds from the first FP32
+ * set-up of a context on it until the process ends; a map needing more
+ * chunk partials than it holds ((64 MiB - 256 B) / 64 B, one per pixel and
+ * chunk of the context's chunk-size launch parameter) returns HTGX_BAD_ARG.
+ * The context's FP32 tables must exist (built by its first FP32 map or by
+ * htgx_get_wavefront_params), and a band or CIE map must use the wavelength
+ * tables of the context's last map (same range and reference temperature):
+ * otherwise HTGX_BAD_OP and nothing is allocated. desc->solver must be
+ * HTGX_SOLVER_WAVEFRONT_FP32. Same arguments and, with the same launch
+ * parameters, the same accumulators bit for bit as htgx_render_accum. Every
+ * call on a GPU shares that GPU's array: calls that overlap on one GPU race
+ * (serialise them, e.g. on one stream). */
+int htgx_render_accum_noalloc
+  (struct htgx_ctx* ctx,
+   const struct htgx_map_desc* desc,
+   struct htgx_mc_accum* accum,
+   void* stream);
+
 /* Launch parameters of the FP32 wavefront solver of a context. No
  * counterpart in the reference, whose tile dispatch has no such knobs. They
  * change the schedule only: the realisations, and so the estimates, are the
