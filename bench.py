@@ -42,7 +42,10 @@ HBM_PEAK_GBS = 8000.0
 # (solver, config) -> rocprofv3 summary of the kernel on that workload and the
 # map width it was taken at, for its 16-realisation launches
 PROFILED = {("wavefront", 1): ("traffic_wavefront_sw_image_1024.json", 1024),
+            ("wavefront", 2): ("traffic_wavefront_lw_flux_512.json", 512),
             ("wavefront", 4): ("traffic_wavefront_mesh_flux_512.json", 512)}
+# the uneven 64-scene batch, one step at full size: one batch-kernel launch
+PROFILED_BATCH = "traffic_wavefront_batch_64.json"
 
 
 def parse():
@@ -278,9 +281,19 @@ def bench_batch(args):
         "back_to_back_launches_s": None if seq_s is None else round(seq_s, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None},
+        "truncated_queries": {"collision_budget": sum(c.counters()["ntrunc_coll"] for c in ctxs),
+                              "iterator_bound": sum(c.counters()["ntrunc_iter"] for c in ctxs)},
         "setup": setup,
         "cpu_baseline": None,
     }
+    tfile = os.path.join(ROOT, "profiles", PROFILED_BATCH)
+    if os.path.exists(tfile) and args.batch_scale == 1.0 and nsteps == 1:
+        with open(tfile) as f:
+            prof = json.load(f)
+        r = out["roofline"]
+        r["traffic"] = prof.get("bytes_per_launch")
+        for k in ("limiter", "valu_busy", "valu_lane_utilisation", "l2_hit_rate", "source"):
+            r[{"limiter": "measured_limiter", "source": "counters_source"}.get(k, k)] = prof.get(k)
     if not args.no_cpu:
         big = max(range(len(items)), key=lambda i: items[i][1].width * items[i][1].height)
         out["cpu_baseline"] = cpu_baseline(items[big][0], items[big][1], args.cpu_seconds)

@@ -1230,10 +1230,10 @@ __device__ __forceinline__ void run_scene(const SceneF& S0, const DevMap& M0, co
 #pragma unroll
         for(int a = 0; a < 3; ++a) x[a] = L.pos[a] + tx * L.dir[a];
         const float v = L.s_cloud ? seg_lwcf<COUNT>(S, L, x) : 0.f;
+        const float2 uu = draw2(L, key);
         const float gka = L.s_gka, gks = L.s_gks;
         const float ka = gka + L.cka * v;
         const float ks = gks + L.cks * v;
-        const float2 uu = draw2(L, key);
         if(is_a) {
           /* residual tracking of the absorption transmissivity: a real
            * collision makes it 0 and ends the walk (..._sw.c:396-400,
