@@ -1230,6 +1230,8 @@ __device__ __forceinline__ void run_scene(const SceneF& S0, const DevMap& M0, co
 #pragma unroll
         for(int a = 0; a < 3; ++a) x[a] = L.pos[a] + tx * L.dir[a];
         const float v = L.s_cloud ? seg_lwcf<COUNT>(S, L, x) : 0.f;
+        /* the draw does not depend on the LWC: written before the LWC's
+         * first use, its Threefry rounds run while the load is in flight */
         const float2 uu = draw2(L, key);
         const float gka = L.s_gka, gks = L.s_gks;
         const float ka = gka + L.cka * v;
