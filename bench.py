@@ -66,7 +66,7 @@ def parse():
                    help="device-time budget of the launch-parameter autotuner run before the "
                         "warm-up (FP32 solver, 4x for --config 3; 0: default parameters)")
     p.add_argument("--launch-params", default="",
-                   help="EV,CHUNK,GRID: fixed FP32 launch parameters instead of the autotuner "
+                   help="EV,CHUNK,GRID[,SEG]: fixed FP32 launch parameters instead of the autotuner "
                         "(0 = default for a field; profiles use the values the tuner picked)")
     p.add_argument("--batch-scale", type=float, default=1.0,
                    help="--config 3: scale of the realisations per pixel of every map")
@@ -396,8 +396,7 @@ def main():
     main_solver = F.SOLVER_WAVEFRONT_FP32 if args.solver == "wavefront" else F.SOLVER_PLAIN_FP64
     tuned = None
     if main_solver == F.SOLVER_WAVEFRONT_FP32 and args.launch_params:
-        ev, ch, gr = (int(x) for x in args.launch_params.split(","))
-        ctx.set_wavefront_params(ev, ch, gr)
+        ctx.set_wavefront_params(*(int(x) for x in args.launch_params.split(",")))
         tuned = {"params": ctx.wavefront_params(), "trials": 0, "fixed": True}
     elif main_solver == F.SOLVER_WAVEFRONT_FP32 and args.autotune_ms > 0:
         dt = A.map_preset(args.config)

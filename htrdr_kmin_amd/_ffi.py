@@ -110,7 +110,7 @@ class MapDesc(C.Structure):
 
 class WfParams(C.Structure):
     _fields_ = [("ev_thresh", C.c_uint32), ("chunk", C.c_uint32), ("grid", C.c_uint32),
-                ("grid_max", C.c_uint32)]
+                ("grid_max", C.c_uint32), ("seg_thresh", C.c_uint32)]
 
 
 class BatchItem(C.Structure):

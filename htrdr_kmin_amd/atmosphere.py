@@ -241,8 +241,8 @@ class Context:
         F.check("htgx_get_wavefront_params", self._lib.htgx_get_wavefront_params(self._h, C.byref(p)))
         return {k: getattr(p, k) for k, _ in F.WfParams._fields_}
 
-    def set_wavefront_params(self, ev_thresh=0, chunk=0, grid=0):
-        p = F.WfParams(ev_thresh, chunk, grid, 0)
+    def set_wavefront_params(self, ev_thresh=0, chunk=0, grid=0, seg_thresh=0):
+        p = F.WfParams(ev_thresh, chunk, grid, 0, seg_thresh)
         F.check("htgx_set_wavefront_params", self._lib.htgx_set_wavefront_params(self._h, C.byref(p)))
 
     def autotune(self, desc, budget_ms=2000.0):

@@ -76,9 +76,11 @@ struct SceneF {
   const float* gas_ka; /* [#quads][nlayers] */
   const float* gas_ks; /* [nbands][nlayers] */
   /* vertical optical depth of gas_ka / gas_ks from the bottom of the gas to
-   * the bottom of each layer, same layout (FP64 sums of the FP32 values) */
-  const double* cum_ka;
-  const double* cum_ks;
+   * the bottom of each layer, same layout (FP64 sums of the FP32 values,
+   * rounded to FP32: a run's depth is a difference of two of them, exact to
+   * about 1e-7 of the whole column) */
+  const float* cum_ka;
+  const float* cum_ks;
   const float* gas_T;
   int has_clouds;
   uint32_t B;
@@ -128,6 +130,7 @@ struct Launch {
   uint32_t xcd_tiles;    /* tiles per XCD range */
   uint32_t spp_offset;
   uint32_t ev_thresh;    /* min #lanes needing the EVENT branch */
+  uint32_t seg_thresh;   /* min #lanes needing the SEG step while others can collide */
   uint32_t max_steps;    /* micro-step budget of one query */
   unsigned int* counters; /* [kXcds] next chunk of each XCD range */
   htgx_mc_accum* partials; /* [ntiles*nsub*64] */
@@ -456,10 +459,11 @@ __device__ __forceinline__ void load_leaf(const SceneF& S, Lane& L, uint32_t cx,
     const uint32_t lo = (c[a] >> lvl) << lvl;
     const uint32_t hi = min(lo + (1u << lvl), S.dims0[a]);
     L.lohi[a] = lo | (hi << 16);
-    if(L.dir[a] != 0.f) {
-      const float ta = (planef(S, a, L.dir[a] > 0.f ? hi : lo) - L.pos[a]) * L.inv_dir[a];
-      if(ta < best) { best = ta; ax = a; }
-    }
+    const float ta0 = (planef(S, a, L.dir[a] > 0.f ? hi : lo) - L.pos[a]) * L.inv_dir[a];
+    const float ta = L.dir[a] != 0.f ? ta0 : INFINITY;
+    const bool better = ta < best;
+    best = better ? ta : best;
+    ax = better ? a : ax;
   }
   L.tnext = best;
   L.exit_axis = ax;
@@ -474,27 +478,36 @@ __device__ __forceinline__ void load_leaf(const SceneF& S, Lane& L, uint32_t cx,
 __device__ __forceinline__ bool next_cell(const SceneF& S, Lane& L, uint32_t c[3])
 {
   const bool step = L.cstate == 4;
+  bool out = false;
+  int oa = 0;
 #pragma unroll
   for(int a = 0; a < 3; ++a) {
     const uint32_t lo = step ? L.lohi[a] & 0xFFFFu : 0u;
     const uint32_t hi = step ? L.lohi[a] >> 16 : S.dims0[a];
-    if(step && a == L.exit_axis) {
-      const bool out = L.dir[a] > 0.f ? hi >= S.dims0[a] : lo == 0;
-      if(out) {
-        /* -r: into the next copy of the grid along X or Y; the lane's frame
-         * moves with it (a bounded number of copies per query) */
-        if(a == 2 || !S.repeat || ++LC(ntiles) > kMaxTilesF) return false;
-        const float dp = L.dir[a] > 0.f ? S.period[a] : -S.period[a];
+    const bool pos_dir = L.dir[a] > 0.f;
+    const bool across = step && a == L.exit_axis;
+    const bool out_a = across && (pos_dir ? hi >= S.dims0[a] : lo == 0);
+    const float p = L.pos[a] + L.tit * L.dir[a];
+    const float f = floorf((p - S.lower[a]) * S.inv_bs[a]);
+    const uint32_t cl = f < (float)lo ? lo : f > (float)(hi - 1) ? hi - 1 : (uint32_t)f;
+    const uint32_t cs = pos_dir ? hi : lo - 1;
+    c[a] = across ? cs : cl;
+    out = out || out_a;
+    oa = out_a ? a : oa;
+  }
+  if(out) {
+    /* -r: into the next copy of the grid along X or Y; the lane's frame
+     * moves with it (a bounded number of copies per query) */
+    if(oa == 2 || !S.repeat || ++LC(ntiles) > kMaxTilesF) return false;
+#pragma unroll
+    for(int a = 0; a < 2; ++a) {
+      if(a == oa) {
+        const bool pos_dir = L.dir[a] > 0.f;
+        const float dp = pos_dir ? S.period[a] : -S.period[a];
         L.pos[a] -= dp;
         LCA(shift, a) += dp;
-        c[a] = L.dir[a] > 0.f ? 0u : S.dims0[a] - 1;
-      } else {
-        c[a] = L.dir[a] > 0.f ? hi : lo - 1;
+        c[a] = pos_dir ? 0u : S.dims0[a] - 1;
       }
-    } else {
-      const float p = L.pos[a] + L.tit * L.dir[a];
-      const float f = floorf((p - S.lower[a]) * S.inv_bs[a]);
-      c[a] = f < (float)lo ? lo : f > (float)(hi - 1) ? hi - 1 : (uint32_t)f;
     }
   }
   return true;
@@ -570,14 +583,14 @@ __device__ __forceinline__ void iter_advance(Lane& L, float t1, bool adv_g, bool
  * into the gas), for the property prop of the lane's band and k-point: the
  * cumulative sum at the layer's bottom plus the layer's coefficient times the
  * height into it */
-__device__ __forceinline__ double gas_column(const SceneF& S, const Lane& L, int prop, float z)
+__device__ __forceinline__ float gas_column(const SceneF& S, const Lane& L, int prop, float z)
 {
   const float fj = floorf((z - S.zb) * S.inv_dzl);
   const uint32_t j = fj < 0.f ? 0u : fj > (float)(S.nlayers - 1) ? S.nlayers - 1 : (uint32_t)fj;
-  const double h = fmin(fmax((double)z - ((double)S.zb + (double)j * (double)S.dzl), 0.0), (double)S.dzl);
-  double c = 0.0;
-  if(prop != HTGX_Ks) c += S.cum_ka[L.gka_off + j] + (double)S.gas_ka[L.gka_off + j] * h;
-  if(prop != HTGX_Ka) c += S.cum_ks[L.gks_off + j] + (double)S.gas_ks[L.gks_off + j] * h;
+  const float h = fminf(fmaxf(z - (S.zb + (float)j * S.dzl), 0.f), S.dzl);
+  float c = 0.f;
+  if(prop != HTGX_Ks) c += S.cum_ka[L.gka_off + j] + S.gas_ka[L.gka_off + j] * h;
+  if(prop != HTGX_Ka) c += S.cum_ks[L.gks_off + j] + S.gas_ks[L.gks_off + j] * h;
   return c;
 }
 
@@ -616,10 +629,10 @@ __device__ __forceinline__ bool iter_next(const SceneF& S, Lane& L, int prop, fl
       const float tx = fmaxf((ze - L.pos[2]) * L.inv_dir[2], t);
       const float t1 = fminf(fminf(tc, tx), tmax);
       if(t1 > t) {
-        const double tau = fabs(gas_column(S, L, prop, L.pos[2] + t1 * L.dir[2])
+        const float tau = fabsf(gas_column(S, L, prop, L.pos[2] + t1 * L.dir[2])
                               - gas_column(S, L, prop, L.pos[2] + t * L.dir[2]))
-                         * fabs((double)L.inv_dir[2]);
-        const float k = (float)(tau / (double)(t1 - t));
+                        * fabsf(L.inv_dir[2]);
+        const float k = tau / (t1 - t);
         *t0 = t;
         L.s_t1 = t1;
         L.s_kmin = L.s_kmax = k;
@@ -1156,14 +1169,17 @@ __device__ __forceinline__ void run_scene(const SceneF& S0, const DevMap& M0, co
     if(!(busy0 | want0)) break;
     if(COUNT) tk = clock64();
 
-    /* next segment of the query, and the first tentative collision in it */
+    /* next segment of the query, and the first tentative collision in it:
+     * while some lanes can take a collision step, the segment step waits
+     * until seg_thresh lanes need it */
+    const uint64_t bs = __ballot(L.lstate == LS_SEG);
+    const bool run_seg = bs && (__popcll(bs) >= P.seg_thresh || !__ballot(L.lstate == LS_COLL));
     if(COUNT) {
-      const uint64_t bs = __ballot(L.lstate == LS_SEG);
       d_it++;
-      d_se += bs != 0;
-      d_sl += __popcll(bs);
+      d_se += run_seg;
+      d_sl += run_seg ? __popcll(bs) : 0;
     }
-    if(L.lstate == LS_SEG) {
+    if(run_seg && L.lstate == LS_SEG) {
       float t0;
       const int sprop = L.mode == MODE_SC ? HTGX_Ks : L.mode == MODE_LW ? HTGX_Kext : L.prop;
       if(iter_next<COUNT>(S, L, sprop, &t0, L.mode == MODE_TR)) {
@@ -1743,6 +1759,7 @@ struct htgx_wavefront_state {
   uint32_t grid_max = 0; /* resident blocks of the device */
   uint32_t K = 4;
   uint32_t ev_thresh = 24;
+  uint32_t seg_thresh = 0;
 };
 
 /* Device array freed with its owner */
@@ -1852,8 +1869,9 @@ wavefront_build(htgx_ctx* ctx, htgx_wavefront_state* st)
     };
     cumulate(ka, nq);
     cumulate(ks, D.nbands);
-    if((res = upload_f(st, ka, &S.cum_ka))) return res;
-    if((res = upload_f(st, ks, &S.cum_ks))) return res;
+    const std::vector<float> fka(ka.begin(), ka.end()), fks(ks.begin(), ks.end());
+    if((res = upload_f(st, fka, &S.cum_ka))) return res;
+    if((res = upload_f(st, fks, &S.cum_ks))) return res;
   }
   if(D.gas_T && (res = upload_as_float(st, D.gas_T, D.nlayers, &S.gas_T))) return res;
   S.has_clouds = D.has_clouds;
@@ -1937,6 +1955,7 @@ wavefront_build(htgx_ctx* ctx, htgx_wavefront_state* st)
   if(const char* e = std::getenv("HTGX_WF_GRID")) st->grid = (uint32_t)std::max(1, atoi(e));
   if(const char* e = std::getenv("HTGX_WF_EVTHRESH")) st->ev_thresh = (uint32_t)std::max(1, atoi(e));
   if(const char* e = std::getenv("HTGX_WF_CHUNK")) st->K = (uint32_t)std::min(16, std::max(1, atoi(e)));
+  if(const char* e = std::getenv("HTGX_WF_SEGTHRESH")) st->seg_thresh = (uint32_t)std::min(64, std::max(0, atoi(e)));
   return HTGX_OK;
 }
 
@@ -1958,6 +1977,7 @@ fill_launch(htgx_ctx* ctx, const DevMap& M, Launch* P)
   P->xcd_tiles = (P->ntiles + kXcds - 1) / kXcds;
   P->spp_offset = M.spp_offset;
   P->ev_thresh = st->ev_thresh;
+  P->seg_thresh = st->seg_thresh;
   P->max_steps = 1u << 20;
   const uint64_t nchunks = (uint64_t)P->ntiles * P->nsub * 64u;
   if(nchunks > 0xFFFFFFFFull) return HTGX_BAD_ARG;
@@ -2066,8 +2086,10 @@ int htgx_wavefront_params(htgx_ctx* ctx, htgx_wf_params* get, const htgx_wf_para
   if((res = wavefront_prepare(ctx)) != HTGX_OK) return res;
   htgx_wavefront_state* st = ctx->wf;
   if(set) {
-    if(set->ev_thresh > 64 || set->chunk > 16 || set->grid > st->grid_max) return HTGX_BAD_ARG;
+    if(set->ev_thresh > 64 || set->chunk > 16 || set->grid > st->grid_max || set->seg_thresh > 64)
+      return HTGX_BAD_ARG;
     st->ev_thresh = set->ev_thresh ? set->ev_thresh : 24;
+    st->seg_thresh = set->seg_thresh;
     st->K = set->chunk ? set->chunk : 4;
     st->grid = set->grid ? set->grid : st->grid_max;
   }
@@ -2076,6 +2098,7 @@ int htgx_wavefront_params(htgx_ctx* ctx, htgx_wf_params* get, const htgx_wf_para
     get->chunk = st->K;
     get->grid = st->grid;
     get->grid_max = st->grid_max;
+    get->seg_thresh = st->seg_thresh;
   }
   return HTGX_OK;
 }
@@ -2332,7 +2355,7 @@ struct TrialClock {
 };
 
 /* Coordinate descent over the chunk size, then the event threshold, then the
- * grid. trial(p, &ms) runs one timed launch with parameters p; each
+ * segment threshold, then the grid. trial(p, &ms) runs one timed launch with parameters p; each
  * candidate is timed twice and keeps its faster time (single launches vary
  * by 1-2%), and replaces the current parameters only when more than 1%
  * faster. The first launch is an untimed warm-up; the search stops before a
@@ -2342,6 +2365,7 @@ static int tune_search(Trial&& trial, double budget_ms, uint32_t grid_max, htgx_
                        double* cur_ms, uint32_t* ntrials)
 {
   static const uint32_t thresholds[] = {8, 16, 24, 32, 40, 48};
+  static const uint32_t seg_thresholds[] = {0, 16, 24};
   static const uint32_t chunks[] = {2, 4, 8};
   const uint32_t grids[] = {grid_max, std::max(1u, grid_max * 3 / 4), std::max(1u, grid_max / 2)};
   double ms = 0, ms2 = 0;
@@ -2353,17 +2377,20 @@ static int tune_search(Trial&& trial, double budget_ms, uint32_t grid_max, htgx_
   double spent = 2 * ms, last = 2 * ms;
   uint32_t n = 1;
   *cur_ms = ms;
-  for(int dim = 0; dim < 3; ++dim) {
-    const uint32_t* vals = dim == 0 ? chunks : dim == 1 ? thresholds : grids;
+  for(int dim = 0; dim < 4; ++dim) {
+    const uint32_t* vals = dim == 0 ? chunks : dim == 1 ? thresholds : dim == 2 ? seg_thresholds : grids;
     const int nv = dim == 1 ? 6 : 3;
     const htgx_wf_params base = *cur;
     for(int i = 0; i < nv; ++i) {
       htgx_wf_params p = base;
       if(dim == 0) p.chunk = vals[i];
       else if(dim == 1) p.ev_thresh = vals[i];
+      else if(dim == 2) p.seg_thresh = vals[i];
       else p.grid = vals[i];
-      if(p.ev_thresh == base.ev_thresh && p.chunk == base.chunk && p.grid == base.grid) continue;
-      if(spent + last > budget_ms) { dim = 3; break; }
+      if(p.ev_thresh == base.ev_thresh && p.chunk == base.chunk && p.grid == base.grid &&
+         p.seg_thresh == base.seg_thresh)
+        continue;
+      if(spent + last > budget_ms) { dim = 4; break; }
       if((res = trial(p, &ms)) != HTGX_OK) return res;
       if((res = trial(p, &ms2)) != HTGX_OK) return res;
       spent += ms + ms2;
@@ -2380,6 +2407,7 @@ static int tune_search(Trial&& trial, double budget_ms, uint32_t grid_max, htgx_
 static void set_params(htgx_wavefront_state* st, const htgx_wf_params& p)
 {
   st->ev_thresh = p.ev_thresh;
+  st->seg_thresh = p.seg_thresh;
   st->K = p.chunk;
   st->grid = p.grid;
 }
@@ -2396,7 +2424,7 @@ int htgx_wavefront_autotune(htgx_ctx* ctx, const DevMap& M, double budget_ms,
   if((res = clk.init()) || (res = acc.alloc(npix))) return res;
   const int counting = ctx->counting;
   ctx->counting = 0;
-  htgx_wf_params cur = {st->ev_thresh, st->K, st->grid, st->grid_max};
+  htgx_wf_params cur = {st->ev_thresh, st->K, st->grid, st->grid_max, st->seg_thresh};
   double ms = 0;
   uint32_t n = 0;
   res = tune_search([&](const htgx_wf_params& p, double* t) {
@@ -2435,7 +2463,7 @@ int htgx_wavefront_batch_autotune(htgx_ctx* const* ctxs, const DevMap* Ms, uint3
   TrialClock clk;
   if((res = clk.init())) return res;
   htgx_wavefront_state* st0 = ctxs[0]->wf;
-  htgx_wf_params cur = {st0->ev_thresh, st0->K, st0->grid, grid_max};
+  htgx_wf_params cur = {st0->ev_thresh, st0->K, st0->grid, grid_max, st0->seg_thresh};
   double ms = 0;
   uint32_t nt = 0;
   res = tune_search([&](const htgx_wf_params& p, double* t) {
@@ -2490,6 +2518,7 @@ int htgx_debug_realisations_fp32(htgx_ctx* ctx, const htgx_map_desc* desc, const
     P.K = 1;
     P.nsub = 1;
     P.ev_thresh = st->ev_thresh;
+    P.seg_thresh = st->seg_thresh;
     P.max_steps = 1u << 20;
     P.counters = st->d_counters;
     P.partials = d_acc.p;

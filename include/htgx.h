@@ -418,6 +418,10 @@ struct htgx_wf_params {
   uint32_t grid;      /* blocks of 256 lanes of the persistent launch
                          (1-grid_max; 0 = grid_max) */
   uint32_t grid_max;  /* read only: blocks resident on the device at once */
+  uint32_t seg_thresh; /* lanes of a 64-lane wavefront that must wait for the
+                          segment step before it runs while other lanes can
+                          take a collision step (0-64; 0 = run it whenever a
+                          lane needs it) */
 };
 
 int htgx_get_wavefront_params(struct htgx_ctx* ctx, struct htgx_wf_params* params);
@@ -426,7 +430,8 @@ int htgx_set_wavefront_params(struct htgx_ctx* ctx, const struct htgx_wf_params*
 /* Autotuner of the parameters above for a map: trial launches of the map as
  * described (desc->spp realisations per pixel: pass the launch size that
  * will be used), timed on the device, by coordinate descent (chunk size,
- * then event threshold, then grid) within about budget_ms of device time;
+ * then event threshold, then segment threshold, then grid) within about
+ * budget_ms of device time;
  * each candidate is timed twice, and must be more than 1% faster to be
  * taken. The best
  * parameters are kept by the context for the maps that follow; best, rate

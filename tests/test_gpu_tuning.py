@@ -42,8 +42,10 @@ def test_defaults_and_bad_values(scene):
     ctx.set_wavefront_params()
     p = ctx.wavefront_params()
     assert p["ev_thresh"] == 24 and p["chunk"] == 4 and p["grid"] == p["grid_max"] > 0
+    assert p["seg_thresh"] == 0
     lib = F.load()
-    for bad in (F.WfParams(65, 0, 0, 0), F.WfParams(0, 17, 0, 0), F.WfParams(0, 0, p["grid_max"] + 1, 0)):
+    for bad in (F.WfParams(65, 0, 0, 0), F.WfParams(0, 17, 0, 0), F.WfParams(0, 0, p["grid_max"] + 1, 0),
+                F.WfParams(0, 0, 0, 0, 65)):
         assert lib.htgx_set_wavefront_params(ctx.handle, C.byref(bad)) == F.BAD_ARG
     assert ctx.wavefront_params() == p
 
@@ -54,11 +56,11 @@ def test_schedule_does_not_change_results(scene):
     ctx.set_wavefront_params()
     ref = render(ctx, d)
     gmax = ctx.wavefront_params()["grid_max"]
-    for ev, grid in ((1, 0), (64, 7), (8, gmax // 2)):
-        ctx.set_wavefront_params(ev_thresh=ev, grid=grid)
+    for ev, grid, seg in ((1, 0, 0), (64, 7, 0), (8, gmax // 2, 0), (24, 0, 16), (1, 0, 64)):
+        ctx.set_wavefront_params(ev_thresh=ev, grid=grid, seg_thresh=seg)
         got = render(ctx, d)
         for k in EXACT:
-            assert np.array_equal(got[k], ref[k]), (ev, grid, k)
+            assert np.array_equal(got[k], ref[k]), (ev, grid, seg, k)
     ctx.set_wavefront_params(chunk=16)
     got = render(ctx, d)
     assert np.array_equal(got["n"], ref["n"]) and np.array_equal(got["nfail"], ref["nfail"])
@@ -80,6 +82,7 @@ def test_autotune(scene):
     best, rate, n = ctx.autotune(d, budget_ms=5000.0)
     assert n >= 2 and rate > 0
     assert best["ev_thresh"] in (8, 16, 24, 32, 40, 48) and best["chunk"] in (2, 4, 8)
+    assert best["seg_thresh"] in (0, 16, 24)
     assert 1 <= best["grid"] <= best["grid_max"]
     assert ctx.wavefront_params() == best  # kept for the maps that follow
     # the tuned context renders the same realisations
