@@ -213,7 +213,12 @@ def bench_batch(args):
     step_descs = [copy_desc(d, spp=d.spp // nsteps) for _, d in items]
     step_items = [(c, d, b.address) for c, d, b in zip(ctxs, step_descs, bufs)]
     tuned = None
-    if args.autotune_ms > 0:
+    if args.launch_params:
+        vals = [int(x) for x in args.launch_params.split(",")]
+        for c in ctxs:
+            c.set_wavefront_params(*vals)
+        tuned = {"params": ctxs[0].wavefront_params(), "trials": 0, "fixed": True}
+    elif args.autotune_ms > 0:
         # trials: the whole batch at 1/8 of each map's realisations per pixel
         budget = 4.0 * args.autotune_ms
         params, rate, ntrials = A.batch_autotune(step_items, budget, spp_div=8)
