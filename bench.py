@@ -113,8 +113,10 @@ def cpu_baseline(synth, desc, target_s):
         setattr(d, f, getattr(desc, f))
     d.spp = 2
     d.spp_offset = 0
-    # calibration on 64 pixels, then a sample sized for about target_s
-    calib = np.arange(0, npix, npix // 64, dtype=np.uint64)[:64]
+    # calibration on 1024 pixels (enough work to keep every thread busy),
+    # then a sample sized for about target_s
+    nc = min(npix, 1024)
+    calib = np.arange(0, npix, max(1, npix // nc), dtype=np.uint64)[:nc]
     t0 = time.perf_counter()
     sky.draw_pixels(d, calib, threads)
     rate = len(calib) * d.spp / max(time.perf_counter() - t0, 1e-6)
