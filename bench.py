@@ -77,6 +77,9 @@ def parse():
                         "realisations per pixel (one step is fastest: no barrier between steps)")
     p.add_argument("--batch-no-graph", action="store_true",
                    help="--config 3: launch the steps directly instead of replaying a HIP graph")
+    p.add_argument("--no-graph", action="store_true",
+                   help="FP32 solver: launch each timed step directly instead of replaying the "
+                        "HIP graph of a one-map plan")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                    help="only run 5 plain device copies of MIB MiB (profiler calibration)")
     return p.parse_args()
@@ -373,12 +376,25 @@ def main():
             step(w)
             stream.synchronize()
         acc.zero()
+        plan = None
+        if solver == F.SOLVER_WAVEFRONT_FP32 and not args.no_graph:
+            # the timed steps replay the HIP graph of a one-map plan: zeroed
+            # chunk queues, the persistent kernel, the chunk merge and the
+            # move to the next slice of realisations
+            dp = A.map_preset(args.config)
+            for f, _ in dp._fields_:
+                setattr(dp, f, getattr(d, f))
+            dp.spp, dp.spp_offset = S, 0
+            plan = A.BatchPlan([(ctx, dp, acc.address)], use_graph=True)
         ev = [(H.Event(), H.Event()) for _ in range(nsteps)]
         H.synchronize()
         t0 = time.perf_counter()
         for k in range(nsteps):
             ev[k][0].record(stream)
-            step(k)
+            if plan is not None:
+                plan.step(stream.h.value)
+            else:
+                step(k)
             ev[k][1].record(stream)
             ev[k][1].synchronize()
             print(f"[bench] {SOLVER_NAMES[solver]}: step {k}: {ev[k][0].elapsed_ms(ev[k][1]):.1f} ms",
@@ -387,6 +403,8 @@ def main():
         H.synchronize()
         wall = time.perf_counter() - t0
         total_s = sum(a.elapsed_ms(b) for a, b in ev) / 1e3
+        if plan is not None:
+            plan.close()
         h = acc.download(np.zeros(npix, A.ACCUM_DTYPE))
         return total_s, wall, h, counters
 
@@ -458,6 +476,9 @@ def main():
             "solver": "FP32 batched wavefront" if args.solver == "wavefront" else "FP64 plain one-path-per-thread",
             "config_index": args.config,
             "spp_per_step": S,
+            "step_launch": ("HIP graph replay of a one-map plan (queue reset, persistent kernel, "
+                            "chunk merge, offset advance)"
+                            if args.solver == "wavefront" and not args.no_graph else "direct launches"),
             "wall_s": round(wall, 3),
             "autotune": tuned,
         },

@@ -40,6 +40,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <cstddef>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -76,11 +77,11 @@ struct SceneF {
   const float* gas_ka; /* [#quads][nlayers] */
   const float* gas_ks; /* [nbands][nlayers] */
   /* vertical optical depth of gas_ka / gas_ks from the bottom of the gas to
-   * the bottom of each layer, same layout (FP64 sums of the FP32 values,
-   * rounded to FP32: a run's depth is a difference of two of them, exact to
+   * the bottom of each layer, same layout (FP64 sums of the FP32 values: a
+   * run's depth is a difference of two of them, which FP32 would round to
    * about 1e-7 of the whole column) */
-  const float* cum_ka;
-  const float* cum_ks;
+  const double* cum_ka;
+  const double* cum_ks;
   const float* gas_T;
   int has_clouds;
   uint32_t B;
@@ -148,6 +149,10 @@ struct Launch {
   /* list launches only (null otherwise): nvalid (pixel, realisation) pairs,
    * one realisation each, whose accumulators go to partials[0..nvalid) */
   const uint32_t* list;
+  /* single-map plans replayed from a HIP graph (null otherwise): the
+   * realisation offset of the step, read when the launch starts instead of
+   * spp_offset and advanced on the device after each step */
+  const uint32_t* spp_offset_dev;
 };
 
 enum { LS_SEG = 0, LS_COLL = 1, LS_EVENT = 2, LS_DONE = 3 };
@@ -583,14 +588,14 @@ __device__ __forceinline__ void iter_advance(Lane& L, float t1, bool adv_g, bool
  * into the gas), for the property prop of the lane's band and k-point: the
  * cumulative sum at the layer's bottom plus the layer's coefficient times the
  * height into it */
-__device__ __forceinline__ float gas_column(const SceneF& S, const Lane& L, int prop, float z)
+__device__ __forceinline__ double gas_column(const SceneF& S, const Lane& L, int prop, float z)
 {
   const float fj = floorf((z - S.zb) * S.inv_dzl);
   const uint32_t j = fj < 0.f ? 0u : fj > (float)(S.nlayers - 1) ? S.nlayers - 1 : (uint32_t)fj;
   const float h = fminf(fmaxf(z - (S.zb + (float)j * S.dzl), 0.f), S.dzl);
-  float c = 0.f;
-  if(prop != HTGX_Ks) c += S.cum_ka[L.gka_off + j] + S.gas_ka[L.gka_off + j] * h;
-  if(prop != HTGX_Ka) c += S.cum_ks[L.gks_off + j] + S.gas_ks[L.gks_off + j] * h;
+  double c = 0.0;
+  if(prop != HTGX_Ks) c += S.cum_ka[L.gka_off + j] + (double)(S.gas_ka[L.gka_off + j] * h);
+  if(prop != HTGX_Ka) c += S.cum_ks[L.gks_off + j] + (double)(S.gas_ks[L.gks_off + j] * h);
   return c;
 }
 
@@ -629,10 +634,10 @@ __device__ __forceinline__ bool iter_next(const SceneF& S, Lane& L, int prop, fl
       const float tx = fmaxf((ze - L.pos[2]) * L.inv_dir[2], t);
       const float t1 = fminf(fminf(tc, tx), tmax);
       if(t1 > t) {
-        const float tau = fabsf(gas_column(S, L, prop, L.pos[2] + t1 * L.dir[2])
-                              - gas_column(S, L, prop, L.pos[2] + t * L.dir[2]))
-                        * fabsf(L.inv_dir[2]);
-        const float k = tau / (t1 - t);
+        /* the difference of the two columns in FP64, the rest in FP32 */
+        const float dcol = (float)fabs(gas_column(S, L, prop, L.pos[2] + t1 * L.dir[2])
+                                     - gas_column(S, L, prop, L.pos[2] + t * L.dir[2]));
+        const float k = dcol * fabsf(L.inv_dir[2]) / (t1 - t);
         *t0 = t;
         L.s_t1 = t1;
         L.s_kmin = L.s_kmax = k;
@@ -1156,6 +1161,7 @@ __device__ __forceinline__ void run_scene(const SceneF& S0, const DevMap& M0, co
   L.pixel = L.r = 0;
   const uint32_t xcd = blockIdx.x % (uint32_t)kXcds;
   const uint32_t key = (uint32_t)M.seed ^ (uint32_t)(M.seed >> 32);
+  const uint32_t spp_offset = P.spp_offset_dev ? *P.spp_offset_dev : P.spp_offset;
   uint64_t d_it = 0, d_ce = 0, d_cl = 0, d_se = 0, d_sl = 0, d_ee = 0, d_el = 0;
   /* shader clock cycles per phase (counting build only) */
   uint64_t cy[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -1335,7 +1341,7 @@ __device__ __forceinline__ void run_scene(const SceneF& S0, const DevMap& M0, co
             const uint32_t y = (t / P.ntx) * 8u + (((m >> 1) & 1u) | ((m >> 2) & 2u) | ((m >> 3) & 4u));
             if(x < P.width && y < P.height) {
               L.pixel = y * P.width + x;
-              L.r = P.spp_offset + s * P.K;
+              L.r = spp_offset + s * P.K;
               LC(r_end) = L.r + P.K;
               LC(n) = LC(nfail) = 0;
               LC(sum) = LC(sum2) = 0.0;
@@ -1869,9 +1875,8 @@ wavefront_build(htgx_ctx* ctx, htgx_wavefront_state* st)
     };
     cumulate(ka, nq);
     cumulate(ks, D.nbands);
-    const std::vector<float> fka(ka.begin(), ka.end()), fks(ks.begin(), ks.end());
-    if((res = upload_f(st, fka, &S.cum_ka))) return res;
-    if((res = upload_f(st, fks, &S.cum_ks))) return res;
+    if((res = upload_f(st, ka, &S.cum_ka))) return res;
+    if((res = upload_f(st, ks, &S.cum_ks))) return res;
   }
   if(D.gas_T && (res = upload_as_float(st, D.gas_T, D.nlayers, &S.gas_T))) return res;
   S.has_clouds = D.has_clouds;
@@ -2109,6 +2114,9 @@ int htgx_wavefront_params(htgx_ctx* ctx, htgx_wf_params* get, const htgx_wf_para
 struct htgx_batch_plan {
   int device = 0;
   uint32_t n = 0;
+  /* one map: its own single-map kernel instead of the batch kernel (the
+   * same realisations, without the scene table's scalar reloads) */
+  bool single = false;
   int sensor = 0, spectral = 0;
   int feat = 0;       /* kernel features the maps need, together */
   uint32_t grid = 0;
@@ -2173,7 +2181,11 @@ static int plan_launch_step(htgx_batch_plan* p, hipStream_t stream)
   zero_u32_kernel<<<(2 * n + 255) / 256, 256, 0, stream>>>(p->flags, 2 * n);
   batch_tick_kernel<<<1, 1, 0, stream>>>(p->ticks + n, p->flags + 2 * n);
   HIP_OK(hipGetLastError());
-  switch(p->feat) {
+  if(p->single) {
+    const BatchScene& e = p->table_h[0];
+    int res;
+    if((res = launch<false>(e.S, e.M, e.P, p->grid, stream)) != HTGX_OK) return res;
+  } else switch(p->feat) {
     case 0: launch_batch<0>(p, stream); break;
     case kFeatRare: launch_batch<kFeatRare>(p, stream); break;
     case kFeatMesh: launch_batch<kFeatMesh>(p, stream); break;
@@ -2247,6 +2259,15 @@ int wf_plan_build(htgx_ctx* const* ctxs, const DevMap* Ms, htgx_mc_accum* const*
     p->maxpix = std::max(p->maxpix, Ms[i].width * Ms[i].height);
     p->feat |= features(e.S, e.M);
     p->us_per_tick[k] = ctxs[i]->us_per_tick;
+  }
+  if(n == 1) {
+    /* the single-map kernel takes its arguments by value, so a graph replays
+     * them unchanged: the step's realisation offset is read from the table,
+     * where batch_advance_kernel moves it on after each step */
+    p->single = true;
+    BatchScene& e = p->table_h[0];
+    e.P.spp_offset_dev = (const uint32_t*)((const char*)p->table + offsetof(BatchScene, P) +
+                                           offsetof(Launch, spp_offset));
   }
   HIP_OK(hipMemcpy(p->table, p->table_h.data(), n * sizeof(BatchScene), hipMemcpyHostToDevice));
   HIP_OK(hipMemset(p->flags, 0, (2 * n + 1) * sizeof(unsigned int)));

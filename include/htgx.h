@@ -472,7 +472,9 @@ int htgx_render_batch
  * reference). Step s renders, for each item, the realisations
  * [desc->spp_offset + s*spp, desc->spp_offset + (s+1)*spp) of every pixel
  * and accumulates them into the item's accumulators: bit-identical to
- * htgx_render_accum with that spp_offset. A step is six launches on
+ * htgx_render_accum with that spp_offset. A plan of one item runs the
+ * single-map kernel of htgx_render_accum (it reads the step's offset from
+ * the plan's table), more items the batch kernel. A step is six launches on
  * `stream', with no allocation, host copy or synchronisation; with use_graph
  * they are recorded once, at creation, as a HIP graph that each step
  * replays. The plan keeps its own copy of the descriptors, chunk queues and

@@ -62,12 +62,16 @@ def test_batch_bad_arguments(batch):
     assert lib.htgx_render_batch(arr, 0, None, None) == F.BAD_ARG
 
 
+@pytest.mark.parametrize("nmaps", [6, 1])
 @pytest.mark.parametrize("use_graph", [True, False])
-def test_plan_steps_equal_sliced_launches(batch, use_graph):
+def test_plan_steps_equal_sliced_launches(batch, use_graph, nmaps):
     """Each step of a plan (a replayed HIP graph, or direct launches) equals
     htgx_render_accum of the next spp slice, after every step, with other
-    launches of the same contexts between the steps."""
+    launches of the same contexts between the steps. A plan of one map runs
+    the single-map kernel, whose graph reads each step's realisation offset
+    from the plan's table."""
     items, ctxs = batch
+    items, ctxs = items[:nmaps], ctxs[:nmaps]
     npx = [d.width * d.height for _, d in items]
     plan_bufs = [H.DeviceBuffer(n * A.ACCUM_DTYPE.itemsize) for n in npx]
     ref_bufs = [H.DeviceBuffer(n * A.ACCUM_DTYPE.itemsize) for n in npx]
@@ -84,6 +88,13 @@ def test_plan_steps_equal_sliced_launches(batch, use_graph):
                 c.render_accum(d, b.address)
                 d.spp_offset = 0
             H.synchronize()
+            if nmaps == 1:
+                # eager work of other contexts between two replays
+                for c, (_, d) in zip(batch[1][1:3], batch[0][1:3]):
+                    tmp = H.DeviceBuffer(d.width * d.height * A.ACCUM_DTYPE.itemsize)
+                    tmp.zero()
+                    c.render_accum(d, tmp.address)
+                H.synchronize()
             for (_, d), n, pb, rb in zip(items, npx, plan_bufs, ref_bufs):
                 got = pb.download(np.zeros(n, A.ACCUM_DTYPE))
                 ref = rb.download(np.zeros(n, A.ACCUM_DTYPE))
