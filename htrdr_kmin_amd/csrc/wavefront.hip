@@ -584,19 +584,29 @@ __device__ __forceinline__ void iter_advance(Lane& L, float t1, bool adv_g, bool
   if(at_end) L.itdone = true;
 }
 
-/* Vertical optical depth of the gas from its bottom up to height z (clamped
- * into the gas), for the property prop of the lane's band and k-point: the
- * cumulative sum at the layer's bottom plus the layer's coefficient times the
- * height into it */
-__device__ __forceinline__ double gas_column(const SceneF& S, const Lane& L, int prop, float z)
+/* Vertical optical depth of the gas between heights za and zb (clamped into
+ * the gas), for the property prop of the lane's band and k-point: at each
+ * height, the cumulative column at its layer's bottom plus the layer's
+ * coefficient times the height into it. Only the difference of the two
+ * cumulative columns is taken in FP64 (the long sum that FP32 would round);
+ * the parts inside the two layers are FP32 */
+__device__ __forceinline__ float gas_depth(const SceneF& S, const Lane& L, int prop, float za, float zb)
 {
-  const float fj = floorf((z - S.zb) * S.inv_dzl);
-  const uint32_t j = fj < 0.f ? 0u : fj > (float)(S.nlayers - 1) ? S.nlayers - 1 : (uint32_t)fj;
-  const float h = fminf(fmaxf(z - (S.zb + (float)j * S.dzl), 0.f), S.dzl);
-  double c = 0.0;
-  if(prop != HTGX_Ks) c += S.cum_ka[L.gka_off + j] + (double)(S.gas_ka[L.gka_off + j] * h);
-  if(prop != HTGX_Ka) c += S.cum_ks[L.gks_off + j] + (double)(S.gas_ks[L.gks_off + j] * h);
-  return c;
+  const float fa = floorf((za - S.zb) * S.inv_dzl), fb = floorf((zb - S.zb) * S.inv_dzl);
+  const float top = (float)(S.nlayers - 1);
+  const uint32_t ja = (uint32_t)fminf(fmaxf(fa, 0.f), top), jb = (uint32_t)fminf(fmaxf(fb, 0.f), top);
+  const float ha = fminf(fmaxf(za - (S.zb + (float)ja * S.dzl), 0.f), S.dzl);
+  const float hb = fminf(fmaxf(zb - (S.zb + (float)jb * S.dzl), 0.f), S.dzl);
+  float d = 0.f;
+  if(prop != HTGX_Ks) {
+    const uint32_t o = L.gka_off;
+    d += (float)(S.cum_ka[o + jb] - S.cum_ka[o + ja]) + (S.gas_ka[o + jb] * hb - S.gas_ka[o + ja] * ha);
+  }
+  if(prop != HTGX_Ka) {
+    const uint32_t o = L.gks_off;
+    d += (float)(S.cum_ks[o + jb] - S.cum_ks[o + ja]) + (S.gas_ks[o + jb] * hb - S.gas_ks[o + ja] * ha);
+  }
+  return fabsf(d);
 }
 
 /* Next segment holding some medium into L.s_*; false at the end of the range.
@@ -605,7 +615,7 @@ __device__ __forceinline__ double gas_column(const SceneF& S, const Lane& L, int
  * segment over all the gas layers it crosses, up to the grid's entry, the
  * range's end or the edge of the gas, whose k_min = k_max is its mean: its
  * length times that is the stretch's optical depth, from the cumulative
- * column of the gas (gas_column) instead of a sum over its layers.
+ * column of the gas (gas_depth) instead of a sum over its layers.
  * The iterator only moves past an emitted segment at the next call, so that
  * the leaf range used to clamp the cells of its collisions stays the one of
  * the segment. Each pass of the loop loads at most one leaf, at one site. */
@@ -634,9 +644,7 @@ __device__ __forceinline__ bool iter_next(const SceneF& S, Lane& L, int prop, fl
       const float tx = fmaxf((ze - L.pos[2]) * L.inv_dir[2], t);
       const float t1 = fminf(fminf(tc, tx), tmax);
       if(t1 > t) {
-        /* the difference of the two columns in FP64, the rest in FP32 */
-        const float dcol = (float)fabs(gas_column(S, L, prop, L.pos[2] + t1 * L.dir[2])
-                                     - gas_column(S, L, prop, L.pos[2] + t * L.dir[2]));
+        const float dcol = gas_depth(S, L, prop, L.pos[2] + t * L.dir[2], L.pos[2] + t1 * L.dir[2]);
         const float k = dcol * fabsf(L.inv_dir[2]) / (t1 - t);
         *t0 = t;
         L.s_t1 = t1;
@@ -1161,7 +1169,6 @@ __device__ __forceinline__ void run_scene(const SceneF& S0, const DevMap& M0, co
   L.pixel = L.r = 0;
   const uint32_t xcd = blockIdx.x % (uint32_t)kXcds;
   const uint32_t key = (uint32_t)M.seed ^ (uint32_t)(M.seed >> 32);
-  const uint32_t spp_offset = P.spp_offset_dev ? *P.spp_offset_dev : P.spp_offset;
   uint64_t d_it = 0, d_ce = 0, d_cl = 0, d_se = 0, d_sl = 0, d_ee = 0, d_el = 0;
   /* shader clock cycles per phase (counting build only) */
   uint64_t cy[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -1341,7 +1348,7 @@ __device__ __forceinline__ void run_scene(const SceneF& S0, const DevMap& M0, co
             const uint32_t y = (t / P.ntx) * 8u + (((m >> 1) & 1u) | ((m >> 2) & 2u) | ((m >> 3) & 4u));
             if(x < P.width && y < P.height) {
               L.pixel = y * P.width + x;
-              L.r = spp_offset + s * P.K;
+              L.r = (P.spp_offset_dev ? *P.spp_offset_dev : P.spp_offset) + s * P.K;
               LC(r_end) = L.r + P.K;
               LC(n) = LC(nfail) = 0;
               LC(sum) = LC(sum2) = 0.0;
