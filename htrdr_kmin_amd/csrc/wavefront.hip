@@ -177,16 +177,20 @@ struct Lane {
   /* realisation */
   uint32_t gka_off, gks_off; /* rows of gas ka (band, k-point) and ks (band) */
   float cka, cks;       /* particle absorption / scattering per LWC of the band */
+  /* flags, one bit each in one word (bools would each hold a 64-bit lane
+   * mask in scalar registers across the loop) */
+  uint32_t surface : 1, rayleigh : 1, gh_hit : 1, pending : 1; /* walk */
+  uint32_t prev_edge : 1, gh_edge : 1;
+  uint32_t nq : 1;        /* a query is requested */
+  uint32_t lazy : 1;      /* the range ends at the mesh's top, not yet cast */
+  uint32_t itdone : 1, adv : 1, adv_g : 1, adv_c : 1, adv_end : 1; /* segment iterator */
+  uint32_t s_cloud : 1;   /* the current segment is in the cloud grid */
   /* walk */
-  bool surface, rayleigh, gh_hit, pending;
-  bool prev_edge, gh_edge;
   float pos[3], dir[3];
   /* query */
   int mode, prop;
-  bool nq;              /* a query is requested */
   int nq_ground;        /* its range comes from a ground cast (1), or the query
                            resumes after a late cast (2) */
-  bool lazy;            /* the range ends at the mesh's top, not yet cast */
   float qr;             /* end of the query range */
   float inv_dir[3];
   float tc, Ts, maj, inv_maj;   /* next tentative collision (INFINITY: none in the segment) */
@@ -209,10 +213,8 @@ struct Lane {
   float lmn, lmx;       /* LWC bounds of the current leaf */
   int exit_axis;
   int32_t j;
-  bool itdone, adv, adv_g, adv_c, adv_end;
   /* current segment */
   float s_t1, s_kmin, s_kmax, s_gka, s_gks; /* gas ka / ks of the segment's layer */
-  bool s_cloud;
   int32_t s_layer;
   /* counters */
   uint32_t c_leaf, c_seg, c_coll, c_lwc, c_ev, c_bvh, c_tri;
