@@ -172,7 +172,6 @@ enum {
 struct Lane {
   /* work */
   uint32_t pixel, r;    /* key of the realisation's random stream */
-  int lstate;
   uint32_t pair;        /* next Threefry-2x32 block of the realisation */
   /* realisation */
   uint32_t gka_off, gks_off; /* rows of gas ka (band, k-point) and ks (band) */
@@ -185,12 +184,18 @@ struct Lane {
   uint32_t lazy : 1;      /* the range ends at the mesh's top, not yet cast */
   uint32_t itdone : 1, adv : 1, adv_g : 1, adv_c : 1, adv_end : 1; /* segment iterator */
   uint32_t s_cloud : 1;   /* the current segment is in the cloud grid */
+  /* small states, bit fields of the same word */
+  uint32_t lstate : 2;    /* LS_* */
+  uint32_t mode : 2, prop : 2; /* the query's MODE_* and property */
+  uint32_t nq_ground : 2; /* its range comes from a ground cast (1), or the
+                             query resumes after a late cast (2) */
+  uint32_t ev : 2;        /* EV_* of the query's end */
+  uint32_t cstate : 3;    /* iterator: 0 before grid, 1 in leaf, 2 outside,
+                             3 locate at tit, 4 step out of leaf */
+  uint32_t exit_axis : 2; /* the current leaf's exit face */
   /* walk */
   float pos[3], dir[3];
   /* query */
-  int mode, prop;
-  int nq_ground;        /* its range comes from a ground cast (1), or the query
-                           resumes after a late cast (2) */
   float qr;             /* end of the query range */
   float inv_dir[3];
   float tc, Ts, maj, inv_maj;   /* next tentative collision (INFINITY: none in the segment) */
@@ -201,17 +206,14 @@ struct Lane {
    * (INFINITY: none), maj_a the segment's residual majorant */
   float Ts_a, tc_a, maj_a;
   double Tmin;
-  int ev;
   /* two counts of the current query: tentative collisions in the low
    * kCollBits bits, and above them the passes of the segment iterator that
    * emitted nothing since its last segment */
   uint32_t nsteps;
   /* segment iterator */
-  int cstate;          /* 0 before grid, 1 in leaf, 2 outside, 3 locate at tit, 4 step out of leaf */
   float tit, tnext;    /* iterator position; grid entry or leaf exit */
   uint32_t lohi[3];    /* leaf block range lo | hi << 16 */
   float lmn, lmx;       /* LWC bounds of the current leaf */
-  int exit_axis;
   int32_t j;
   /* current segment */
   float s_t1, s_kmin, s_kmax, s_gka, s_gks; /* gas ka / ks of the segment's layer */
