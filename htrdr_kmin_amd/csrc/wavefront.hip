@@ -150,7 +150,7 @@ struct Launch {
    * one realisation each, whose accumulators go to partials[0..nvalid) */
   const uint32_t* list;
   /* single-map plans replayed from a HIP graph (null otherwise): the
-   * realisation offset of the step, read when the launch starts instead of
+   * realisation offset of the step, read at each chunk fetch instead of
    * spp_offset and advanced on the device after each step */
   const uint32_t* spp_offset_dev;
 };
