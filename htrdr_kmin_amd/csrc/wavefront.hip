@@ -155,6 +155,15 @@ struct Launch {
   const uint32_t* spp_offset_dev;
 };
 
+/* The arguments of the single-map kernels, one struct: the first argument
+ * lies at offset 0 of the kernel argument segment, so its fields can be
+ * read from there at any point of the kernel */
+struct KArgs {
+  SceneF S;
+  DevMap M;
+  Launch P;
+};
+
 enum { LS_SEG = 0, LS_COLL = 1, LS_EVENT = 2, LS_DONE = 3 };
 enum { MODE_TR = 0, MODE_SC = 1, MODE_LW = 2 };
 enum { EV_NONE = 0, EV_HIT = 1, EV_ABS = 2, EV_SCA = 3 };
@@ -1141,16 +1150,44 @@ __device__ __forceinline__ uint64_t wave_sum64(uint64_t v)
 enum { kFeatRare = 1, kFeatMesh = 2 };
 
 /* The persistent loop of one wavefront over the chunks of one map */
-template<int SENSOR, int SPECTRAL, bool COUNT, int FEAT, bool LIST = false>
-__device__ __forceinline__ void run_scene(const SceneF& S0, const DevMap& M0, const Launch& P)
+/* A pointer to constant memory: the kernel argument segment, or a table in
+ * device memory that no kernel writes while it is read */
+template<class T>
+using KPtr = const __attribute__((address_space(4))) T*;
+
+/* The same address, opaque to the optimiser (an empty asm statement in a
+ * scalar register): loads through it stay where they are written, in the
+ * loop, instead of being hoisted out of it */
+template<class T>
+__device__ __forceinline__ KPtr<T> launder(KPtr<T> p)
 {
-  constexpr bool RARE = (FEAT & kFeatRare) != 0;
-  SceneF S = S0;
-  DevMap M = M0;
+  uint64_t a = (uint64_t)p;
+  asm volatile("" : "+s"(a));
+  return (KPtr<T>)a;
+}
+
+/* A copy of a constant-memory struct (scalar loads; the host compilation
+ * pass, which only needs the declaration, gets an empty value) */
+template<class T>
+__device__ __forceinline__ T kload(KPtr<T> p)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  return *p;
+#else
+  (void)p;
+  return T{};
+#endif
+}
+
+/* The scene and map of a kernel instance: the features it was not compiled
+ * for (FEAT) read as constant zeros, so that their code folds away */
+template<int FEAT>
+__device__ __forceinline__ void fold_features(SceneF& S, DevMap& M)
+{
   /* no mesh: the ground type is NONE or PLANE (0 or 1), so that every test
    * for HTGX_GROUND_MESH (2) folds to false */
   if(!(FEAT & kFeatMesh)) S.ground_type &= 1;
-  if(!RARE) {
+  if(!(FEAT & kFeatRare)) {
     S.repeat = 0;
     S.ground_repeat = 0;
     S.gmat = 0;
@@ -1158,6 +1195,16 @@ __device__ __forceinline__ void run_scene(const SceneF& S0, const DevMap& M0, co
     M.ortho = 0;
     M.lens_radius = 0.0;
   }
+}
+
+template<int SENSOR, int SPECTRAL, bool COUNT, int FEAT, bool LIST = false>
+__device__ __forceinline__ void run_scene(KPtr<SceneF> pS, KPtr<DevMap> pM, KPtr<Launch> pP)
+{
+  constexpr bool RARE = (FEAT & kFeatRare) != 0;
+  SceneF S = kload(pS);
+  DevMap M = kload(pM);
+  const Launch P = kload(pP);
+  fold_features<FEAT>(S, M);
   /* canonicals per event: 8, plus 2 for the sun of a SW flux map */
   constexpr int NP = (SENSOR == HTGX_SENSOR_RECTANGLE && SPECTRAL == HTGX_SPECTRAL_SW) ? 5 : 4;
   Lane L;
@@ -1322,6 +1369,13 @@ __device__ __forceinline__ void run_scene(const SceneF& S0, const DevMap& M0, co
     const uint64_t want = __ballot(L.lstate == LS_EVENT);
     const uint64_t other = __ballot(L.lstate == LS_SEG || L.lstate == LS_COLL);
     if(want && (__popcll(want) >= P.ev_thresh || !other)) {
+      /* the event step reads its constants afresh: the many fields it uses
+       * then hold scalar registers only while it runs, not across the
+       * tracking steps (fewer scalar spills there) */
+      SceneF S = kload(launder(pS));
+      DevMap M = kload(launder(pM));
+      const Launch P = kload(launder(pP));
+      fold_features<FEAT>(S, M);
       if(COUNT) { d_ee++; d_el += __popcll(want); }
       if(L.lstate == LS_EVENT) {
         if(LIST && LC(stage) == ST_FETCH) {
@@ -1487,9 +1541,11 @@ __device__ __forceinline__ void run_scene(const SceneF& S0, const DevMap& M0, co
  * latency than 2 waves without spills (measured 14% faster at 1024^2) */
 template<int SENSOR, int SPECTRAL, bool COUNT, int FEAT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
-wavefront_kernel(SceneF S, DevMap M, Launch P)
+wavefront_kernel(KArgs A)
 {
-  run_scene<SENSOR, SPECTRAL, COUNT, FEAT>(S, M, P);
+  (void)A;
+  const KPtr<KArgs> k = (KPtr<KArgs>)__builtin_amdgcn_kernarg_segment_ptr();
+  run_scene<SENSOR, SPECTRAL, COUNT, FEAT>(&k->S, &k->M, &k->P);
 }
 
 /* Given (pixel, realisation) pairs of one map, one realisation each (the
@@ -1498,9 +1554,11 @@ wavefront_kernel(SceneF S, DevMap M, Launch P)
  * arithmetic as the leaner instances */
 template<int SENSOR, int SPECTRAL>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
-wavefront_list_kernel(SceneF S, DevMap M, Launch P)
+wavefront_list_kernel(KArgs A)
 {
-  run_scene<SENSOR, SPECTRAL, false, kFeatRare | kFeatMesh, true>(S, M, P);
+  (void)A;
+  const KPtr<KArgs> k = (KPtr<KArgs>)__builtin_amdgcn_kernarg_segment_ptr();
+  run_scene<SENSOR, SPECTRAL, false, kFeatRare | kFeatMesh, true>(&k->S, &k->M, &k->P);
 }
 
 /* The FP32 segment iterator on given rays over [0, range[1]] (a debug
@@ -1567,7 +1625,7 @@ wavefront_batch_kernel(const BatchScene* __restrict__ scenes, uint32_t nscenes)
     const unsigned int ex = __hip_atomic_load(B.P.exhausted, __ATOMIC_RELAXED,
                                               __HIP_MEMORY_SCOPE_AGENT);
     if(__builtin_amdgcn_readfirstlane(ex)) continue;
-    run_scene<SENSOR, SPECTRAL, false, FEAT>(B.S, B.M, B.P);
+    run_scene<SENSOR, SPECTRAL, false, FEAT>((KPtr<SceneF>)&B.S, (KPtr<DevMap>)&B.M, (KPtr<Launch>)&B.P);
   }
 }
 
@@ -1682,7 +1740,7 @@ template<class F>
 int launch_variant(F kernel, const SceneF& S, const DevMap& M, const Launch& P, uint32_t grid,
                    hipStream_t stream)
 {
-  kernel<<<grid, 256, 0, stream>>>(S, M, P);
+  kernel<<<grid, 256, 0, stream>>>(KArgs{S, M, P});
   HIP_OK(hipGetLastError());
   return HTGX_OK;
 }
@@ -2561,14 +2619,14 @@ int htgx_debug_realisations_fp32(htgx_ctx* ctx, const htgx_map_desc* desc, const
     const uint32_t grid = std::min(st->grid_max, (m + 255) / 256);
     if(Mc.sensor == HTGX_SENSOR_CAMERA) {
       if(Mc.spectral_type == HTGX_SPECTRAL_LW)
-        wavefront_list_kernel<HTGX_SENSOR_CAMERA, HTGX_SPECTRAL_LW><<<grid, 256>>>(st->S, Mc, P);
+        wavefront_list_kernel<HTGX_SENSOR_CAMERA, HTGX_SPECTRAL_LW><<<grid, 256>>>(KArgs{st->S, Mc, P});
       else
-        wavefront_list_kernel<HTGX_SENSOR_CAMERA, HTGX_SPECTRAL_SW><<<grid, 256>>>(st->S, Mc, P);
+        wavefront_list_kernel<HTGX_SENSOR_CAMERA, HTGX_SPECTRAL_SW><<<grid, 256>>>(KArgs{st->S, Mc, P});
     } else {
       if(Mc.spectral_type == HTGX_SPECTRAL_LW)
-        wavefront_list_kernel<HTGX_SENSOR_RECTANGLE, HTGX_SPECTRAL_LW><<<grid, 256>>>(st->S, Mc, P);
+        wavefront_list_kernel<HTGX_SENSOR_RECTANGLE, HTGX_SPECTRAL_LW><<<grid, 256>>>(KArgs{st->S, Mc, P});
       else
-        wavefront_list_kernel<HTGX_SENSOR_RECTANGLE, HTGX_SPECTRAL_SW><<<grid, 256>>>(st->S, Mc, P);
+        wavefront_list_kernel<HTGX_SENSOR_RECTANGLE, HTGX_SPECTRAL_SW><<<grid, 256>>>(KArgs{st->S, Mc, P});
     }
     HIP_OK(hipGetLastError());
     std::vector<htgx_mc_accum> acc(m);
