@@ -1369,13 +1369,20 @@ __device__ __forceinline__ void run_scene(KPtr<SceneF> pS, KPtr<DevMap> pM, KPtr
     const uint64_t want = __ballot(L.lstate == LS_EVENT);
     const uint64_t other = __ballot(L.lstate == LS_SEG || L.lstate == LS_COLL);
     if(want && (__popcll(want) >= P.ev_thresh || !other)) {
-      /* the event step reads its constants afresh: the many fields it uses
-       * then hold scalar registers only while it runs, not across the
-       * tracking steps (fewer scalar spills there) */
-      SceneF S = kload(launder(pS));
-      DevMap M = kload(launder(pM));
-      const Launch P = kload(launder(pP));
-      fold_features<FEAT>(S, M);
+      /* the shortwave event step reads its constants afresh: the many
+       * fields it uses then hold scalar registers only while it runs, not
+       * across the tracking steps (fewer scalar spills there). The
+       * longwave one, which runs twice as often per realisation, keeps the
+       * values computed before the loop: recomputing them at each event
+       * measured slower there */
+      constexpr bool kReload = SPECTRAL != HTGX_SPECTRAL_LW, kReloadP = kReload;
+      const SceneF& S_h = S;
+      const DevMap& M_h = M;
+      const Launch& P_h = P;
+      SceneF S = kReload ? kload(launder(pS)) : S_h;
+      DevMap M = kReload ? kload(launder(pM)) : M_h;
+      const Launch P = kReloadP ? kload(launder(pP)) : P_h;
+      if(kReload) fold_features<FEAT>(S, M);
       if(COUNT) { d_ee++; d_el += __popcll(want); }
       if(L.lstate == LS_EVENT) {
         if(LIST && LC(stage) == ST_FETCH) {
