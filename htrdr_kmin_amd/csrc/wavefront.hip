@@ -820,10 +820,14 @@ __device__ __forceinline__ float query_tr(const Lane& L)
 }
 
 /* Consume the finished query of the current stage */
-template<bool COUNT>
+template<bool COUNT, int SPECTRAL>
 __device__ __forceinline__ void consume(const SceneF& S, Lane& L, const float* u)
 {
-  switch(LC(stage)) {
+  /* the stages of the other spectral domain never occur: compiled out */
+  constexpr bool kLW = SPECTRAL == HTGX_SPECTRAL_LW;
+  const int stage = LC(stage);
+  if(kLW ? stage != ST_LW : stage == ST_LW) return;
+  switch(stage) {
     case ST_DIRECT:
       LC(w) = LC(L_sun) * query_tr(L);
       sw_after_direct(L);
@@ -1201,6 +1205,7 @@ template<int SENSOR, int SPECTRAL, bool COUNT, int FEAT, bool LIST = false>
 __device__ __forceinline__ void run_scene(KPtr<SceneF> pS, KPtr<DevMap> pM, KPtr<Launch> pP)
 {
   constexpr bool RARE = (FEAT & kFeatRare) != 0;
+  constexpr bool kLW = SPECTRAL == HTGX_SPECTRAL_LW;
   SceneF S = kload(pS);
   DevMap M = kload(pM);
   const Launch P = kload(pP);
@@ -1245,11 +1250,13 @@ __device__ __forceinline__ void run_scene(KPtr<SceneF> pS, KPtr<DevMap> pM, KPtr
     }
     if(run_seg && L.lstate == LS_SEG) {
       float t0;
-      const int sprop = L.mode == MODE_SC ? HTGX_Ks : L.mode == MODE_LW ? HTGX_Kext : L.prop;
-      if(iter_next<COUNT>(S, L, sprop, &t0, L.mode == MODE_TR)) {
+      /* longwave kernels run MODE_LW queries only, shortwave ones MODE_SC
+       * and MODE_TR: the other modes' code is compiled out */
+      const int sprop = kLW ? HTGX_Kext : L.mode == MODE_SC ? HTGX_Ks : L.prop;
+      if(iter_next<COUNT>(S, L, sprop, &t0, !kLW && L.mode == MODE_TR)) {
         const float len = L.s_t1 - t0;
-        if(L.mode == MODE_TR) L.Tmin += (double)len * (double)L.s_kmin;
-        if(L.mode == MODE_SC) {
+        if(!kLW && L.mode == MODE_TR) L.Tmin += (double)len * (double)L.s_kmin;
+        if(!kLW && L.mode == MODE_SC) {
           /* the absorption transmissivity over the segment: its k_min part
            * (taken back beyond a scattering event) and the first tentative
            * collision of its residual tracking */
@@ -1265,7 +1272,7 @@ __device__ __forceinline__ void run_scene(KPtr<SceneF> pS, KPtr<DevMap> pM, KPtr
             L.lstate = LS_COLL;
           }
         }
-        const float maj = L.mode == MODE_TR ? L.s_kmax - L.s_kmin : L.s_kmax;
+        const float maj = !kLW && L.mode == MODE_TR ? L.s_kmax - L.s_kmin : L.s_kmax;
         const float T = len * maj;
         if(L.Ts > T || !(maj > 0.f)) {
           L.Ts -= T; /* no collision in this segment */
@@ -1300,9 +1307,9 @@ __device__ __forceinline__ void run_scene(KPtr<SceneF> pS, KPtr<DevMap> pM, KPtr
     if(L.lstate == LS_COLL) {
       /* ..._sw.c:123: a scattering candidate beyond 1e9 m is dropped, the
        * query moving on to the next segment with the same optical thickness */
-      if(L.mode == MODE_SC && L.tc > 1e9f) L.tc = INFINITY;
+      if(!kLW && L.mode == MODE_SC && L.tc > 1e9f) L.tc = INFINITY;
       /* the nearer of the scattering and the absorption candidates */
-      const bool is_a = L.tc_a < L.tc;
+      const bool is_a = !kLW && L.tc_a < L.tc;
       const float tx = is_a ? L.tc_a : L.tc;
       if(tx < INFINITY) {
         if(COUNT) L.c_coll++;
@@ -1322,14 +1329,14 @@ __device__ __forceinline__ void run_scene(KPtr<SceneF> pS, KPtr<DevMap> pM, KPtr
            * 206-209) */
           const float amin = gka + (L.s_cloud ? L.cka * L.lmn : 0.f);
           L.ev = uu.x < (ka - amin) / L.maj_a ? EV_ABS : EV_NONE;
-        } else if(L.mode == MODE_LW) {
+        } else if(kLW) {
           const float pa = ka * L.inv_maj, ps = ks * L.inv_maj;
           L.ev = uu.x < pa ? EV_ABS : uu.x < pa + ps ? EV_SCA : EV_NONE;
         } else {
           const float k = L.prop == HTGX_Ka ? ka : L.prop == HTGX_Ks ? ks : ka + ks;
           const float proba = (L.mode == MODE_TR ? k - L.s_kmin : k) * L.inv_maj;
           L.ev = uu.x < proba ? EV_HIT : EV_NONE;
-          if(L.mode == MODE_SC && L.ev != EV_NONE) {
+          if(!kLW && L.mode == MODE_SC && L.ev != EV_NONE) {
             /* gas vs particle scattering at the event, ..._sw.c:427-438 */
             LC(gas_ratio) = gks / ks;
             /* the absorption optical depth stops at the event */
@@ -1442,7 +1449,7 @@ __device__ __forceinline__ void run_scene(KPtr<SceneF> pS, KPtr<DevMap> pM, KPtr
         WF_TICK(4)
         if(!begin && L.pending) {
           L.pending = false;
-          consume<COUNT>(S, L, u);
+          consume<COUNT, SPECTRAL>(S, L, u);
         }
         WF_TICK(5)
         while(LC(stage) == ST_WALK_DONE) walk_done<SENSOR, SPECTRAL, COUNT>(S, L);
@@ -1507,7 +1514,7 @@ __device__ __forceinline__ void run_scene(KPtr<SceneF> pS, KPtr<DevMap> pM, KPtr
             L.Ts = exp1f(u[2 * NP - 1]);
             L.Tmin = 0.0;
             L.pending = false;
-            if(L.mode == MODE_SC) L.Ts_a = exp1f(draw2(L, key).x);
+            if(!kLW && L.mode == MODE_SC) L.Ts_a = exp1f(draw2(L, key).x);
             iter_init(S, L);
           }
           L.tc = L.tc_a = INFINITY;
