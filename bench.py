@@ -324,7 +324,7 @@ def main():
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:
-        sys.exit("bench.py runs on one GPU only: this build has no multi-GPU path "
+        sys.exit("bench.py runs on one GPU only: the library has no multi-GPU path "
                  "(pixels would shard across GPUs by Morton tiles); use --gpus 1")
 
     import numpy as np

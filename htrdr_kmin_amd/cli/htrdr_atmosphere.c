@@ -9,7 +9,7 @@
  * and the sky material of -n, read by htgx_ground_files_load
  * (include/htgx_files.h). The atmosphere inputs, the htgop (-a), htcp (-c)
  * and htmie (-m) files, are read by htsky in the reference; no reader for
- * these formats exists in this build, so those options take a synthetic
+ * these formats exists in this library, so those options take a synthetic
  * source instead of a path:
  *   -a synth:<config>[,<seed>]   gas layers and spectral data of a seeded
  *                                benchmark configuration (0, 1, 2, 4)

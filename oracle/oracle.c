@@ -1,7 +1,7 @@
 /*
  * CPU FP64 oracle of the htrdr-atmosphere null-collision core.
  *
- * TEST INFRASTRUCTURE ONLY: the checker for libhtgx.so. It is never linked
+ * TEST INFRASTRUCTURE ONLY: the CPU reference of libhtgx.so's tests. It is never linked
  * into, loaded by or called from the product; only tests/,
  * __graft_entry__.smoke() and bench.py's CPU-baseline leg use it.
  *

@@ -1,14 +1,14 @@
 /*
  * CPU FP64 oracle of the htrdr-atmosphere null-collision core.
  *
- * TEST INFRASTRUCTURE ONLY. This library is the checker the GPU library
- * (libhtgx.so) is compared against; nothing in the product links, loads or
+ * TEST INFRASTRUCTURE ONLY. This library is the CPU reference the GPU
+ * library (libhtgx.so) is compared against in the tests; nothing in the product links, loads or
  * calls it. Only tests/, __graft_entry__.smoke() and bench.py's CPU baseline
  * leg may use it.
  *
  * It restates, in plain C with recursion and no GPU-oriented layout, the
  * algorithms of the reference path (citations are file:line under the htrdr
- * tree) and the semantics this build fixes for the un-vendored libraries
+ * tree) and the semantics the GPU library fixes for the un-vendored libraries
  * (htsky property model and octree, svx traversal, star-sp samplers, star-sf
  * phase functions, star-camera pinhole); see oracle.c for each function.
  *

@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (oracle/_build/liboracle.so).
 
-Test infrastructure only: the oracle is the checker the GPU library is
-compared against.
+Test infrastructure only: the oracle is the CPU reference the GPU library
+is compared against.
 """
 import ctypes as C
 import os
