@@ -283,11 +283,13 @@ HTGX_HD double cie_band_sample(int channel, double lmin, double len, double r1)
 /* htrdr_ran_wlen_planck_sample, src/core/htrdr_ran_wlen_planck.c:359-376,
  * and htrdr_ran_wlen_cie_xyz_sample_X/Y/Z, htrdr_ran_wlen_cie_xyz.c:346-397.
  * The pdf is in nm^-1; for a CIE channel it is the constant 1 / integral of
- * the channel, as the reference returns it. */
-template<bool CIE = true>
+ * the channel, as the reference returns it. Without RARE, the modes that few
+ * maps use (3, CIE channels, and 2, Planck over a continuous range) are
+ * compiled out: the caller guarantees the mode is 0 or 1. */
+template<bool RARE = true>
 HTGX_D double sample_wavelength(const DevMap& M, double r0, double r1, double* pdf)
 {
-  if(CIE && M.wmode == 3) {
+  if(RARE && M.wmode == 3) {
     uint32_t lo = 0, hi = M.nwbands - 1;
     while(lo < hi) {
       const uint32_t mid = (lo + hi) >> 1;
@@ -304,7 +306,7 @@ HTGX_D double sample_wavelength(const DevMap& M, double r0, double r1, double* p
     }
     *pdf = M.wpdf[lo] * (1.0 / M.band_len);
     return M.range[0] + (double)lo * M.band_len + M.band_len * r1;
-  } else if(M.wmode == 0) {
+  } else if(!RARE || M.wmode == 0) {
     *pdf = 1.0;
     return M.range[0];
   } else {

@@ -723,19 +723,57 @@ __device__ __forceinline__ bool iter_next(const SceneF& S, Lane& L, int prop, fl
   }
 }
 
+/* What the collision step reads of the scene at every tentative collision
+ * (the cloud grid's addressing, the random-stream key, the collision
+ * budget). With VREG they are held in vector registers for the whole launch
+ * (the empty asm statements make them lane values): in scalar registers
+ * they stay live across the whole loop, where the kernel runs out of scalar
+ * registers, and are reloaded from their spill slots at every step. The
+ * kernels with the rare features have no vector registers to spare and keep
+ * them in scalar ones. */
+struct CollRegs {
+  float lower[3], inv_cs[3];
+  uint32_t def0, def1, B, key, max_steps;
+  const float* lwc;
+};
+
+template<bool VREG>
+__device__ __forceinline__ CollRegs coll_regs(const SceneF& S, uint32_t key, uint32_t max_steps)
+{
+  CollRegs g;
+#pragma unroll
+  for(int a = 0; a < 3; ++a) {
+    g.lower[a] = S.lower[a];
+    g.inv_cs[a] = S.inv_cs[a];
+  }
+  g.def0 = S.def[0];
+  g.def1 = S.def[1];
+  g.B = S.B;
+  g.key = key;
+  g.max_steps = max_steps;
+  g.lwc = S.lwc;
+  if(VREG) {
+    asm volatile("" : "+v"(g.lower[0]), "+v"(g.lower[1]), "+v"(g.lower[2]), "+v"(g.inv_cs[0]),
+                      "+v"(g.inv_cs[1]), "+v"(g.inv_cs[2]));
+    asm volatile("" : "+v"(g.def0), "+v"(g.def1), "+v"(g.B), "+v"(g.key), "+v"(g.max_steps),
+                      "+v"(g.lwc));
+  }
+  return g;
+}
+
 /* LWC of the cell holding x, clamped into the current segment's leaf */
 template<bool COUNT>
-__device__ __forceinline__ float seg_lwcf(const SceneF& S, Lane& L, const float x[3])
+__device__ __forceinline__ float seg_lwcf(const CollRegs& G, Lane& L, const float x[3])
 {
   uint32_t ci[3];
 #pragma unroll
   for(int a = 0; a < 3; ++a) {
-    const uint32_t lo = (L.lohi[a] & 0xFFFFu) * S.B, hi = (L.lohi[a] >> 16) * S.B - 1;
-    const float f = floorf((x[a] - S.lower[a]) * S.inv_cs[a]);
+    const uint32_t lo = (L.lohi[a] & 0xFFFFu) * G.B, hi = (L.lohi[a] >> 16) * G.B - 1;
+    const float f = floorf((x[a] - G.lower[a]) * G.inv_cs[a]);
     ci[a] = f < (float)lo ? lo : f > (float)hi ? hi : (uint32_t)f;
   }
   if(COUNT) L.c_lwc++;
-  return S.lwc[(ci[2] * S.def[1] + ci[1]) * S.def[0] + ci[0]];
+  return G.lwc[(ci[2] * G.def1 + ci[1]) * G.def0 + ci[0]];
 }
 
 /*******************************************************************************
@@ -1147,10 +1185,11 @@ __device__ __forceinline__ uint64_t wave_sum64(uint64_t v)
 
 /* Kernel features: without kFeatRare, the features that few maps use (cloud
  * and ground repetition, spectral BRDFs and interfaces, orthographic and
- * thin-lens cameras, CIE XYZ wavelength sampling) are compiled out; without
- * kFeatMesh, the mesh ground (its BVH casts, its deferred casts). Their flags
- * become constants, so that their code and registers do not weigh on the
- * kernels of the maps that do without them. */
+ * thin-lens cameras, CIE XYZ wavelength sampling, Planck wavelength sampling
+ * over a continuous range) are compiled out; without kFeatMesh, the mesh
+ * ground (its BVH casts, its deferred casts). Their flags become constants,
+ * so that their code and registers do not weigh on the kernels of the maps
+ * that do without them. */
 enum { kFeatRare = 1, kFeatMesh = 2 };
 
 /* The persistent loop of one wavefront over the chunks of one map */
@@ -1224,7 +1263,9 @@ __device__ __forceinline__ void run_scene(KPtr<SceneF> pS, KPtr<DevMap> pM, KPtr
   LC(tsum) = LC(tsum2) = 0.f;
   L.pixel = L.r = 0;
   const uint32_t xcd = blockIdx.x % (uint32_t)kXcds;
-  const uint32_t key = (uint32_t)M.seed ^ (uint32_t)(M.seed >> 32);
+  const CollRegs G =
+      coll_regs<!RARE>(S, (uint32_t)M.seed ^ (uint32_t)(M.seed >> 32), P.max_steps);
+  const uint32_t key = G.key;
   uint64_t d_it = 0, d_ce = 0, d_cl = 0, d_se = 0, d_sl = 0, d_ee = 0, d_el = 0;
   /* shader clock cycles per phase (counting build only) */
   uint64_t cy[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -1316,7 +1357,7 @@ __device__ __forceinline__ void run_scene(KPtr<SceneF> pS, KPtr<DevMap> pM, KPtr
         float x[3];
 #pragma unroll
         for(int a = 0; a < 3; ++a) x[a] = L.pos[a] + tx * L.dir[a];
-        const float v = L.s_cloud ? seg_lwcf<COUNT>(S, L, x) : 0.f;
+        const float v = L.s_cloud ? seg_lwcf<COUNT>(G, L, x) : 0.f;
         /* the draw does not depend on the LWC: written before the LWC's
          * first use, its Threefry rounds run while the load is in flight */
         const float2 uu = draw2(L, key);
@@ -1349,7 +1390,7 @@ __device__ __forceinline__ void run_scene(KPtr<SceneF> pS, KPtr<DevMap> pM, KPtr
           LC(ev_layer) = L.s_layer;
           L.pending = true;
           L.lstate = LS_EVENT;
-        } else if((++L.nsteps & kCollMask) > P.max_steps) {
+        } else if((++L.nsteps & kCollMask) > G.max_steps) {
           /* tentative-collision budget of a query exhausted: end it without
            * event, and count it */
           if(P.trunc) atomicAdd(P.trunc, 1ull);
@@ -1763,7 +1804,7 @@ int launch_variant(F kernel, const SceneF& S, const DevMap& M, const Launch& P, 
 int features(const SceneF& S, const DevMap& M)
 {
   const bool rare = S.repeat || S.ground_repeat || S.gmat || M.ortho || M.lens_radius > 0 ||
-                    M.wmode == 3;
+                    M.wmode == 3 || M.wmode == 2;
   return (rare ? kFeatRare : 0) | (S.ground_type == HTGX_GROUND_MESH ? kFeatMesh : 0);
 }
 
