@@ -447,6 +447,48 @@ __device__ __forceinline__ void face_forwardf(float N[3], const float dir[3])
   }
 }
 
+/* What the tracking steps read of the scene at every step (the cloud grid's
+ * addressing, the leaf records, the random-stream key, the collision
+ * budget). With VREG they are held in vector registers for the whole launch
+ * (the empty asm statements make them lane values): in scalar registers
+ * they stay live across the whole loop, where the kernel runs out of scalar
+ * registers, and are reloaded from their spill slots at every step. The
+ * kernels with the rare features have no vector registers to spare and keep
+ * them in scalar ones. */
+struct TrackRegs {
+  float lower[3], inv_cs[3];
+  uint32_t def0, def1, B, key, max_steps;
+  const float* lwc;
+  const uint2* leafrec; /* the leaf records and their LWC scale */
+  float lwc_scale;
+};
+
+template<bool VREG>
+__device__ __forceinline__ TrackRegs track_regs(const SceneF& S, uint32_t key, uint32_t max_steps)
+{
+  TrackRegs g;
+#pragma unroll
+  for(int a = 0; a < 3; ++a) {
+    g.lower[a] = S.lower[a];
+    g.inv_cs[a] = S.inv_cs[a];
+  }
+  g.def0 = S.def[0];
+  g.def1 = S.def[1];
+  g.B = S.B;
+  g.key = key;
+  g.max_steps = max_steps;
+  g.lwc = S.lwc;
+  g.leafrec = S.leafrec;
+  g.lwc_scale = S.lwc_scale;
+  if(VREG) {
+    asm volatile("" : "+v"(g.lower[0]), "+v"(g.lower[1]), "+v"(g.lower[2]), "+v"(g.inv_cs[0]),
+                      "+v"(g.inv_cs[1]), "+v"(g.inv_cs[2]));
+    asm volatile("" : "+v"(g.def0), "+v"(g.def1), "+v"(g.B), "+v"(g.key), "+v"(g.max_steps),
+                      "+v"(g.lwc), "+v"(g.leafrec), "+v"(g.lwc_scale));
+  }
+  return g;
+}
+
 /*******************************************************************************
  * Segment iterator (FP32 version of SegIter in htgx_device.h)
  ******************************************************************************/
@@ -461,14 +503,14 @@ __device__ __forceinline__ float planef(const SceneF& S, int a, uint32_t k)
 }
 
 template<bool COUNT>
-__device__ __forceinline__ void load_leaf(const SceneF& S, Lane& L, uint32_t cx, uint32_t cy,
-                                          uint32_t cz)
+__device__ __forceinline__ void load_leaf(const SceneF& S, const TrackRegs& G, Lane& L, uint32_t cx,
+                                          uint32_t cy, uint32_t cz)
 {
   /* one 8-byte load: the leaf's level and its LWC bounds */
-  const uint2 rec = S.leafrec[(cz * S.dims0[1] + cy) * S.dims0[0] + cx];
+  const uint2 rec = G.leafrec[(cz * S.dims0[1] + cy) * S.dims0[0] + cx];
   const uint32_t lvl = rec.x >> kLeafIdBits;
-  L.lmn = half_bits_to_float(rec.y & 0xFFFFu) * S.lwc_scale;
-  L.lmx = half_bits_to_float(rec.y >> 16) * S.lwc_scale;
+  L.lmn = half_bits_to_float(rec.y & 0xFFFFu) * G.lwc_scale;
+  L.lmx = half_bits_to_float(rec.y >> 16) * G.lwc_scale;
   const uint32_t c[3] = {cx, cy, cz};
   float best = INFINITY;
   int ax = 0;
@@ -633,8 +675,8 @@ __device__ __forceinline__ float gas_depth(const SceneF& S, const Lane& L, int p
  * the leaf range used to clamp the cells of its collisions stays the one of
  * the segment. Each pass of the loop loads at most one leaf, at one site. */
 template<bool COUNT>
-__device__ __forceinline__ bool iter_next(const SceneF& S, Lane& L, int prop, float* t0,
-                                          bool runs = false)
+__device__ __forceinline__ bool iter_next(const SceneF& S, const TrackRegs& G, Lane& L, int prop,
+                                          float* t0, bool runs = false)
 {
   if(L.adv) {
     L.adv = false;
@@ -645,7 +687,7 @@ __device__ __forceinline__ bool iter_next(const SceneF& S, Lane& L, int prop, fl
     if(L.itdone || !(L.tit < tmax)) return false;
     if(L.cstate >= 3) {
       uint32_t c[3];
-      if(next_cell(S, L, c)) { load_leaf<COUNT>(S, L, c[0], c[1], c[2]); L.cstate = 1; }
+      if(next_cell(S, L, c)) { load_leaf<COUNT>(S, G, L, c[0], c[1], c[2]); L.cstate = 1; }
       else L.cstate = 2;
     }
     const float t = L.tit;
@@ -723,47 +765,9 @@ __device__ __forceinline__ bool iter_next(const SceneF& S, Lane& L, int prop, fl
   }
 }
 
-/* What the collision step reads of the scene at every tentative collision
- * (the cloud grid's addressing, the random-stream key, the collision
- * budget). With VREG they are held in vector registers for the whole launch
- * (the empty asm statements make them lane values): in scalar registers
- * they stay live across the whole loop, where the kernel runs out of scalar
- * registers, and are reloaded from their spill slots at every step. The
- * kernels with the rare features have no vector registers to spare and keep
- * them in scalar ones. */
-struct CollRegs {
-  float lower[3], inv_cs[3];
-  uint32_t def0, def1, B, key, max_steps;
-  const float* lwc;
-};
-
-template<bool VREG>
-__device__ __forceinline__ CollRegs coll_regs(const SceneF& S, uint32_t key, uint32_t max_steps)
-{
-  CollRegs g;
-#pragma unroll
-  for(int a = 0; a < 3; ++a) {
-    g.lower[a] = S.lower[a];
-    g.inv_cs[a] = S.inv_cs[a];
-  }
-  g.def0 = S.def[0];
-  g.def1 = S.def[1];
-  g.B = S.B;
-  g.key = key;
-  g.max_steps = max_steps;
-  g.lwc = S.lwc;
-  if(VREG) {
-    asm volatile("" : "+v"(g.lower[0]), "+v"(g.lower[1]), "+v"(g.lower[2]), "+v"(g.inv_cs[0]),
-                      "+v"(g.inv_cs[1]), "+v"(g.inv_cs[2]));
-    asm volatile("" : "+v"(g.def0), "+v"(g.def1), "+v"(g.B), "+v"(g.key), "+v"(g.max_steps),
-                      "+v"(g.lwc));
-  }
-  return g;
-}
-
 /* LWC of the cell holding x, clamped into the current segment's leaf */
 template<bool COUNT>
-__device__ __forceinline__ float seg_lwcf(const CollRegs& G, Lane& L, const float x[3])
+__device__ __forceinline__ float seg_lwcf(const TrackRegs& G, Lane& L, const float x[3])
 {
   uint32_t ci[3];
 #pragma unroll
@@ -1263,8 +1267,8 @@ __device__ __forceinline__ void run_scene(KPtr<SceneF> pS, KPtr<DevMap> pM, KPtr
   LC(tsum) = LC(tsum2) = 0.f;
   L.pixel = L.r = 0;
   const uint32_t xcd = blockIdx.x % (uint32_t)kXcds;
-  const CollRegs G =
-      coll_regs<!RARE>(S, (uint32_t)M.seed ^ (uint32_t)(M.seed >> 32), P.max_steps);
+  const TrackRegs G =
+      track_regs<!RARE>(S, (uint32_t)M.seed ^ (uint32_t)(M.seed >> 32), P.max_steps);
   const uint32_t key = G.key;
   uint64_t d_it = 0, d_ce = 0, d_cl = 0, d_se = 0, d_sl = 0, d_ee = 0, d_el = 0;
   /* shader clock cycles per phase (counting build only) */
@@ -1294,7 +1298,7 @@ __device__ __forceinline__ void run_scene(KPtr<SceneF> pS, KPtr<DevMap> pM, KPtr
       /* longwave kernels run MODE_LW queries only, shortwave ones MODE_SC
        * and MODE_TR: the other modes' code is compiled out */
       const int sprop = kLW ? HTGX_Kext : L.mode == MODE_SC ? HTGX_Ks : L.prop;
-      if(iter_next<COUNT>(S, L, sprop, &t0, !kLW && L.mode == MODE_TR)) {
+      if(iter_next<COUNT>(S, G, L, sprop, &t0, !kLW && L.mode == MODE_TR)) {
         const float len = L.s_t1 - t0;
         if(!kLW && L.mode == MODE_TR) L.Tmin += (double)len * (double)L.s_kmin;
         if(!kLW && L.mode == MODE_SC) {
@@ -1638,7 +1642,8 @@ wf_trace_segments_kernel(SceneF S, const double* rays, uint32_t nrays, int prop,
   iter_init(S, L);
   uint32_t n = 0;
   float t0;
-  while(iter_next<false>(S, L, prop, &t0)) {
+  const TrackRegs G = track_regs<false>(S, 0u, 0u);
+  while(iter_next<false>(S, G, L, prop, &t0)) {
     if(n < max_segs) {
       htgx_segment& o = segs[(size_t)i * max_segs + n];
       o.t0 = t0;
