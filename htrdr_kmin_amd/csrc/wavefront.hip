@@ -2505,11 +2505,14 @@ struct TrialClock {
 };
 
 /* Coordinate descent over the chunk size, then the event threshold, then the
- * segment threshold, then the grid. trial(p, &ms) runs one timed launch with parameters p; each
- * candidate is timed twice and keeps its faster time (single launches vary
- * by 1-2%), and replaces the current parameters only when more than 1%
- * faster. The first launch is an untimed warm-up; the search stops before a
- * trial that would exceed the budget, estimated from the last one. */
+ * segment threshold, then the grid. The two thresholds interact: when the
+ * segment pass moves, the event threshold is searched again, and then the
+ * segment threshold if that moved. trial(p, &ms) runs one timed launch with
+ * parameters p; each candidate is timed twice and keeps its faster time
+ * (single launches vary by 1-2%), and replaces the current parameters only
+ * when more than 1% faster. The first launch is an untimed warm-up; the
+ * search stops before a trial that would exceed the budget, estimated from
+ * the last one. */
 template<class Trial>
 static int tune_search(Trial&& trial, double budget_ms, uint32_t grid_max, htgx_wf_params* cur,
                        double* cur_ms, uint32_t* ntrials)
@@ -2527,7 +2530,14 @@ static int tune_search(Trial&& trial, double budget_ms, uint32_t grid_max, htgx_
   double spent = 2 * ms, last = 2 * ms;
   uint32_t n = 1;
   *cur_ms = ms;
-  for(int dim = 0; dim < 4; ++dim) {
+  /* passes: chunk, event threshold, segment threshold, the event and
+   * segment thresholds again (each only if the pass before it moved), grid */
+  static const int order[] = {0, 1, 2, 1, 2, 3};
+  bool moved = false, stop = false;
+  for(int k = 0; k < 6 && !stop; ++k) {
+    const int dim = order[k];
+    if((k == 3 || k == 4) && !moved) continue;
+    moved = false;
     const uint32_t* vals = dim == 0 ? chunks : dim == 1 ? thresholds : dim == 2 ? seg_thresholds : grids;
     const int nv = dim == 1 ? 6 : 3;
     const htgx_wf_params base = *cur;
@@ -2540,14 +2550,14 @@ static int tune_search(Trial&& trial, double budget_ms, uint32_t grid_max, htgx_
       if(p.ev_thresh == base.ev_thresh && p.chunk == base.chunk && p.grid == base.grid &&
          p.seg_thresh == base.seg_thresh)
         continue;
-      if(spent + last > budget_ms) { dim = 4; break; }
+      if(spent + last > budget_ms) { stop = true; break; }
       if((res = trial(p, &ms)) != HTGX_OK) return res;
       if((res = trial(p, &ms2)) != HTGX_OK) return res;
       spent += ms + ms2;
       last = ms + ms2;
       ms = std::min(ms, ms2);
       ++n;
-      if(ms < 0.99 * *cur_ms) { *cur = p; *cur_ms = ms; }
+      if(ms < 0.99 * *cur_ms) { *cur = p; *cur_ms = ms; moved = true; }
     }
   }
   *ntrials = n;

@@ -430,8 +430,8 @@ int htgx_set_wavefront_params(struct htgx_ctx* ctx, const struct htgx_wf_params*
 /* Autotuner of the parameters above for a map: trial launches of the map as
  * described (desc->spp realisations per pixel: pass the launch size that
  * will be used), timed on the device, by coordinate descent (chunk size,
- * then event threshold, then segment threshold, then grid) within about
- * budget_ms of device time;
+ * then event threshold, then segment threshold, then both thresholds again
+ * while they move, then grid) within about budget_ms of device time;
  * each candidate is timed twice, and must be more than 1% faster to be
  * taken. The best
  * parameters are kept by the context for the maps that follow; best, rate
