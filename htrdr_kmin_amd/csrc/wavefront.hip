@@ -454,7 +454,8 @@ __device__ __forceinline__ void face_forwardf(float N[3], const float dir[3])
  * they stay live across the whole loop, where the kernel runs out of scalar
  * registers, and are reloaded from their spill slots at every step. The
  * kernels with the rare features have no vector registers to spare and keep
- * them in scalar ones. */
+ * them in scalar ones. The mesh kernels keep the copy: the shortwave flux
+ * one then spills 8 bytes per lane and still runs faster. */
 struct TrackRegs {
   float lower[3], inv_cs[3];
   uint32_t def0, def1, B, key, max_steps;
