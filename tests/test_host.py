@@ -198,3 +198,16 @@ def test_synth_presets():
     assert lo[0] == 4000.0 and hi[-1] == 20000.0 and np.all(lo[1:] == hi[:-1])
     d = A.map_preset(1)
     assert (d.width, d.height, d.spp) == (1024, 1024, 256)
+
+
+@pytest.mark.parametrize("argv", [["--gpus", "2"], ["--gpus", "8", "--config", "3"]])
+def test_bench_refuses_several_gpus(argv):
+    """bench.py measures one GPU: with --gpus above 1 it exits with a
+    message saying so, before it touches a device."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py")] + argv, cwd=ROOT,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0
+    assert "one GPU" in r.stderr
+    assert r.stdout == ""
